@@ -1,0 +1,116 @@
+"""Seeded synthetic scenes of the benchmark workloads (entity specs + per-env initial states + control schedules).
+
+* arm:       9-DoF Panda arm (7 revolute + 2 prismatic fingers, 11 links, capsule/box collision) on a plane,
+             self-collision off, PD targets uniform within joint limits, resampled every 100 steps.
+* pile:      free boxes (edges U[0.05, 0.2] m, density 1000 kg/m^3, friction 1.0) dropped in a 1 m column onto a plane.
+* quadruped: go2 floating base + 4 legs x 3 revolute joints (18 DoF, 13 bodies) on a plane,
+             PD targets = standing pose + U(-0.3, 0.3) per dof per step.
+"""
+from __future__ import annotations
+
+import copy
+import json
+import os
+from dataclasses import dataclass
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from .loaders import entity_from_json
+from .model import GEOM_BOX, GEOM_PLANE, JOINT_FIXED, JOINT_FREE, CompileOptions, EntitySpec, GeomSpec, JointSpec, LinkSpec
+
+ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "assets")
+
+GO2_STANDING = np.array([0.0, 0.0, 0.0, 0.0, 0.8, 0.8, 1.0, 1.0, -1.5, -1.5, -1.5, -1.5])
+GO2_INIT_BASE = np.array([0.0, 0.0, 0.42, 1.0, 0.0, 0.0, 0.0])
+
+
+def plane_entity(normal=(0.0, 0.0, 1.0)) -> EntitySpec:
+    """Plane primitive: fixed link, one PLANE geom (rigid_entity.py:155-242)."""
+    link = LinkSpec("plane_baselink", -1, np.zeros(3), np.array([1.0, 0, 0, 0]),
+                    [JointSpec("plane_baselink_joint", JOINT_FIXED)], [GeomSpec(GEOM_PLANE, data=tuple(normal))])
+    return EntitySpec([link], name="plane")
+
+
+def box_entity(size, pos, quat=(1.0, 0, 0, 0), fixed=False, rho=200.0, friction=None) -> EntitySpec:
+    """Box primitive: one link, FREE joint with init qpos = (pos, quat) unless fixed (rigid_entity.py:155-242)."""
+    pos = np.asarray(pos, np.float64)
+    quat = np.asarray(quat, np.float64)
+    j = JointSpec("box_baselink_joint", JOINT_FIXED) if fixed else JointSpec(
+        "box_baselink_joint", JOINT_FREE, kp=0.0, kv=0.0, init_qpos=np.concatenate([pos, quat]))
+    link = LinkSpec("box_baselink", -1, pos, quat, [j], [GeomSpec(GEOM_BOX, data=tuple(size))])
+    return EntitySpec([link], rho=rho, friction=friction, name="box")
+
+
+def load_asset(name: str) -> EntitySpec:
+    with open(os.path.join(ASSETS, f"{name}.json")) as f:
+        return entity_from_json(json.load(f))
+
+
+def panda_entity() -> EntitySpec:
+    return load_asset("panda")
+
+
+def go2_entity() -> EntitySpec:
+    ent = load_asset("go2")
+    ent = copy.deepcopy(ent)
+    # initial base pose (0, 0, 0.42) and the standing joint configuration
+    for l in ent.links:
+        for j in l.joints:
+            if j.type == JOINT_FREE:
+                j.init_qpos = GO2_INIT_BASE.copy()
+    return ent
+
+
+def _uniform_quats(rng: np.random.Generator, n):
+    u = rng.random((n, 3))
+    q = np.stack([np.sqrt(1 - u[:, 0]) * np.sin(2 * np.pi * u[:, 1]), np.sqrt(1 - u[:, 0]) * np.cos(2 * np.pi * u[:, 1]),
+                  np.sqrt(u[:, 0]) * np.sin(2 * np.pi * u[:, 2]), np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], 1)
+    return q[:, [3, 0, 1, 2]]
+
+
+@dataclass
+class Workload:
+    name: str
+    entities: List[EntitySpec]
+    options: CompileOptions
+    n_envs: int
+    n_steps: int
+    init_qpos: Optional[Callable] = None  # (n_envs) -> (n_envs, n_qs) float64, or None for qpos0
+    control: Optional[Callable] = None  # (step, state) -> None, applies PD targets
+
+
+def arm_workload(n_envs=64, n_steps=1000, seed=0) -> Workload:
+    opts = CompileOptions(dt=0.01, enable_self_collision=False)
+    return Workload("arm", [plane_entity(), panda_entity()], opts, n_envs, n_steps)
+
+
+def quadruped_workload(n_envs=32768, n_steps=1000, seed=2) -> Workload:
+    opts = CompileOptions(dt=0.005)
+    return Workload("quadruped", [plane_entity(), go2_entity()], opts, n_envs, n_steps)
+
+
+def pile_entities(n_boxes: int, rng: np.random.Generator, rho=1000.0, friction=1.0):
+    ents = [plane_entity()]
+    sizes = rng.uniform(0.05, 0.2, (n_boxes, 3))
+    for i in range(n_boxes):
+        ents.append(box_entity(sizes[i], (0.0, 0.0, 0.5 + i), rho=rho, friction=friction))
+    return ents
+
+
+def pile_workload(n_boxes=64, n_envs=4096, n_steps=500, seed=1, max_collision_pairs=1000) -> Workload:
+    """Box sizes are shared by every env of one scene type; poses differ per env (seeded)."""
+    rng = np.random.default_rng(seed)
+    opts = CompileOptions(dt=0.01, max_collision_pairs=max_collision_pairs)
+    ents = pile_entities(n_boxes, rng)
+
+    def init_qpos(n, n_qs):
+        r = np.random.default_rng(seed + 1)
+        q = np.zeros((n, n_qs))
+        pos = np.stack([r.uniform(-0.5, 0.5, (n, n_boxes)), r.uniform(-0.5, 0.5, (n, n_boxes)),
+                        r.uniform(0.1, 1.1, (n, n_boxes))], -1)
+        quat = _uniform_quats(r, n * n_boxes).reshape(n, n_boxes, 4)
+        q[:, :] = np.concatenate([pos, quat], -1).reshape(n, n_boxes * 7)
+        return q
+
+    return Workload("pile", ents, opts, n_envs, n_steps, init_qpos=init_qpos)
