@@ -1,0 +1,92 @@
+"""ctypes binding of libgenesis_rigid.so (the C-ABI declared in include/genesis_rigid.h).
+
+The ctypes mirrors of gsr_model / gsr_state are generated from the header itself, so the field order cannot drift
+from the C declaration; the library's own sizeof() is checked at load time.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HEADER = os.path.join(ROOT, "include", "genesis_rigid.h")
+LIB_PATH = os.path.join(HERE, "_build", "libgenesis_rigid.so")
+
+EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_step", "gsr_forward_kinematics",
+           "gsr_reset_envs", "gsr_get_errno", "gsr_run_stage"]
+
+_CTYPE = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "float": ctypes.c_float}
+
+
+def _parse_struct(name: str):
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    m = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), src, flags=re.S)
+    body = m.group(1)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        decl = decl.replace("const ", "")
+        base, rest = decl.split(None, 1)
+        for var in rest.split(","):
+            var = var.strip()
+            if var.startswith("*"):
+                fields.append((var[1:].strip(), ctypes.c_void_p))
+            elif "[" in var:
+                vname, n = re.match(r"(\w+)\[(\d+)\]", var).groups()
+                fields.append((vname, _CTYPE[base] * int(n)))
+            else:
+                fields.append((var, _CTYPE[base]))
+    return fields
+
+
+class GsrModel(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_model")
+
+
+class GsrState(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_state")
+
+
+MODEL_POINTERS = [n for n, t in GsrModel._fields_ if t is ctypes.c_void_p]
+STATE_POINTERS = [n for n, t in GsrState._fields_ if t is ctypes.c_void_p]
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library. Raises LibraryMissing if it has not been built: there is no fallback path."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LibraryMissing(f"{path} not found; run __graft_entry__.build() (hipcc --offload-arch=gfx950) first")
+    L = ctypes.CDLL(path)
+    for fn in EXPORTS:
+        getattr(L, fn)  # raises AttributeError if a declared symbol is missing
+    L.gsr_abi_version.restype = ctypes.c_int
+    L.gsr_model_size.restype = ctypes.c_int
+    L.gsr_state_size.restype = ctypes.c_int
+    P = ctypes.POINTER
+    L.gsr_step.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
+    L.gsr_forward_kinematics.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+    L.gsr_reset_envs.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p]
+    L.gsr_get_errno.argtypes = [P(GsrState), ctypes.c_void_p, ctypes.c_void_p]
+    L.gsr_run_stage.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
+    if L.gsr_model_size() != ctypes.sizeof(GsrModel) or L.gsr_state_size() != ctypes.sizeof(GsrState):
+        raise RuntimeError("gsr_model/gsr_state layout mismatch between include/genesis_rigid.h and the built library")
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")

@@ -1,0 +1,312 @@
+"""Host side of the MI355X rigid step: model tables and per-env state in PyTorch tensors, steps via the C-ABI.
+
+Mirrors the parts of the reference's RigidSolver (genesis/engine/solvers/rigid/rigid_solver_decomp.py) that sit on
+the batched step path: build (:190-352), substep (:865-933), check_errno (:935-956), control_dofs_* (:2123-2201),
+get_dofs_* (:2366-2376), set_qpos (:1801-1857), set_dofs_velocity, set_state/get_state (:1442-1512). State tensors
+are stored env-minor, shape (n, B), and exposed as zero-copy (B, n) views like ti_to_torch(transpose=True)
+(genesis/utils/misc.py:615-659).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .model import CTRL_FORCE, CTRL_POSITION, CTRL_VELOCITY, RigidModel
+
+RATE_CHECK_ERRNO = 10  # genesis/engine/simulator.py:47
+
+GSR_RESET_COLLIDER = 1
+GSR_RESET_WARMSTART = 2
+
+ERRNO_MESSAGES = [
+    (1, "Exceeding max number of broad phase candidate contact pairs. Increase 'multiplier_collision_broad_phase'."),
+    (2, "Exceeding max number of contact pairs. Increase 'max_collision_pairs'."),
+    (4, "Invalid constraint forces causing 'nan'. Please decrease Rigid simulation timestep."),
+    (8, "Invalid accelerations causing 'nan'. Please decrease Rigid simulation timestep."),
+]
+
+
+class GenesisException(Exception):
+    pass
+
+
+MODEL_TABLES_I = [
+    "link_parent", "link_root", "link_q_start", "link_q_end", "link_dof_start", "link_dof_end", "link_joint_start",
+    "link_joint_end", "link_is_fixed", "joint_type", "joint_q_start", "joint_dof_start", "joint_dof_end", "dof_entity",
+    "ent_link_start", "ent_link_end", "ent_dof_start", "ent_dof_end", "ent_mass_offset", "geom_type", "geom_link",
+    "geom_vert_start", "geom_vert_end", "eq_type", "eq_obj1", "eq_obj2", "pair_idx",
+]
+MODEL_TABLES_F = [
+    "link_pos", "link_quat", "link_inertial_pos", "link_inertial_quat", "link_inertial_i", "link_inertial_mass",
+    "link_invweight", "joint_pos", "joint_sol_params", "dof_motion_ang", "dof_motion_vel", "dof_limit",
+    "dof_force_range", "dof_kp", "dof_kv", "dof_damping", "dof_armature", "dof_stiffness", "dof_frictionloss",
+    "dof_invweight", "qpos0", "ent_gravity_compensation", "geom_pos", "geom_quat", "geom_data", "geom_friction",
+    "geom_sol_params", "geom_center", "geom_init_aabb", "verts", "eq_data", "eq_sol_params", "mass_parent_mask",
+]
+
+INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", "n_broad", "broad_pairs",
+             "n_contacts", "contact_geoms", "active_buffer", "n_con", "jac_idx", "jac_nnz", "efc_active",
+             "solver_iters"}
+
+
+def state_sizes(M: RigidModel) -> dict:
+    """Per-env element count of every gsr_state array."""
+    L, J, D, Q, G = M.n_links, M.n_joints, M.n_dofs, M.n_qs, M.n_geoms
+    C, R, W = max(M.max_contacts, 1), max(M.max_rows, 1), M.row_width
+    P = max(M.n_pairs, 1)
+    sz = dict(
+        qpos=Q, vel=D, acc=D, acc_smooth=D, qacc_ws=D, is_warmstart=1, ctrl_mode=D, ctrl_pos=D, ctrl_vel=D,
+        ctrl_force=D, dofs_pos=D, links_pos=3 * L, links_quat=4 * L, geoms_pos=3 * G, geoms_quat=4 * G,
+        aabb_min=3 * G, aabb_max=3 * G, cache_normal=3 * P, sort_value=2 * G, sort_ig=2 * G, first_time=1,
+        errno_env=1,
+        cdof_ang=3 * D, cdof_vel=3 * D, cdofd_ang=3 * D, cdofd_vel=3 * D, root_com=3 * L, cd_vel=3 * L,
+        cd_ang=3 * L, cinr_inertial=9 * L, cinr_pos=3 * L, cinr_mass=L, links_iquat=4 * L,
+        mass_blk=M.mass_blk_size, mass_L_blk=M.mass_blk_size, mass_D_inv=D, qf_smooth=D, force=D,
+        xanchor=3 * J, xaxis=3 * J, crb_inertial=9 * L, crb_pos=3 * L, crb_mass=L, f_ang=3 * D, f_vel=3 * D,
+        cdd_vel=3 * L, cdd_ang=3 * L, cfrc_vel=3 * L, cfrc_ang=3 * L, qf_applied=D, qf_passive=D, qf_bias=D,
+        n_broad=1, broad_pairs=max(M.max_broad, 1), n_contacts=1, contact_geoms=2 * C, contact_pos=3 * C,
+        contact_normal=3 * C, contact_pen=C, contact_friction=C, contact_force=3 * C, active_buffer=G,
+        n_con=3, jac_val=R * W, jac_idx=R * W, jac_nnz=R, efc_diag=R, efc_aref=R, efc_D=R, efc_fl=R,
+        efc_force=R, efc_Jaref=R, efc_jv=R, efc_quad=3 * R, efc_active=R,
+        qacc=D, Ma=D, grad=D, Mgrad=D, search=D, mv=D, qfrc_constraint=D, nt_vec=D, nt_H=D * (D + 1) // 2,
+        solver_iters=1, qpos_next=Q, vel_next=D,
+    )
+    missing = set(_lib.STATE_POINTERS) - set(sz)
+    assert not missing, f"state arrays without a size rule: {missing}"
+    return {k: max(int(v), 1) for k, v in sz.items()}
+
+
+def finalize_model(M: RigidModel):
+    """Derived tables the library needs beyond the compiler output."""
+    ns = (M.ent_dof_end - M.ent_dof_start).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(ns * ns)[:-1]]) if len(ns) else np.zeros(0)
+    M.ent_mass_offset = off.astype(np.int32)
+    M.mass_blk_size = int((ns * ns).sum())
+
+
+class DeviceModel:
+    """Static tables of one scene type uploaded once (torch-owned device memory) + the gsr_model descriptor."""
+
+    def __init__(self, M: RigidModel, device):
+        finalize_model(M)
+        self.M = M
+        self.device = torch.device(device)
+        self.tensors = {}
+        for k in MODEL_TABLES_I:
+            a = np.ascontiguousarray(np.asarray(getattr(M, k)), dtype=np.int32).ravel()
+            self.tensors[k] = torch.from_numpy(a if a.size else np.zeros(1, np.int32)).to(self.device)
+        for k in MODEL_TABLES_F:
+            a = np.ascontiguousarray(np.asarray(getattr(M, k)), dtype=np.float32).ravel()
+            self.tensors[k] = torch.from_numpy(a if a.size else np.zeros(1, np.float32)).to(self.device)
+        o = M.options
+        d = _lib.GsrModel()
+        d.n_links, d.n_joints, d.n_dofs, d.n_qs = M.n_links, M.n_joints, M.n_dofs, M.n_qs
+        d.n_geoms, d.n_entities, d.n_equalities, d.n_verts, d.n_pairs = (M.n_geoms, M.n_entities, M.n_equalities,
+                                                                         M.n_verts, M.n_pairs)
+        d.max_contacts, d.max_broad, d.max_rows, d.row_width = M.max_contacts, M.max_broad, M.max_rows, M.row_width
+        d.mass_blk_size, d.max_entity_dofs = M.mass_blk_size, M.max_entity_dofs
+        d.iterations, d.ls_iterations, d.n_contacts_per_pair = o.iterations, o.ls_iterations, o.n_contacts_per_pair
+        d.enable_collision, d.enable_joint_limit = int(o.enable_collision), int(o.enable_joint_limit)
+        d.enable_multi_contact = int(o.enable_multi_contact)
+        d.dt, d.tolerance, d.ls_tolerance = o.dt, o.tolerance, o.ls_tolerance
+        d.eps = float(np.finfo(np.float32).eps)
+        d.meaninertia = float(M.meaninertia)
+        d.mc_perturbation, d.mc_tolerance, d.mpr_to_gjk = 1e-2, 1e-2, 0.25
+        for i in range(3):
+            d.gravity[i] = float(M.gravity[i])
+        for k in _lib.MODEL_POINTERS:
+            setattr(d, k, self.tensors[k].data_ptr())
+        self.desc = d
+
+
+class DeviceState:
+    """Per-env state of n_envs envs of one model (torch tensors, env-minor) + the gsr_state descriptor."""
+
+    def __init__(self, dm: DeviceModel, n_envs: int):
+        self.dm = dm
+        self.B = int(n_envs)
+        M = dm.M
+        self.sizes = state_sizes(M)
+        self.t = {}
+        for k, n in self.sizes.items():
+            dt = torch.int32 if k in INT_STATE else torch.float32
+            self.t[k] = torch.zeros((n, self.B), dtype=dt, device=dm.device)
+        d = _lib.GsrState()
+        d.n_envs = self.B
+        for k in _lib.STATE_POINTERS:
+            setattr(d, k, self.t[k].data_ptr())
+        self.desc = d
+        self.init_state()
+
+    def init_state(self, envs=None):
+        M = self.dm.M
+        sl = slice(None) if envs is None else envs
+        self.t["qpos"][:, sl] = torch.as_tensor(M.qpos0, device=self.dm.device)[:, None]
+        for k in ("vel", "acc", "acc_smooth", "qacc_ws", "ctrl_pos", "ctrl_vel", "ctrl_force", "cache_normal",
+                  "efc_active"):
+            self.t[k][:, sl] = 0
+        self.t["ctrl_mode"][:, sl] = CTRL_FORCE
+        self.t["is_warmstart"][:, sl] = 0
+        self.t["first_time"][:, sl] = 1
+        self.t["errno_env"][:, sl] = 0
+
+    def memory_bytes(self):
+        return sum(t.numel() * t.element_size() for t in self.t.values())
+
+
+class RigidSolver:
+    """Batched rigid solver for one scene type on one device."""
+
+    def __init__(self, model: RigidModel, n_envs: int, device="cuda"):
+        self.model = model
+        self.n_envs = n_envs
+        self._B = max(1, n_envs)
+        self.device = torch.device(device)
+        self._lib = _lib.load()
+        self.dm = DeviceModel(model, self.device)
+        self.st = DeviceState(self.dm, self._B)
+        self._errno_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._substep = 0
+        self.forward_kinematics()
+
+    # -- raw launches -------------------------------------------------------------------------------------------
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream) if self.device.type == "cuda" else None
+
+    def forward_kinematics(self, envs_idx: Optional[torch.Tensor] = None):
+        if envs_idx is None:
+            rc = self._lib.gsr_forward_kinematics(self.dm.desc, self.st.desc, None, 0, self._stream())
+        else:
+            idx = envs_idx.to(device=self.device, dtype=torch.int32).contiguous()
+            rc = self._lib.gsr_forward_kinematics(self.dm.desc, self.st.desc, ctypes.c_void_p(idx.data_ptr()),
+                                                  int(idx.numel()), self._stream())
+        _lib.check(rc, "gsr_forward_kinematics")
+
+    def reset_caches(self, envs_idx: Optional[torch.Tensor] = None, what=GSR_RESET_COLLIDER | GSR_RESET_WARMSTART):
+        if envs_idx is None:
+            rc = self._lib.gsr_reset_envs(self.dm.desc, self.st.desc, None, 0, what, self._stream())
+        else:
+            idx = envs_idx.to(device=self.device, dtype=torch.int32).contiguous()
+            rc = self._lib.gsr_reset_envs(self.dm.desc, self.st.desc, ctypes.c_void_p(idx.data_ptr()),
+                                          int(idx.numel()), what, self._stream())
+        _lib.check(rc, "gsr_reset_envs")
+
+    def substep(self):
+        _lib.check(self._lib.gsr_step(self.dm.desc, self.st.desc, 1, self._stream()), "gsr_step")
+        self._substep += 1
+        if self._substep % RATE_CHECK_ERRNO == 0:
+            self.check_errno()
+
+    def run_stage(self, stage: int):
+        _lib.check(self._lib.gsr_run_stage(self.dm.desc, self.st.desc, stage, self._stream()), "gsr_run_stage")
+
+    def check_errno(self):
+        e = self._reduce_errno()
+        for bit, msg in ERRNO_MESSAGES:
+            if e & bit:
+                raise GenesisException(msg)
+
+    def _reduce_errno(self):
+        rc = self._lib.gsr_get_errno(self.st.desc, ctypes.c_void_p(self._errno_dev.data_ptr()), self._stream())
+        _lib.check(rc, "gsr_get_errno")
+        return int(self._errno_dev.item())
+
+    # -- zero-copy views ----------------------------------------------------------------------------------------
+    def view(self, name: str) -> torch.Tensor:
+        """(B, n) view of a per-env array (env-minor storage)."""
+        return self.st.t[name].T
+
+    def _idx(self, idx, n, device=None):
+        if idx is None:
+            return torch.arange(n, device=device or self.device)
+        t = torch.as_tensor(idx, device=device or self.device)
+        if t.dtype == torch.bool:
+            t = t.nonzero().flatten()
+        return t.long().reshape(-1)
+
+    # -- control and readback (rsd:2123-2201, 2366-2376) --------------------------------------------------------
+    def _write(self, name, values, dofs_idx, envs_idx, n):
+        d = self._idx(dofs_idx, n)
+        e = self._idx(envs_idx, self._B)
+        v = torch.as_tensor(values, dtype=torch.float32, device=self.device)
+        if self.n_envs == 0:
+            v = v.reshape(1, -1)
+        v = v.reshape(len(e), len(d))
+        self.st.t[name][d[:, None], e[None, :]] = v.T
+
+    def control_dofs_position(self, position, dofs_idx=None, envs_idx=None):
+        d = self._idx(dofs_idx, self.model.n_dofs)
+        e = self._idx(envs_idx, self._B)
+        self.st.t["ctrl_mode"][d[:, None], e[None, :]] = CTRL_POSITION
+        self._write("ctrl_pos", position, d, e, self.model.n_dofs)
+        self.st.t["ctrl_vel"][d[:, None], e[None, :]] = 0.0
+
+    def control_dofs_velocity(self, velocity, dofs_idx=None, envs_idx=None):
+        d = self._idx(dofs_idx, self.model.n_dofs)
+        e = self._idx(envs_idx, self._B)
+        self.st.t["ctrl_mode"][d[:, None], e[None, :]] = CTRL_VELOCITY
+        self._write("ctrl_vel", velocity, d, e, self.model.n_dofs)
+
+    def control_dofs_force(self, force, dofs_idx=None, envs_idx=None):
+        d = self._idx(dofs_idx, self.model.n_dofs)
+        e = self._idx(envs_idx, self._B)
+        self.st.t["ctrl_mode"][d[:, None], e[None, :]] = CTRL_FORCE
+        self._write("ctrl_force", force, d, e, self.model.n_dofs)
+
+    def _read(self, name, idx, envs_idx, n):
+        d = self._idx(idx, n)
+        v = self.st.t[name][d]
+        if envs_idx is not None:
+            v = v[:, self._idx(envs_idx, self._B)]
+        v = v.T
+        return v[0] if self.n_envs == 0 else v
+
+    def get_dofs_position(self, dofs_idx=None, envs_idx=None):
+        return self._read("dofs_pos", dofs_idx, envs_idx, self.model.n_dofs)
+
+    def get_dofs_velocity(self, dofs_idx=None, envs_idx=None):
+        return self._read("vel", dofs_idx, envs_idx, self.model.n_dofs)
+
+    def get_qpos(self, qs_idx=None, envs_idx=None):
+        return self._read("qpos", qs_idx, envs_idx, self.model.n_qs)
+
+    def set_qpos(self, qpos, qs_idx=None, envs_idx=None, skip_forward=False):
+        q = self._idx(qs_idx, self.model.n_qs)
+        e = self._idx(envs_idx, self._B)
+        v = torch.as_tensor(qpos, dtype=torch.float32, device=self.device).reshape(len(e), len(q))
+        self.st.t["qpos"][q[:, None], e[None, :]] = v.T
+        self.reset_caches(e.to(torch.int32))
+        if not skip_forward:
+            self.forward_kinematics(e.to(torch.int32))
+
+    def set_dofs_velocity(self, velocity, dofs_idx=None, envs_idx=None, skip_forward=False):
+        d = self._idx(dofs_idx, self.model.n_dofs)
+        e = self._idx(envs_idx, self._B)
+        self._write("vel", velocity, d, e, self.model.n_dofs)
+        self.reset_caches(e.to(torch.int32), GSR_RESET_WARMSTART)
+        if not skip_forward:
+            self.forward_kinematics(e.to(torch.int32))
+
+    def get_links_pos(self, envs_idx=None):
+        v = self.view("links_pos").reshape(self._B, self.model.n_links, 3)
+        return v if envs_idx is None else v[self._idx(envs_idx, self._B)]
+
+    def get_links_quat(self, envs_idx=None):
+        v = self.view("links_quat").reshape(self._B, self.model.n_links, 4)
+        return v if envs_idx is None else v[self._idx(envs_idx, self._B)]
+
+    # -- state snapshot (rsd:1442-1512) ------------------------------------------------------------------------
+    def get_state(self):
+        return {"qpos": self.st.t["qpos"].clone(), "vel": self.st.t["vel"].clone()}
+
+    def set_state(self, state, envs_idx=None):
+        e = self._idx(envs_idx, self._B)
+        self.st.t["qpos"][:, e] = state["qpos"][:, e]
+        self.st.t["vel"][:, e] = state["vel"][:, e]
+        ei = e.to(torch.int32)
+        self.reset_caches(ei)
+        self.forward_kinematics(ei)

@@ -1,0 +1,137 @@
+/*
+ * genesis_rigid.h -- C-ABI of the MI355X batched rigid-body step (libgenesis_rigid.so).
+ *
+ * The reference has no FFI for this path: its boundary is the Python solver object. Each entry point below replaces
+ * one call the reference's Simulator / RigidSolver makes on that object:
+ *
+ *   gsr_step               <- Simulator.step rigid-only branch (genesis/engine/simulator.py:268-292) calling
+ *                             RigidSolver.substep (rigid_solver_decomp.py:865-933) = kernel_step_1 ->
+ *                             _func_constraint_force (collider detection + constraint solver) -> kernel_step_2.
+ *   gsr_forward_kinematics <- RigidSolver.set_qpos's FK refresh (rigid_solver_decomp.py:1801-1857,
+ *                             kernel_forward_kinematics_links_geoms rsd:4599).
+ *   gsr_reset_envs         <- RigidSolver.set_state cache resets: collider.clear(envs_idx) (collider_decomp.py:328-366)
+ *                             and constraint_solver.clear/reset(envs_idx) (constraint_solver_decomp.py:110-145).
+ *   gsr_get_errno          <- RigidSolver.check_errno (rigid_solver_decomp.py:935-956).
+ *   gsr_step_fused         <- no counterpart (batching form: one launch per substep over a heterogeneous batch).
+ *
+ * Conventions: every pointer in gsr_model / gsr_state is device memory owned by the caller (PyTorch tensors). The
+ * library never allocates or frees on the step path. Per-env arrays are env-minor: element i of env b lives at
+ * [i * n_envs + b]; a vector field of k components stores component c of element i at [(i * k + c) * n_envs + b].
+ * Calls are stream-ordered and never synchronise the host. Return 0 on success, a negative GSR_E* code otherwise.
+ */
+#ifndef GENESIS_RIGID_H
+#define GENESIS_RIGID_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSR_ABI_VERSION 1
+
+#define GSR_OK 0
+#define GSR_E_ARG -1      /* null pointer, bad count */
+#define GSR_E_SHAPE -2    /* model sizes exceed compiled-in limits */
+#define GSR_E_LAUNCH -3   /* HIP launch error */
+
+/* errno bits, same meaning as the reference's errno word (rigid_solver_decomp.py:941-956) */
+#define GSR_ERR_BROAD_OVERFLOW 1u
+#define GSR_ERR_CONTACT_OVERFLOW 2u
+#define GSR_ERR_NAN_QACC 4u
+#define GSR_ERR_NAN_STATE 8u
+#define GSR_ERR_DEEP_PENETRATION 0x100u /* MPR result kept where the reference would fall back to GJK/EPA */
+
+/* reset flags for gsr_reset_envs */
+#define GSR_RESET_COLLIDER 1u  /* contact-normal cache + first_time (sort warm start) */
+#define GSR_RESET_WARMSTART 2u /* qacc_ws = 0, is_warmstart = 0 */
+
+/* Static tables of one scene type (shared by all its envs). Index arrays int32, real arrays float32. */
+typedef struct gsr_model {
+  int32_t n_links, n_joints, n_dofs, n_qs, n_geoms, n_entities, n_equalities, n_verts, n_pairs;
+  int32_t max_contacts, max_broad, max_rows, row_width, mass_blk_size, max_entity_dofs;
+  int32_t iterations, ls_iterations, n_contacts_per_pair;
+  int32_t enable_collision, enable_joint_limit, enable_multi_contact, pad0;
+  float dt, tolerance, ls_tolerance, eps, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
+  float gravity[4];
+  /* links */
+  const int32_t *link_parent, *link_root, *link_q_start, *link_q_end, *link_dof_start, *link_dof_end;
+  const int32_t *link_joint_start, *link_joint_end, *link_is_fixed;
+  const float *link_pos, *link_quat, *link_inertial_pos, *link_inertial_quat, *link_inertial_i, *link_inertial_mass;
+  const float *link_invweight;
+  /* joints */
+  const int32_t *joint_type, *joint_q_start, *joint_dof_start, *joint_dof_end;
+  const float *joint_pos, *joint_sol_params;
+  /* dofs */
+  const float *dof_motion_ang, *dof_motion_vel, *dof_limit, *dof_force_range, *dof_kp, *dof_kv, *dof_damping;
+  const float *dof_armature, *dof_stiffness, *dof_frictionloss, *dof_invweight;
+  const int32_t *dof_entity;
+  const float *qpos0;
+  /* entities */
+  const int32_t *ent_link_start, *ent_link_end, *ent_dof_start, *ent_dof_end, *ent_mass_offset;
+  const float *ent_gravity_compensation;
+  /* geoms */
+  const int32_t *geom_type, *geom_link, *geom_vert_start, *geom_vert_end;
+  const float *geom_pos, *geom_quat, *geom_data, *geom_friction, *geom_sol_params, *geom_center, *geom_init_aabb;
+  const float *verts;
+  /* equalities */
+  const int32_t *eq_type, *eq_obj1, *eq_obj2;
+  const float *eq_data, *eq_sol_params;
+  /* collision pair table (G*G, -1 = never collide) and CRBA ancestor mask (D*D) */
+  const int32_t *pair_idx;
+  const float *mass_parent_mask;
+} gsr_model;
+
+/* Per-env state and scratch of n_envs envs of one scene type (all env-minor). */
+typedef struct gsr_state {
+  int32_t n_envs, pad0;
+  /* persistent state and controls */
+  float *qpos, *vel, *acc, *acc_smooth, *qacc_ws;
+  int32_t *is_warmstart, *ctrl_mode;
+  float *ctrl_pos, *ctrl_vel, *ctrl_force;
+  float *dofs_pos, *links_pos, *links_quat, *geoms_pos, *geoms_quat, *aabb_min, *aabb_max;
+  float *cache_normal, *sort_value;
+  int32_t *sort_ig, *first_time;
+  uint32_t *errno_env;
+  /* kinematics / dynamics intermediates */
+  float *cdof_ang, *cdof_vel, *cdofd_ang, *cdofd_vel, *root_com, *cd_vel, *cd_ang, *cinr_inertial, *cinr_pos;
+  float *cinr_mass, *links_iquat, *mass_blk, *mass_L_blk, *mass_D_inv, *qf_smooth, *force;
+  float *xanchor, *xaxis, *crb_inertial, *crb_pos, *crb_mass, *f_ang, *f_vel, *cdd_vel, *cdd_ang, *cfrc_vel;
+  float *cfrc_ang, *qf_applied, *qf_passive, *qf_bias;
+  /* collision */
+  int32_t *n_broad, *broad_pairs, *n_contacts, *contact_geoms;
+  float *contact_pos, *contact_normal, *contact_pen, *contact_friction, *contact_force;
+  int32_t *active_buffer;
+  /* constraint rows (sparse, row_width nonzeros per row) */
+  int32_t *n_con; /* 3 per env: total, equality, frictionloss */
+  float *jac_val;
+  int32_t *jac_idx, *jac_nnz;
+  float *efc_diag, *efc_aref, *efc_D, *efc_fl, *efc_force, *efc_Jaref, *efc_jv, *efc_quad;
+  int32_t *efc_active;
+  /* Newton solver */
+  float *qacc, *Ma, *grad, *Mgrad, *search, *mv, *qfrc_constraint, *nt_vec, *nt_H;
+  int32_t *solver_iters;
+  /* integration scratch */
+  float *qpos_next, *vel_next;
+} gsr_state;
+
+int gsr_abi_version(void);
+/* sizeof(gsr_model) and sizeof(gsr_state), for binding checks */
+int gsr_model_size(void);
+int gsr_state_size(void);
+
+int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream);
+int gsr_forward_kinematics(const gsr_model* model, const gsr_state* state, const int32_t* envs_idx, int32_t n_idx,
+                           hipStream_t stream);
+int gsr_reset_envs(const gsr_model* model, const gsr_state* state, const int32_t* envs_idx, int32_t n_idx,
+                   uint32_t what, hipStream_t stream);
+int gsr_get_errno(const gsr_state* state, uint32_t* dev_out, hipStream_t stream);
+
+/* Stage runner for parity tests: 1 = kinematics, 2 = + smooth dynamics, 3 = + collision detection, 0 = full step. */
+int gsr_run_stage(const gsr_model* model, const gsr_state* state, int32_t stage, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GENESIS_RIGID_H */

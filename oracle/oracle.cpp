@@ -1085,7 +1085,7 @@ void add_equality(const Model<T>& m, Env<T>& e) {
     T diff = pos2 - ref2, pos = pos1 - ref1, deriv = 0;
     const T* data = &m.eq_data[11 * ie];
     for (int i5 = 0; i5 < 5; i5++) {
-      T dp = std::pow(diff, T(i5));
+      T dp = tpow(diff, T(i5));
       pos = pos - dp * data[i5];
       if (i5 < 4) deriv = deriv + data[i5 + 1] * dp * T(i5 + 1);
     }

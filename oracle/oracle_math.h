@@ -8,6 +8,13 @@
 
 namespace orc {
 
+// Transcendentals: evaluated in double, rounded once to T (the correctly rounded value for T = float in all but
+// vanishingly rare ties), so the float result does not depend on the float libm.
+template <class T> inline T tsin(T x) { return T(std::sin(double(x))); }
+template <class T> inline T tcos(T x) { return T(std::cos(double(x))); }
+template <class T> inline T tatan2(T y, T x) { return T(std::atan2(double(y), double(x))); }
+template <class T> inline T tpow(T x, T y) { return T(std::pow(double(x), double(y))); }
+
 template <class T>
 struct V3 {
   T x, y, z;
@@ -124,7 +131,7 @@ template <class T> inline Q4<T> rotvec_to_quat(V3<T> rv, T eps) {
   if (thetasq > eps * eps) {
     T theta = std::sqrt(thetasq);
     T th = T(0.5) * theta;
-    T c = std::cos(th), s = std::sin(th);
+    T c = tcos(th), s = tsin(th);
     q.w = c;
     V3<T> xyz = (s / theta) * rv;
     q.x = xyz.x; q.y = xyz.y; q.z = xyz.z;
@@ -150,12 +157,12 @@ template <class T> inline V3<T> quat_to_xyz(Q4<T> q, T eps) {
     T sinycosp = q_wz - q_xy;
     T cosycosp = T(1) - (q_yy + q_zz);
     T cosp = std::sqrt(cosycosp * cosycosp + sinycosp * sinycosp);
-    pitch = std::atan2(q_xz + q_wy, cosp);
+    pitch = tatan2(q_xz + q_wy, cosp);
     if (cosp > eps) {
-      roll = std::atan2(q_wx - q_yz, T(1) - (q_xx + q_yy));
-      yaw = std::atan2(sinycosp, cosycosp);
+      roll = tatan2(q_wx - q_yz, T(1) - (q_xx + q_yy));
+      yaw = tatan2(sinycosp, cosycosp);
     } else {
-      yaw = std::atan2(q_wz + q_xy, T(1) - (q_xx + q_zz));
+      yaw = tatan2(q_wz + q_xy, T(1) - (q_xx + q_zz));
     }
   }
   return {roll, pitch, yaw};
@@ -164,8 +171,8 @@ template <class T> inline V3<T> quat_to_xyz(Q4<T> q, T eps) {
 // ti_xyz_to_quat (geom.py:18-36)
 template <class T> inline Q4<T> xyz_to_quat(V3<T> xyz) {
   T ai = T(0.5) * xyz.z, aj = T(-0.5) * xyz.y, ak = T(0.5) * xyz.x;
-  T si = std::sin(ai), sj = std::sin(aj), sk = std::sin(ak);
-  T ci = std::cos(ai), cj = std::cos(aj), ck = std::cos(ak);
+  T si = tsin(ai), sj = tsin(aj), sk = tsin(ak);
+  T ci = tcos(ai), cj = tcos(aj), ck = tcos(ak);
   T cc = ci * ck, cs = ci * sk, sc = si * ck, ss = si * sk;
   return {cj * cc + sj * ss, cj * cs - sj * sc, -cj * ss - sj * cc, cj * sc - sj * cs};
 }
@@ -175,7 +182,7 @@ template <class T> inline V3<T> quat_to_rotvec(Q4<T> q, T eps) {
   V3<T> rv(q.x, q.y, q.z);
   T s2 = norm(rv);
   if (s2 > eps) {
-    T angle = T(2) * std::atan2(s2, std::fabs(q.w));
+    T angle = T(2) * tatan2(s2, std::fabs(q.w));
     T inv_sinc = angle / s2;
     rv = ((q.w < 0 ? T(-1) : T(1)) * inv_sinc) * rv;
   }
@@ -212,8 +219,8 @@ template <class T> inline void orthogonals(V3<T> a, V3<T>& b, V3<T>& c) {
 template <class T> inline void imp_aref(const T* p, T neg_pen, T vel, T pos, T& imp, T& aref) {
   T timeconst = p[0], dampratio = p[1], dmin = p[2], dmax = p[3], width = p[4], mid = p[5], power = p[6];
   T imp_x = std::fabs(neg_pen) / width;
-  T imp_a = (T(1) / std::pow(mid, power - T(1))) * std::pow(imp_x, power);
-  T imp_b = T(1) - (T(1) / std::pow(T(1) - mid, power - T(1))) * std::pow(T(1) - imp_x, power);
+  T imp_a = (T(1) / tpow(mid, power - T(1))) * tpow(imp_x, power);
+  T imp_b = T(1) - (T(1) / tpow(T(1) - mid, power - T(1))) * tpow(T(1) - imp_x, power);
   T imp_y = imp_x < mid ? imp_a : imp_b;
   imp = dmin + imp_y * (dmax - dmin);
   imp = std::min(std::max(imp, dmin), dmax);

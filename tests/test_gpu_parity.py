@@ -1,0 +1,184 @@
+"""Parity of the HIP rigid step against the CPU oracle on identical seeded inputs (runs on an MI355X).
+
+Tolerances: FP32 qpos/qvel within rtol = atol = 1e-4 after one step from identical state; broadphase candidate sets
+equal as sets when both see the same AABBs (bit-exact integer output); contact counts equal per env.
+"""
+import numpy as np
+import pytest
+import torch
+
+from genesis_amd.model import CTRL_POSITION, JOINT_FREE, compile_model, CompileOptions
+from genesis_amd import workloads as W
+from tests.sync import gpu_to_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _solver(M, B):
+    from genesis_amd.rigid_solver import RigidSolver
+    return RigidSolver(M, B, "cuda")
+
+
+def _oracle(M, B, prec=32):
+    from oracle.oracle import Oracle
+    return Oracle(M, B, prec)
+
+
+def _random_qpos(M, B, rng, base_z=(0.3, 0.5)):
+    q = np.tile(M.qpos0.astype(np.float64), (B, 1))
+    for j in range(M.n_joints):
+        qs, ds = M.joint_q_start[j], M.joint_dof_start[j]
+        if M.joint_type[j] in (1, 2):
+            lo, hi = M.dof_limit[ds]
+            lo = lo if np.isfinite(lo) else -1.0
+            hi = hi if np.isfinite(hi) else 1.0
+            q[:, qs] = rng.uniform(lo, hi, B)
+        elif M.joint_type[j] == JOINT_FREE:
+            q[:, qs:qs + 2] = rng.uniform(-0.2, 0.2, (B, 2))
+            q[:, qs + 2] = rng.uniform(*base_z, B)
+            quat = rng.normal(size=(B, 4))
+            q[:, qs + 3:qs + 7] = quat / np.linalg.norm(quat, axis=1, keepdims=True)
+    return q
+
+
+CASES = {
+    "arm": lambda: (W.arm_workload().entities, CompileOptions(dt=0.01, enable_self_collision=False)),
+    "quadruped": lambda: (W.quadruped_workload().entities, CompileOptions(dt=0.005)),
+    "pile8": lambda: (W.pile_entities(8, np.random.default_rng(1)), CompileOptions(dt=0.01, max_collision_pairs=200)),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_kinematics_and_dynamics_parity(case):
+    _need_gpu()
+    ents, opt = CASES[case]()
+    M = compile_model(ents, opt)
+    B = 64
+    rng = np.random.default_rng(0)
+    q = _random_qpos(M, B, rng)
+    v = rng.normal(scale=0.5, size=(B, M.n_dofs))
+    S = _solver(M, B)
+    S.st.t["qpos"][:] = torch.as_tensor(q.T, dtype=torch.float32, device="cuda")
+    S.st.t["vel"][:] = torch.as_tensor(v.T, dtype=torch.float32, device="cuda")
+    S.run_stage(2)
+    torch.cuda.synchronize()
+    O = _oracle(M, B)
+    O.set("qpos", S.st.t["qpos"].T.cpu().numpy().astype(np.float64))
+    O.set("vel", S.st.t["vel"].T.cpu().numpy().astype(np.float64))
+    O.run(stage=2)
+    g = lambda k: S.st.t[k].T.cpu().numpy()  # noqa: E731
+    np.testing.assert_allclose(g("links_pos").reshape(B, -1, 3), O.get("links_pos"), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(g("links_quat").reshape(B, -1, 4), O.get("links_quat"), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(g("geoms_pos").reshape(B, -1, 3), O.get("geoms_pos"), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(g("dofs_pos"), O.get("dofs_pos"), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(g("cdof_vel").reshape(B, -1, 3), O.get("cdof_vel"), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(g("cd_vel").reshape(B, -1, 3), O.get("cd_vel"), rtol=1e-4, atol=1e-5)
+    # mass matrix: packed entity blocks on the GPU, dense on the oracle
+    mb = g("mass_blk")
+    Mo = O.get("mass_mat")
+    for e in range(M.n_entities):
+        s_, t_ = M.ent_dof_start[e], M.ent_dof_end[e]
+        n = t_ - s_
+        if n == 0:
+            continue
+        blk = mb[:, M.ent_mass_offset[e]:M.ent_mass_offset[e] + n * n].reshape(B, n, n)
+        np.testing.assert_allclose(blk, Mo[:, s_:t_, s_:t_], rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(g("acc_smooth"), O.get("acc_smooth"), rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_broadphase_bit_exact_and_contacts(case):
+    _need_gpu()
+    ents, opt = CASES[case]()
+    M = compile_model(ents, opt)
+    B = 64
+    rng = np.random.default_rng(1)
+    q = _random_qpos(M, B, rng, base_z=(0.05, 0.35))
+    S = _solver(M, B)
+    S.set_qpos(torch.as_tensor(q, dtype=torch.float32))
+    S.run_stage(3)
+    torch.cuda.synchronize()
+    O = _oracle(M, B)
+    amin = S.st.t["aabb_min"].T.cpu().numpy().reshape(B, -1, 3)
+    amax = S.st.t["aabb_max"].T.cpu().numpy().reshape(B, -1, 3)
+    nb = S.st.t["n_broad"][0].cpu().numpy()
+    bp = S.st.t["broad_pairs"].T.cpu().numpy()
+    total = 0
+    for b in range(B):
+        gpu_pairs = {(int(p) >> 16, int(p) & 0xFFFF) for p in bp[b, :nb[b]]}
+        ref = O.broadphase_from_aabbs(b, amin[b], amax[b])
+        assert gpu_pairs == {tuple(p) for p in ref}, f"env {b}"
+        total += len(gpu_pairs)
+    # full stage-3 contact counts from the same positions
+    O.set("qpos", S.st.t["qpos"].T.cpu().numpy().astype(np.float64))
+    O.run(stage=3)
+    nc_gpu = S.st.t["n_contacts"][0].cpu().numpy()
+    nc_orc = O.get("n_contacts")
+    assert (nc_gpu == nc_orc).all(), (nc_gpu, nc_orc)
+    if case != "arm":
+        assert total > 0
+
+
+def _control(case, M, B, step, rng, S):
+    if case == "arm" and step % 100 == 0:
+        lo = np.where(np.isfinite(M.dof_limit[:, 0]), M.dof_limit[:, 0], -1)
+        hi = np.where(np.isfinite(M.dof_limit[:, 1]), M.dof_limit[:, 1], 1)
+        S.control_dofs_position(torch.as_tensor(rng.uniform(lo, hi, (B, M.n_dofs)), dtype=torch.float32))
+    if case == "quadruped":
+        tgt = W.GO2_STANDING + rng.uniform(-0.3, 0.3, (B, 12))
+        S.control_dofs_position(torch.as_tensor(tgt, dtype=torch.float32), dofs_idx=np.arange(6, 18))
+
+
+@pytest.mark.parametrize("case,B,steps", [("arm", 64, 150), ("quadruped", 128, 150), ("pile8", 64, 150)])
+def test_step_parity_resynced(case, B, steps):
+    """Each step starts the oracle from the HIP state (full persistent state incl. caches), then compares."""
+    _need_gpu()
+    ents, opt = CASES[case]()
+    M = compile_model(ents, opt)
+    rng = np.random.default_rng(7)
+    S = _solver(M, B)
+    if case == "pile8":
+        q = np.tile(M.qpos0.astype(np.float64), (B, 1))
+        n = M.n_qs // 7
+        q = q.reshape(B, n, 7)
+        q[:, :, 0:2] = rng.uniform(-0.3, 0.3, (B, n, 2))
+        q[:, :, 2] = rng.uniform(0.1, 0.8, (B, n))
+        qq = rng.normal(size=(B, n, 4))
+        q[:, :, 3:7] = qq / np.linalg.norm(qq, axis=-1, keepdims=True)
+        S.set_qpos(torch.as_tensor(q.reshape(B, -1), dtype=torch.float32))
+    if case == "quadruped":
+        S.set_qpos(torch.as_tensor(np.tile(np.concatenate([W.GO2_INIT_BASE, W.GO2_STANDING]), (B, 1)), dtype=torch.float32))
+    O = _oracle(M, B)
+    worst = 0.0
+    mismatched_contacts = 0
+    total_contacts = 0
+    for step in range(steps):
+        _control(case, M, B, step, rng, S)
+        torch.cuda.synchronize()
+        gpu_to_oracle(S, O)
+        S.substep()
+        O.run(1)
+        torch.cuda.synchronize()
+        qg, vg = S.st.t["qpos"].T.cpu().numpy(), S.st.t["vel"].T.cpu().numpy()
+        qo, vo = O.get("qpos"), O.get("vel")
+        ncg, nco = S.st.t["n_contacts"][0].cpu().numpy(), O.get("n_contacts")
+        mismatched_contacts += int((ncg != nco).sum())
+        total_contacts += int(nco.sum())
+        ok = ncg == nco
+        # compare envs whose contact sets agree; a disagreement is counted and reported
+        dq = np.abs(qg[ok] - qo[ok]) - TOL * np.abs(qo[ok])
+        dv = np.abs(vg[ok] - vo[ok]) - TOL * np.abs(vo[ok])
+        worst = max(worst, float(dq.max(initial=0)), float(dv.max(initial=0)))
+        np.testing.assert_allclose(qg[ok], qo[ok], rtol=TOL, atol=TOL, err_msg=f"{case} step {step} qpos")
+        np.testing.assert_allclose(vg[ok], vo[ok], rtol=TOL, atol=TOL, err_msg=f"{case} step {step} vel")
+    print(f"{case}: worst excess {worst:.3g}, contact-count mismatches {mismatched_contacts} / {B * steps} env-steps, "
+          f"{total_contacts} contacts")
+    assert mismatched_contacts == 0
+    assert int(S.st.t["errno_env"].max().item()) & 0xF == 0
