@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "genesis_rigid.h")
 LIB_PATH = os.path.join(HERE, "_build", "libgenesis_rigid.so")
 
 EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_step", "gsr_forward_kinematics",
-           "gsr_reset_envs", "gsr_get_errno", "gsr_run_stage"]
+           "gsr_reset_envs", "gsr_get_errno", "gsr_run_stage", "gsr_launch_kernel"]
 
 _CTYPE = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "float": ctypes.c_float}
 
@@ -81,6 +81,7 @@ def load(path: str = LIB_PATH):
     L.gsr_reset_envs.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p]
     L.gsr_get_errno.argtypes = [P(GsrState), ctypes.c_void_p, ctypes.c_void_p]
     L.gsr_run_stage.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
+    L.gsr_launch_kernel.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
     if L.gsr_model_size() != ctypes.sizeof(GsrModel) or L.gsr_state_size() != ctypes.sizeof(GsrState):
         raise RuntimeError("gsr_model/gsr_state layout mismatch between include/genesis_rigid.h and the built library")
     _lib = L

@@ -110,14 +110,20 @@ int launch_status() {
 
 inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
-int run_substep(const gsr_model* m, const gsr_state* s, hipStream_t st) {
+void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
   dim3 g = grid_for(s->n_envs);
-  // kinematics of the current pose are refreshed at the end of every step (and by gsr_forward_kinematics after a
-  // pose write), so kernel_step_1 never needs to recompute them (enable_mujoco_compatibility = False)
-  hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, st, *m, *s, 0);
-  hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, st, *m, *s);
-  hipLaunchKernelGGL(k_solve, g, dim3(kBlock), 0, st, *m, *s);
-  hipLaunchKernelGGL(k_step2, g, dim3(kBlock), 0, st, *m, *s);
+  switch (k) {
+    // kinematics of the current pose are refreshed at the end of every step (and by gsr_forward_kinematics after a
+    // pose write), so kernel_step_1 never needs to recompute them (enable_mujoco_compatibility = False)
+    case 0: hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, st, *m, *s, 0); break;
+    case 1: hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, st, *m, *s); break;
+    case 2: hipLaunchKernelGGL(k_solve, g, dim3(kBlock), 0, st, *m, *s); break;
+    default: hipLaunchKernelGGL(k_step2, g, dim3(kBlock), 0, st, *m, *s); break;
+  }
+}
+
+int run_substep(const gsr_model* m, const gsr_state* s, hipStream_t st) {
+  for (int k = 0; k < 4; k++) launch_one(m, s, k, st);
   return launch_status();
 }
 
@@ -163,6 +169,14 @@ int gsr_reset_envs(const gsr_model* model, const gsr_state* state, const int32_t
 int gsr_get_errno(const gsr_state* state, uint32_t* dev_out, hipStream_t stream) {
   if (!state || !dev_out || state->n_envs <= 0) return GSR_E_ARG;
   hipLaunchKernelGGL(k_errno_reduce, dim3(1), dim3(256), 0, stream, *state, dev_out);
+  return launch_status();
+}
+
+int gsr_launch_kernel(const gsr_model* model, const gsr_state* state, int32_t kernel, hipStream_t stream) {
+  int rc = check(model, state);
+  if (rc) return rc;
+  if (kernel < 0 || kernel > 3) return GSR_E_ARG;
+  launch_one(model, state, kernel, stream);
   return launch_status();
 }
 

@@ -201,6 +201,9 @@ class RigidSolver:
         if self._substep % RATE_CHECK_ERRNO == 0:
             self.check_errno()
 
+    def launch_kernel(self, k: int):
+        _lib.check(self._lib.gsr_launch_kernel(self.dm.desc, self.st.desc, k, self._stream()), "gsr_launch_kernel")
+
     def run_stage(self, stage: int):
         _lib.check(self._lib.gsr_run_stage(self.dm.desc, self.st.desc, stage, self._stream()), "gsr_run_stage")
 

@@ -128,6 +128,11 @@ int gsr_reset_envs(const gsr_model* model, const gsr_state* state, const int32_t
                    uint32_t what, hipStream_t stream);
 int gsr_get_errno(const gsr_state* state, uint32_t* dev_out, hipStream_t stream);
 
+/* One kernel of the plain substep (0 = step_1 dynamics, 1 = collision + constraint rows, 2 = Newton solve,
+ * 3 = integration + end-of-step kinematics); gsr_step launches exactly these four in order. No counterpart in the
+ * reference: used to time each kernel on the caller's stream. */
+int gsr_launch_kernel(const gsr_model* model, const gsr_state* state, int32_t kernel, hipStream_t stream);
+
 /* Stage runner for parity tests: 1 = kinematics, 2 = + smooth dynamics, 3 = + collision detection, 0 = full step. */
 int gsr_run_stage(const gsr_model* model, const gsr_state* state, int32_t stage, hipStream_t stream);
 
