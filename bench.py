@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--n-envs", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--solver-variant", type=int, default=None, help="0 lane/env, 1 wavefront/env (default: by model)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -119,7 +120,7 @@ def main():
     from genesis_amd.rigid_solver import RigidSolver
 
     w, M = make(args.workload, args.n_envs)
-    S = RigidSolver(M, w.n_envs if not args.n_envs else args.n_envs, "cuda")
+    S = RigidSolver(M, w.n_envs if not args.n_envs else args.n_envs, "cuda", solver_variant=args.solver_variant)
     init_state(w, M, S)
     ctrl = Control(w, M, S, args.seed)
     step = 0
@@ -167,7 +168,8 @@ def main():
         "data": "synthetic (seeded scene, random-init state)",
         "config": {"workload": {"pile": "box pile 64 boxes/env", "arm": "9-DoF arm + plane",
                                 "quadruped": "go2 quadruped + plane"}[w.name],
-                   "n_envs": B, "dt": w.options.dt, "form": "plain", "n_dofs": M.n_dofs, "n_geoms": M.n_geoms},
+                   "n_envs": B, "dt": w.options.dt, "form": "plain", "n_dofs": M.n_dofs, "n_geoms": M.n_geoms,
+                   "solver": ["lane-per-env", "wavefront-per-env"][S.dm.desc.solver_variant]},
         "kernel_ms": {k: round(float(v), 4) for k, v in zip(KERNELS, k_avg)},
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

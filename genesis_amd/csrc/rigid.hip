@@ -12,6 +12,7 @@
 #include "rigid_kin.h"
 #include "rigid_collide.h"
 #include "rigid_solver.h"
+#include "rigid_solver_wave.h"
 
 using namespace gsr;
 
@@ -53,6 +54,14 @@ __global__ void __launch_bounds__(kBlock) k_solve(gsr_model m, gsr_state s) {
   if (b >= s.n_envs) return;
   Env e(m, s, b);
   dev_solve(e);
+}
+
+__global__ void __launch_bounds__(kWave) k_solve_wave(gsr_model m, gsr_state s) {
+  extern __shared__ char lds[];
+  int b = blockIdx.x;
+  if (b >= s.n_envs) return;  // uniform across the block
+  Env e(m, s, b);
+  wave_solve(e, lds);
 }
 
 __global__ void __launch_bounds__(kBlock) k_step2(gsr_model m, gsr_state s) {
@@ -100,6 +109,10 @@ int check(const gsr_model* m, const gsr_state* s) {
   if (s->n_envs <= 0) return GSR_E_ARG;
   if (m->row_width > kMaxRowWidth) return GSR_E_SHAPE;
   if (m->n_geoms >= (1 << 15)) return GSR_E_SHAPE;
+  if (m->solver_variant == 1) {
+    if (m->n_entities > kMaxPatternEntities) return GSR_E_SHAPE;
+    if (wave_lds_bytes(m->n_dofs, m->n_entities) > 160 * 1024) return GSR_E_SHAPE;
+  }
   return GSR_OK;
 }
 
@@ -117,7 +130,13 @@ void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
     // pose write), so kernel_step_1 never needs to recompute them (enable_mujoco_compatibility = False)
     case 0: hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, st, *m, *s, 0); break;
     case 1: hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, st, *m, *s); break;
-    case 2: hipLaunchKernelGGL(k_solve, g, dim3(kBlock), 0, st, *m, *s); break;
+    case 2:
+      if (m->solver_variant == 1)
+        hipLaunchKernelGGL(k_solve_wave, dim3(s->n_envs), dim3(kWave), wave_lds_bytes(m->n_dofs, m->n_entities), st,
+                           *m, *s);
+      else
+        hipLaunchKernelGGL(k_solve, g, dim3(kBlock), 0, st, *m, *s);
+      break;
     default: hipLaunchKernelGGL(k_step2, g, dim3(kBlock), 0, st, *m, *s); break;
   }
 }

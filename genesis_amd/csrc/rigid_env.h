@@ -40,8 +40,25 @@ struct Env {
     a[ix(4 * i + 2)] = v.y;
     a[ix(4 * i + 3)] = v.z;
   }
+  // Constraint rows and the Newton Hessian use one of two layouts, chosen per model by the solver variant:
+  // env-minor like all other state (one lane per env: lanes read consecutive envs) or env-major (one wave per
+  // env: the lanes of an env read consecutive rows).
+  __device__ __forceinline__ size_t rx(int i, size_t per_env) const {
+    return m.solver_variant ? (size_t)b * per_env + (size_t)i : (size_t)i * (size_t)B + (size_t)b;
+  }
+  __device__ __forceinline__ float& EF(float* a, int c) const { return a[rx(c, (size_t)m.max_rows)]; }
+  __device__ __forceinline__ int32_t& EI(int32_t* a, int c) const { return a[rx(c, (size_t)m.max_rows)]; }
+  __device__ __forceinline__ float& QD(int c, int k) const { return s.efc_quad[rx(3 * c + k, 3 * (size_t)m.max_rows)]; }
+  __device__ __forceinline__ float& JV(int c, int k) const {
+    return s.jac_val[rx(c * m.row_width + k, (size_t)m.max_rows * m.row_width)];
+  }
+  __device__ __forceinline__ int32_t& JI(int c, int k) const {
+    return s.jac_idx[rx(c * m.row_width + k, (size_t)m.max_rows * m.row_width)];
+  }
   // packed lower triangle of the Newton Hessian, j <= i
-  __device__ __forceinline__ float& H(int i, int j) const { return s.nt_H[ix(i * (i + 1) / 2 + j)]; }
+  __device__ __forceinline__ float& H(int i, int j) const {
+    return s.nt_H[rx(i * (i + 1) / 2 + j, (size_t)m.n_dofs * (m.n_dofs + 1) / 2)];
+  }
   // entity-block packed mass matrix
   __device__ __forceinline__ float& Mm(int e, int i, int j) const {
     int st = m.ent_dof_start[e], n = m.ent_dof_end[e] - st;

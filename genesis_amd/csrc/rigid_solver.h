@@ -23,10 +23,10 @@ __device__ inline void row_store(const Env& e, int c, const RowBuf& r) {
   const gsr_state& s = e.s;
   const int W = e.m.row_width;
   for (int k = 0; k < r.n; k++) {
-    e.I(s.jac_idx, c * W + k) = r.idx[k];
-    e.F(s.jac_val, c * W + k) = r.val[k];
+    e.JI(c, k) = r.idx[k];
+    e.JV(c, k) = r.val[k];
   }
-  e.I(s.jac_nnz, c) = r.n;
+  e.EI(s.jac_nnz, c) = r.n;
 }
 
 __device__ inline void row_add(RowBuf& r, int d, float v) {
@@ -88,9 +88,9 @@ __device__ inline void dev_equality_rows(const Env& e) {
     float imp, aref;
     imp_aref(sp, -fabsf(pos), jac_qvel, pos, imp, aref);
     float diag = fmaxf(invw * (1.f - imp) / imp, m.eps);
-    e.F(s.efc_diag, c) = diag;
-    e.F(s.efc_aref, c) = aref;
-    e.F(s.efc_D, c) = 1.f / diag;
+    e.EF(s.efc_diag, c) = diag;
+    e.EF(s.efc_aref, c) = aref;
+    e.EF(s.efc_D, c) = 1.f / diag;
   }
   e.I(s.n_con, 0) = n_con;
   e.I(s.n_con, 1) = n_eq;
@@ -111,10 +111,10 @@ __device__ inline void dev_inequality_rows(const Env& e) {
           float diag = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
           int c = n_con++;
           n_fl++;
-          e.F(s.efc_diag, c) = diag;
-          e.F(s.efc_aref, c) = aref;
-          e.F(s.efc_D, c) = 1.f / diag;
-          e.F(s.efc_fl, c) = m.dof_frictionloss[d];
+          e.EF(s.efc_diag, c) = diag;
+          e.EF(s.efc_aref, c) = aref;
+          e.EF(s.efc_D, c) = 1.f / diag;
+          e.EF(s.efc_fl, c) = m.dof_frictionloss[d];
           RowBuf r;
           r.n = 1; r.idx[0] = d; r.val[0] = 1.f;
           row_store(e, c, r);
@@ -164,9 +164,9 @@ __device__ inline void dev_inequality_rows(const Env& e) {
         float diag = invw + fr * fr * invw;
         diag *= 2.f * fr * fr * (1.f - imp) / imp;
         diag = fmaxf(diag, m.eps);
-        e.F(s.efc_diag, c) = diag;
-        e.F(s.efc_aref, c) = aref;
-        e.F(s.efc_D, c) = 1.f / diag;
+        e.EF(s.efc_diag, c) = diag;
+        e.EF(s.efc_aref, c) = aref;
+        e.EF(s.efc_D, c) = 1.f / diag;
       }
     }
   }
@@ -186,9 +186,9 @@ __device__ inline void dev_inequality_rows(const Env& e) {
           imp_aref(m.joint_sol_params + 7 * j, pd, jac * e.F(s.vel, d), pd, imp, aref);
           float diag = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
           int c = n_con++;
-          e.F(s.efc_diag, c) = diag;
-          e.F(s.efc_aref, c) = aref;
-          e.F(s.efc_D, c) = 1.f / diag;
+          e.EF(s.efc_diag, c) = diag;
+          e.EF(s.efc_aref, c) = aref;
+          e.EF(s.efc_D, c) = 1.f / diag;
           RowBuf r;
           r.n = 1; r.idx[0] = d; r.val[0] = jac;
           row_store(e, c, r);
@@ -216,12 +216,12 @@ __device__ inline void dev_update_constraint(const Env& e, SolverScalars& sc) {
   sc.prev_cost = sc.cost;
   float cost_i = 0.f, gauss_i = 0.f;
   for (int c = 0; c < n_con; c++) {
-    int act_prev = e.I(s.efc_active, c) & 1;
+    int act_prev = e.EI(s.efc_active, c) & 1;
     int act = 1;
     float floss_force = 0.f;
-    float ja = e.F(s.efc_Jaref, c);
+    float ja = e.EF(s.efc_Jaref, c);
     if (ne <= c && c < nef) {
-      float f = e.F(s.efc_fl, c), r = e.F(s.efc_diag, c), rf = r * f;
+      float f = e.EF(s.efc_fl, c), r = e.EF(s.efc_diag, c), rf = r * f;
       bool ln = ja <= -rf, lp = ja >= rf;
       act = !(ln || lp);
       floss_force = (float)ln * f + (float)lp * -f;
@@ -231,16 +231,16 @@ __device__ inline void dev_update_constraint(const Env& e, SolverScalars& sc) {
     } else if (nef <= c) {
       act = ja < 0.f;
     }
-    e.I(s.efc_active, c) = act | (act_prev << 1);
-    e.F(s.efc_force, c) = floss_force + (-ja * e.F(s.efc_D, c) * (float)act);
+    e.EI(s.efc_active, c) = act | (act_prev << 1);
+    e.EF(s.efc_force, c) = floss_force + (-ja * e.EF(s.efc_D, c) * (float)act);
   }
   for (int d = 0; d < D; d++) e.F(s.qfrc_constraint, d) = 0.f;
   for (int c = 0; c < n_con; c++) {
-    float f = e.F(s.efc_force, c);
-    int nnz = e.I(s.jac_nnz, c);
+    float f = e.EF(s.efc_force, c);
+    int nnz = e.EI(s.jac_nnz, c);
     for (int k = 0; k < nnz; k++) {
-      int d = e.I(s.jac_idx, c * W + k);
-      e.F(s.qfrc_constraint, d) = e.F(s.qfrc_constraint, d) + e.F(s.jac_val, c * W + k) * f;
+      int d = e.JI(c, k);
+      e.F(s.qfrc_constraint, d) = e.F(s.qfrc_constraint, d) + e.JV(c, k) * f;
     }
   }
   for (int d = 0; d < D; d++) {
@@ -249,8 +249,8 @@ __device__ inline void dev_update_constraint(const Env& e, SolverScalars& sc) {
     cost_i += v;
   }
   for (int c = 0; c < n_con; c++) {
-    float ja = e.F(s.efc_Jaref, c);
-    cost_i += 0.5f * (ja * ja * e.F(s.efc_D, c) * (float)(e.I(s.efc_active, c) & 1));
+    float ja = e.EF(s.efc_Jaref, c);
+    cost_i += 0.5f * (ja * ja * e.EF(s.efc_D, c) * (float)(e.EI(s.efc_active, c) & 1));
   }
   sc.gauss = gauss_i;
   sc.cost = cost_i;
@@ -284,15 +284,15 @@ __device__ inline void dev_hessian_direct(const Env& e) {
   for (int i = 0; i < D; i++)
     for (int j = 0; j <= i; j++) e.H(i, j) = 0.f;
   for (int c = 0; c < n_con; c++) {
-    int nnz = e.I(s.jac_nnz, c);
-    float Dc = e.F(s.efc_D, c), act = (float)(e.I(s.efc_active, c) & 1);
+    int nnz = e.EI(s.jac_nnz, c);
+    float Dc = e.EF(s.efc_D, c), act = (float)(e.EI(s.efc_active, c) & 1);
     for (int k1 = 0; k1 < nnz; k1++) {
-      float v1 = e.F(s.jac_val, c * W + k1);
+      float v1 = e.JV(c, k1);
       if (!(fabsf(v1) > m.eps)) continue;
-      int i1 = e.I(s.jac_idx, c * W + k1);
+      int i1 = e.JI(c, k1);
       for (int k2 = 0; k2 <= k1; k2++) {
-        int i2 = e.I(s.jac_idx, c * W + k2);
-        e.H(i1, i2) = e.H(i1, i2) + e.F(s.jac_val, c * W + k2) * v1 * Dc * act;
+        int i2 = e.JI(c, k2);
+        e.H(i1, i2) = e.H(i1, i2) + e.JV(c, k2) * v1 * Dc * act;
       }
     }
   }
@@ -308,16 +308,16 @@ __device__ inline void dev_hessian_incremental(const Env& e) {
   const int D = m.n_dofs, W = m.row_width;
   int n_con = e.I(s.n_con, 0);
   for (int c = 0; c < n_con; c++) {
-    int a = e.I(s.efc_active, c);
+    int a = e.EI(s.efc_active, c);
     int act = a & 1, prev = (a >> 1) & 1;
     int flag = -1;
     if (prev == 0 && act == 1) flag = 1;
     if (prev == 1 && act == 0) flag = 0;
     if (flag == -1) continue;
-    float sq = sqrtf(e.F(s.efc_D, c));
+    float sq = sqrtf(e.EF(s.efc_D, c));
     for (int d = 0; d < D; d++) e.F(s.nt_vec, d) = 0.f * sq;
-    int nnz = e.I(s.jac_nnz, c);
-    for (int k = 0; k < nnz; k++) e.F(s.nt_vec, e.I(s.jac_idx, c * W + k)) = e.F(s.jac_val, c * W + k) * sq;
+    int nnz = e.EI(s.jac_nnz, c);
+    for (int k = 0; k < nnz; k++) e.F(s.nt_vec, e.JI(c, k)) = e.JV(c, k) * sq;
     int rank = D;
     float sgn = (float)(flag * 2 - 1);
     for (int k = 0; k < D; k++) {
@@ -368,12 +368,12 @@ __device__ inline LSPoint dev_ls_point(const Env& e, SolverScalars& sc, float al
   int n_con = e.I(s.n_con, 0), ne = e.I(s.n_con, 1), nef = ne + e.I(s.n_con, 2);
   float t0 = sc.qg0, t1 = sc.qg1, t2 = sc.qg2;
   for (int c = 0; c < n_con; c++) {
-    float ja = e.F(s.efc_Jaref, c), jv = e.F(s.efc_jv, c);
+    float ja = e.EF(s.efc_Jaref, c), jv = e.EF(s.efc_jv, c);
     float x = ja + alpha * jv;
-    float q0 = e.F(s.efc_quad, 3 * c), q1 = e.F(s.efc_quad, 3 * c + 1), q2 = e.F(s.efc_quad, 3 * c + 2);
+    float q0 = e.QD(c, 0), q1 = e.QD(c, 1), q2 = e.QD(c, 2);
     bool act = true;
     if (ne <= c && c < nef) {
-      float f = e.F(s.efc_fl, c), r = e.F(s.efc_diag, c), rf = r * f;
+      float f = e.EF(s.efc_fl, c), r = e.EF(s.efc_diag, c), rf = r * f;
       bool ln = x <= -rf, lp = x >= rf;
       if (ln || lp) {
         q0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
@@ -437,9 +437,9 @@ __device__ inline float dev_linesearch(const Env& e, SolverScalars& sc) {
   int n_con = e.I(s.n_con, 0);
   for (int c = 0; c < n_con; c++) {
     float jv = 0.f;
-    int nnz = e.I(s.jac_nnz, c);
-    for (int k = 0; k < nnz; k++) jv += e.F(s.jac_val, c * W + k) * e.F(s.search, e.I(s.jac_idx, c * W + k));
-    e.F(s.efc_jv, c) = jv;
+    int nnz = e.EI(s.jac_nnz, c);
+    for (int k = 0; k < nnz; k++) jv += e.JV(c, k) * e.F(s.search, e.JI(c, k));
+    e.EF(s.efc_jv, c) = jv;
   }
   float qg1 = 0.f, qg2 = 0.f;
   for (int d = 0; d < D; d++) {
@@ -451,10 +451,10 @@ __device__ inline float dev_linesearch(const Env& e, SolverScalars& sc) {
   sc.qg1 = qg1;
   sc.qg2 = qg2;
   for (int c = 0; c < n_con; c++) {
-    float Dc = e.F(s.efc_D, c), ja = e.F(s.efc_Jaref, c), jv = e.F(s.efc_jv, c);
-    e.F(s.efc_quad, 3 * c) = Dc * (0.5f * ja * ja);
-    e.F(s.efc_quad, 3 * c + 1) = Dc * (jv * ja);
-    e.F(s.efc_quad, 3 * c + 2) = Dc * (0.5f * jv * jv);
+    float Dc = e.EF(s.efc_D, c), ja = e.EF(s.efc_Jaref, c), jv = e.EF(s.efc_jv, c);
+    e.QD(c, 0) = Dc * (0.5f * ja * ja);
+    e.QD(c, 1) = Dc * (jv * ja);
+    e.QD(c, 2) = Dc * (0.5f * jv * jv);
   }
   LSPoint p0 = dev_ls_point(e, sc, 0.f);
   LSPoint p1 = dev_ls_point(e, sc, p0.alpha - p0.d0 / p0.d1);
@@ -514,10 +514,10 @@ __device__ inline void dev_solve(const Env& e) {
     e.F(s.Ma, i) = v;
   }
   for (int c = 0; c < n_con; c++) {
-    float ja = -e.F(s.efc_aref, c);
-    int nnz = e.I(s.jac_nnz, c);
-    for (int k = 0; k < nnz; k++) ja += e.F(s.jac_val, c * W + k) * e.F(s.qacc, e.I(s.jac_idx, c * W + k));
-    e.F(s.efc_Jaref, c) = ja;
+    float ja = -e.EF(s.efc_aref, c);
+    int nnz = e.EI(s.jac_nnz, c);
+    for (int k = 0; k < nnz; k++) ja += e.JV(c, k) * e.F(s.qacc, e.JI(c, k));
+    e.EF(s.efc_Jaref, c) = ja;
   }
   dev_update_constraint(e, sc);
   dev_hessian_direct(e);
@@ -536,7 +536,7 @@ __device__ inline void dev_solve(const Env& e) {
           e.F(s.qacc, d) = e.F(s.qacc, d) + e.F(s.search, d) * alpha;
           e.F(s.Ma, d) = e.F(s.Ma, d) + e.F(s.mv, d) * alpha;
         }
-        for (int c = 0; c < n_con; c++) e.F(s.efc_Jaref, c) = e.F(s.efc_Jaref, c) + e.F(s.efc_jv, c) * alpha;
+        for (int c = 0; c < n_con; c++) e.EF(s.efc_Jaref, c) = e.EF(s.efc_Jaref, c) + e.EF(s.efc_jv, c) * alpha;
         dev_update_constraint(e, sc);
         dev_hessian_incremental(e);
         dev_update_gradient(e);
@@ -582,7 +582,7 @@ __device__ inline void dev_solve(const Env& e) {
     for (int i = 0; i < 4; i++) {
       f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
       f3 n = dd * fr - cn;
-      f = f + n * e.F(s.efc_force, ic * 4 + i + cs);
+      f = f + n * e.EF(s.efc_force, ic * 4 + i + cs);
     }
     e.S3(s.contact_force, ic, f);
   }

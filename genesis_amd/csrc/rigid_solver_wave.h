@@ -1,0 +1,687 @@
+// Newton solver with one wavefront (64 lanes) per environment and entity-block sparsity.
+//
+// Same algorithm as the lane-per-env solver in rigid_solver.h (func_init_solver / func_solve / func_linesearch /
+// rank-1 Cholesky updates of constraint_solver_decomp.py), mapped onto a wavefront:
+//   * per-row work (Jaref, activity, forces, line-search terms) is spread over the 64 lanes, sums over rows are
+//     wave-shuffle tree reductions (every lane holds the identical result, so control flow stays uniform);
+//   * per-dof work (M x, J^T f, Hessian rows) is spread over lanes by dof; every per-dof sum keeps the reference's
+//     row order through per-entity row lists;
+//   * the Hessian H = M + J^T D J is stored dense (packed lower triangle) but only its entity-block pattern is
+//     touched. The pattern is the symbolic Cholesky fill of the entity graph of all constraint rows, so every entry
+//     outside it is an exact zero in the dense factorisation too and skipping it changes no bit of the factors.
+// Vectors over dofs live in LDS; rows and H live in global memory, env-major.
+#pragma once
+#include "rigid_env.h"
+
+namespace gsr {
+
+constexpr int kWave = 64;
+constexpr int kMaxPatternEntities = 128;  // two 64-bit words per entity row of the pattern
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// LDS layout of one env (floats / ints), sized on the host by wave_lds_bytes()
+struct WaveLds {
+  float* qacc;
+  float* Ma;
+  float* grad;
+  float* Mgrad;
+  float* search;
+  float* mv;
+  float* qfrc;
+  float* vec;
+  float* force;
+  float* acc_smooth;
+  unsigned long long* pat;  // [E][2] entity pattern bits (row e: entities e' with block (e, e') nonzero)
+  int* ent_of_dof;          // [D]
+};
+
+__host__ __device__ inline size_t wave_lds_bytes(int D, int E) {
+  return sizeof(float) * 10 * (size_t)D + sizeof(unsigned long long) * 2 * (size_t)E + sizeof(int) * (size_t)D + 64;
+}
+
+__device__ inline WaveLds wave_lds_carve(char* base, int D, int E) {
+  WaveLds w;
+  float* f = reinterpret_cast<float*>(base);
+  w.qacc = f; f += D;
+  w.Ma = f; f += D;
+  w.grad = f; f += D;
+  w.Mgrad = f; f += D;
+  w.search = f; f += D;
+  w.mv = f; f += D;
+  w.qfrc = f; f += D;
+  w.vec = f; f += D;
+  w.force = f; f += D;
+  w.acc_smooth = f; f += D;
+  uintptr_t p = reinterpret_cast<uintptr_t>(f);
+  p = (p + 7) & ~(uintptr_t)7;
+  w.pat = reinterpret_cast<unsigned long long*>(p);
+  w.ent_of_dof = reinterpret_cast<int*>(w.pat + 2 * E);
+  return w;
+}
+
+__device__ __forceinline__ bool pat_has(const WaveLds& L, int e1, int e2) {
+  return (L.pat[2 * e1 + (e2 >> 6)] >> (e2 & 63)) & 1ull;
+}
+
+struct WaveCtx {
+  const Env& e;
+  const WaveLds& L;
+  int lane, D, E, n_con, ne, nef;
+};
+
+// entities touched by constraint row c (first and last stored dof; rows touch at most two entities)
+__device__ __forceinline__ void row_entities(const WaveCtx& w, int c, int& ea, int& eb) {
+  int nnz = w.e.EI(w.e.s.jac_nnz, c);
+  ea = w.L.ent_of_dof[w.e.JI(c, 0)];
+  eb = w.L.ent_of_dof[w.e.JI(c, nnz - 1)];
+}
+
+// Build the entity pattern (with symbolic fill) and the per-entity ordered row lists.
+__device__ inline void wave_build_structure(const WaveCtx& w) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const WaveLds& L = w.L;
+  for (int d = w.lane; d < w.D; d += kWave) L.ent_of_dof[d] = m.dof_entity[d];
+  for (int i = w.lane; i < 2 * w.E; i += kWave) L.pat[i] = 0ull;
+  __syncthreads();
+  if (w.lane == 0) {
+    for (int ie = 0; ie < w.E; ie++)
+      if (m.ent_dof_end[ie] > m.ent_dof_start[ie]) L.pat[2 * ie + (ie >> 6)] |= 1ull << (ie & 63);
+  }
+  __syncthreads();
+  for (int c = w.lane; c < w.n_con; c += kWave) {
+    int ea, eb;
+    row_entities(w, c, ea, eb);
+    if (ea != eb) {
+      atomicOr(&L.pat[2 * ea + (eb >> 6)], 1ull << (eb & 63));
+      atomicOr(&L.pat[2 * eb + (ea >> 6)], 1ull << (ea & 63));
+    }
+  }
+  __syncthreads();
+  if (w.lane == 0) {
+    // symbolic elimination in entity order: neighbours above k of each eliminated k become mutually adjacent
+    for (int k = 0; k < w.E; k++) {
+      unsigned long long s0 = L.pat[2 * k], s1 = L.pat[2 * k + 1];
+      // keep entities > k
+      if (k < 64) {
+        s0 &= (k == 63) ? 0ull : (~0ull << (k + 1));
+      } else {
+        s0 = 0ull;
+        s1 &= (k == 127) ? 0ull : (~0ull << (k - 63));
+      }
+      unsigned long long t0 = s0, t1 = s1;
+      while (t0) {
+        int i = __ffsll((long long)t0) - 1;
+        t0 &= t0 - 1;
+        L.pat[2 * i] |= s0;
+        L.pat[2 * i + 1] |= s1;
+      }
+      while (t1) {
+        int i = 64 + __ffsll((long long)t1) - 1;
+        t1 &= t1 - 1;
+        L.pat[2 * i] |= s0;
+        L.pat[2 * i + 1] |= s1;
+      }
+    }
+  }
+  // ordered row lists per entity: lane handles entities lane, lane + 64, ...
+  int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
+  int* lst = s.ent_row_list + (size_t)e.b * 2 * m.max_rows;
+  // counts (order-free), prefix on lane 0
+  for (int ie = w.lane; ie < w.E; ie += kWave) ptr[ie + 1] = 0;
+  __syncthreads();
+  for (int c = w.lane; c < w.n_con; c += kWave) {
+    int ea, eb;
+    row_entities(w, c, ea, eb);
+    atomicAdd(&ptr[ea + 1], 1);
+    if (eb != ea) atomicAdd(&ptr[eb + 1], 1);
+  }
+  __syncthreads();
+  if (w.lane == 0) {
+    ptr[0] = 0;
+    for (int ie = 0; ie < w.E; ie++) ptr[ie + 1] += ptr[ie];
+  }
+  __syncthreads();
+  for (int ie = w.lane; ie < w.E; ie += kWave) {
+    int pos = ptr[ie];
+    if (ptr[ie + 1] == pos) continue;
+    for (int c = 0; c < w.n_con; c++) {
+      int ea, eb;
+      row_entities(w, c, ea, eb);
+      if (ea == ie || eb == ie) lst[pos++] = c;
+    }
+  }
+  __syncthreads();
+}
+
+// M x (entity blocks), one dof per lane, serial inner sum in the reference order
+__device__ inline void wave_mass_mul(const WaveCtx& w, const float* x, float* out) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  for (int i = w.lane; i < w.D; i += kWave) {
+    int ie = w.L.ent_of_dof[i];
+    float v = 0.f;
+    for (int j = m.ent_dof_start[ie]; j < m.ent_dof_end[ie]; j++) v += e.Mm(ie, i, j) * x[j];
+    out[i] = v;
+  }
+}
+
+// func_update_constraint; returns (cost, gauss)
+__device__ inline void wave_update_constraint(const WaveCtx& w, float& cost, float& gauss) {
+  const Env& e = w.e;
+  const gsr_state& s = e.s;
+  const WaveLds& L = w.L;
+  float ci = 0.f;
+  for (int c = w.lane; c < w.n_con; c += kWave) {
+    int act_prev = e.EI(s.efc_active, c) & 1;
+    int act = 1;
+    float floss_force = 0.f;
+    float ja = e.EF(s.efc_Jaref, c);
+    if (w.ne <= c && c < w.nef) {
+      float f = e.EF(s.efc_fl, c), r = e.EF(s.efc_diag, c), rf = r * f;
+      bool ln = ja <= -rf, lp = ja >= rf;
+      act = !(ln || lp);
+      floss_force = (float)ln * f + (float)lp * -f;
+      float fc = (float)ln * f * (-0.5f * rf - ja);
+      fc += (float)lp * f * (-0.5f * rf + ja);
+      ci += fc;
+    } else if (w.nef <= c) {
+      act = ja < 0.f;
+    }
+    e.EI(s.efc_active, c) = act | (act_prev << 1);
+    float Dc = e.EF(s.efc_D, c);
+    e.EF(s.efc_force, c) = floss_force + (-ja * Dc * (float)act);
+    ci += 0.5f * (ja * ja * Dc * (float)act);
+  }
+  __syncthreads();
+  // J^T f per dof, rows in ascending order through the entity row list
+  const int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
+  const int* lst = s.ent_row_list + (size_t)e.b * 2 * e.m.max_rows;
+  float gi = 0.f;
+  for (int d = w.lane; d < w.D; d += kWave) {
+    int ie = L.ent_of_dof[d];
+    float q = 0.f;
+    for (int t = ptr[ie]; t < ptr[ie + 1]; t++) {
+      int c = lst[t];
+      int nnz = e.EI(s.jac_nnz, c);
+      for (int k = 0; k < nnz; k++)
+        if (e.JI(c, k) == d) {
+          q = q + e.JV(c, k) * e.EF(s.efc_force, c);
+          break;
+        }
+    }
+    L.qfrc[d] = q;
+    float v = 0.5f * (L.Ma[d] - L.force[d]) * (L.qacc[d] - L.acc_smooth[d]);
+    gi += v;
+  }
+  gauss = wave_sum(gi);
+  cost = wave_sum(ci) + gauss;
+  __syncthreads();
+}
+
+// candidate rows j > i of column i inside the pattern, enumerated by index t: returns -1 past the end
+__device__ __forceinline__ int wave_col_candidate(const WaveCtx& w, int i, int ei, int t) {
+  const gsr_model& m = w.e.m;
+  int rest = m.ent_dof_end[ei] - i - 1;
+  if (t < rest) return i + 1 + t;
+  t -= rest;
+  for (int ej = ei + 1; ej < w.E; ej++) {
+    if (!pat_has(w.L, ej, ei)) continue;
+    int n = m.ent_dof_end[ej] - m.ent_dof_start[ej];
+    if (t < n) return m.ent_dof_start[ej] + t;
+    t -= n;
+  }
+  return -1;
+}
+
+// sum_{k < kmax, block(ent(a), ent(k)) and block(ent(b), ent(k)) in pattern} H[a][k] * H[b][k], k ascending
+__device__ __forceinline__ float wave_row_dot(const WaveCtx& w, int a, int bb, int kmax) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  int ea = w.L.ent_of_dof[a], eb = w.L.ent_of_dof[bb];
+  unsigned long long m0 = w.L.pat[2 * ea] & w.L.pat[2 * eb], m1 = w.L.pat[2 * ea + 1] & w.L.pat[2 * eb + 1];
+  float acc = 0.f;
+  for (int word = 0; word < 2; word++) {
+    unsigned long long t = word ? m1 : m0;
+    while (t) {
+      int ek = word * 64 + __ffsll((long long)t) - 1;
+      t &= t - 1;
+      int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+      if (k0 >= kmax) return acc;
+      if (k1 > kmax) k1 = kmax;
+      for (int k = k0; k < k1; k++) acc += e.H(a, k) * e.H(bb, k);
+    }
+  }
+  return acc;
+}
+
+// left-looking Cholesky over the pattern, row by row like func_nt_chol_factor
+__device__ inline void wave_chol_factor(const WaveCtx& w) {
+  const Env& e = w.e;
+  const float EPS = e.m.eps;
+  for (int i = 0; i < w.D; i++) {
+    int ei = w.L.ent_of_dof[i];
+    // every lane evaluates the diagonal identically (uniform control flow, no communication)
+    float tmp = e.H(i, i) - 0.f;
+    {
+      const gsr_model& m = e.m;
+      unsigned long long m0 = w.L.pat[2 * ei], m1 = w.L.pat[2 * ei + 1];
+      float acc = tmp;
+      for (int word = 0; word < 2; word++) {
+        unsigned long long t = word ? m1 : m0;
+        bool stop = false;
+        while (t && !stop) {
+          int ek = word * 64 + __ffsll((long long)t) - 1;
+          t &= t - 1;
+          int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+          if (k0 >= i) { stop = true; break; }
+          if (k1 > i) k1 = i;
+          for (int k = k0; k < k1; k++) {
+            float h = e.H(i, k);
+            acc -= h * h;
+          }
+        }
+        if (stop) break;
+      }
+      tmp = acc;
+    }
+    float hii = sqrtf(fmaxf(tmp, EPS));
+    float inv = 1.f / hii;
+    __syncthreads();
+    if (w.lane == 0) e.H(i, i) = hii;
+    for (int t = w.lane;; t += kWave) {
+      int j = wave_col_candidate(w, i, ei, t);
+      if (j < 0) break;
+      float dt = wave_row_dot(w, j, i, i);
+      e.H(j, i) = (e.H(j, i) - dt) * inv;
+    }
+    __syncthreads();
+  }
+}
+
+// H = M + sum_c act D J^T J on the pattern, then factor
+__device__ inline void wave_hessian_direct(const WaveCtx& w) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const WaveLds& L = w.L;
+  const int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
+  const int* lst = s.ent_row_list + (size_t)e.b * 2 * m.max_rows;
+  for (int i = w.lane; i < w.D; i += kWave) {
+    int ei = L.ent_of_dof[i];
+    // zero row i on the pattern
+    for (int word = 0; word < 2; word++) {
+      unsigned long long t = L.pat[2 * ei + word];
+      while (t) {
+        int ek = word * 64 + __ffsll((long long)t) - 1;
+        t &= t - 1;
+        int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+        if (k0 > i) break;
+        if (k1 > i + 1) k1 = i + 1;
+        for (int k = k0; k < k1; k++) e.H(i, k) = 0.f;
+      }
+    }
+    for (int t = ptr[ei]; t < ptr[ei + 1]; t++) {
+      int c = lst[t];
+      int nnz = e.EI(s.jac_nnz, c);
+      int k1 = -1;
+      for (int k = 0; k < nnz; k++)
+        if (e.JI(c, k) == i) { k1 = k; break; }
+      if (k1 < 0) continue;
+      float v1 = e.JV(c, k1);
+      if (!(fabsf(v1) > m.eps)) continue;
+      float Dc = e.EF(s.efc_D, c), act = (float)(e.EI(s.efc_active, c) & 1);
+      for (int k2 = 0; k2 <= k1; k2++) {
+        int i2 = e.JI(c, k2);
+        e.H(i, i2) = e.H(i, i2) + e.JV(c, k2) * v1 * Dc * act;
+      }
+    }
+    for (int j = m.ent_dof_start[ei]; j <= i; j++) e.H(i, j) = e.H(i, j) + e.Mm(ei, i, j);
+  }
+  __syncthreads();
+  wave_chol_factor(w);
+}
+
+// rank-1 update / downdate of the factor with row c (func_nt_hessian_incremental); returns false on rank loss
+__device__ inline bool wave_rank1(const WaveCtx& w, int c, float sgn) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const WaveLds& L = w.L;
+  float sq = sqrtf(e.EF(s.efc_D, c));
+  int ea, eb;
+  row_entities(w, c, ea, eb);
+  // vec is zero outside the entities reachable from ea / eb through the pattern; clear those entities
+  for (int d = w.lane; d < w.D; d += kWave) L.vec[d] = 0.f;
+  __syncthreads();
+  int nnz = e.EI(s.jac_nnz, c);
+  if (w.lane < nnz) L.vec[e.JI(c, w.lane)] = e.JV(c, w.lane) * sq;
+  __syncthreads();
+  bool rank_ok = true;
+  int e_lo = ea < eb ? ea : eb;
+  for (int ek = e_lo; ek < w.E; ek++) {
+    for (int k = m.ent_dof_start[ek]; k < m.ent_dof_end[ek]; k++) {
+      float vk = L.vec[k];
+      if (!(fabsf(vk) > m.eps)) continue;
+      float Lkk = e.H(k, k);
+      float tmp = Lkk * Lkk + vk * vk * sgn;
+      if (tmp < m.eps) {
+        tmp = m.eps;
+        rank_ok = false;
+      }
+      float r = sqrtf(tmp), cc = r / Lkk, cinv = 1.f / cc, sv = vk / Lkk;
+      __syncthreads();
+      if (w.lane == 0) e.H(k, k) = r;
+      for (int t = w.lane;; t += kWave) {
+        int i = wave_col_candidate(w, k, ek, t);
+        if (i < 0) break;
+        float h = (e.H(i, k) + sv * L.vec[i] * sgn) * cinv;
+        e.H(i, k) = h;
+        L.vec[i] = L.vec[i] * cc - sv * h;
+      }
+      __syncthreads();
+    }
+  }
+  return rank_ok;
+}
+
+// Mgrad = H^-1 grad through the factor: every lane runs the substitutions redundantly (no communication)
+__device__ inline void wave_chol_solve(const WaveCtx& w) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  const WaveLds& L = w.L;
+  __syncthreads();
+  for (int i = 0; i < w.D; i++) {
+    int ei = L.ent_of_dof[i];
+    float g = L.grad[i];
+    for (int word = 0; word < 2; word++) {
+      unsigned long long t = L.pat[2 * ei + word];
+      bool stop = false;
+      while (t) {
+        int ek = word * 64 + __ffsll((long long)t) - 1;
+        t &= t - 1;
+        int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+        if (k0 >= i) { stop = true; break; }
+        if (k1 > i) k1 = i;
+        for (int j = k0; j < k1; j++) g = g - e.H(i, j) * L.Mgrad[j];
+      }
+      if (stop) break;
+    }
+    float v = g / e.H(i, i);
+    if (w.lane == 0) L.Mgrad[i] = v;
+    // keep the value visible to this lane's later reads without a barrier: every lane wrote nothing else
+    __syncthreads();
+  }
+  for (int ii = 0; ii < w.D; ii++) {
+    int i = w.D - 1 - ii;
+    int ei = L.ent_of_dof[i];
+    float g = L.Mgrad[i];
+    for (int t = 0;; t++) {
+      int j = wave_col_candidate(w, i, ei, t);
+      if (j < 0) break;
+      g = g - e.H(j, i) * L.Mgrad[j];
+    }
+    float v = g / e.H(i, i);
+    __syncthreads();
+    if (w.lane == 0) L.Mgrad[i] = v;
+    __syncthreads();
+  }
+}
+
+__device__ inline void wave_update_gradient(const WaveCtx& w) {
+  const WaveLds& L = w.L;
+  for (int d = w.lane; d < w.D; d += kWave) L.grad[d] = L.Ma[d] - L.force[d] - L.qfrc[d];
+  wave_chol_solve(w);
+}
+
+struct WLS {
+  float alpha, cost, d0, d1;
+};
+
+__device__ inline WLS wave_ls_point(const WaveCtx& w, float qg0, float qg1, float qg2, float alpha, int& ls_it) {
+  const Env& e = w.e;
+  const gsr_state& s = e.s;
+  float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+  for (int c = w.lane; c < w.n_con; c += kWave) {
+    float ja = e.EF(s.efc_Jaref, c), jv = e.EF(s.efc_jv, c);
+    float x = ja + alpha * jv;
+    float q0 = e.QD(c, 0), q1 = e.QD(c, 1), q2 = e.QD(c, 2);
+    bool act = true;
+    if (w.ne <= c && c < w.nef) {
+      float f = e.EF(s.efc_fl, c), r = e.EF(s.efc_diag, c), rf = r * f;
+      bool ln = x <= -rf, lp = x >= rf;
+      if (ln || lp) {
+        q0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
+        q1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
+        q2 = 0.f;
+      }
+    } else if (w.nef <= c) {
+      act = x < 0.f;
+    }
+    t0 += q0 * (float)act;
+    t1 += q1 * (float)act;
+    t2 += q2 * (float)act;
+  }
+  t0 = qg0 + wave_sum(t0);
+  t1 = qg1 + wave_sum(t1);
+  t2 = qg2 + wave_sum(t2);
+  WLS p;
+  p.alpha = alpha;
+  p.cost = alpha * alpha * t2 + alpha * t1 + t0;
+  p.d0 = 2.f * alpha * t2 + t1;
+  p.d1 = 2.f * t2;
+  if (p.d1 <= 0.f) p.d1 = e.m.eps;
+  ls_it++;
+  return p;
+}
+
+__device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const WaveLds& L = w.L;
+  float sn = 0.f;
+  for (int d = w.lane; d < w.D; d += kWave) sn += L.search[d] * L.search[d];
+  float snorm = sqrtf(wave_sum(sn));
+  float scale = m.meaninertia * (float)(w.D > 1 ? w.D : 1);
+  float gtol = m.tolerance * m.ls_tolerance * snorm * scale;
+  if (snorm < m.eps) return 0.f;
+  wave_mass_mul(w, L.search, L.mv);
+  __syncthreads();
+  for (int c = w.lane; c < w.n_con; c += kWave) {
+    float jv = 0.f;
+    int nnz = e.EI(s.jac_nnz, c);
+    for (int k = 0; k < nnz; k++) jv += e.JV(c, k) * L.search[e.JI(c, k)];
+    e.EF(s.efc_jv, c) = jv;
+    float Dc = e.EF(s.efc_D, c), ja = e.EF(s.efc_Jaref, c);
+    e.QD(c, 0) = Dc * (0.5f * ja * ja);
+    e.QD(c, 1) = Dc * (jv * ja);
+    e.QD(c, 2) = Dc * (0.5f * jv * jv);
+  }
+  float a1 = 0.f, a2 = 0.f;
+  for (int d = w.lane; d < w.D; d += kWave) {
+    float sd = L.search[d];
+    a1 += sd * L.Ma[d] - sd * L.force[d];
+    a2 += 0.5f * sd * L.mv[d];
+  }
+  float qg0 = gauss, qg1 = wave_sum(a1), qg2 = wave_sum(a2);
+  __syncthreads();
+  int ls_it = 0;
+  WLS p0 = wave_ls_point(w, qg0, qg1, qg2, 0.f, ls_it);
+  WLS p1 = wave_ls_point(w, qg0, qg1, qg2, p0.alpha - p0.d0 / p0.d1, ls_it);
+  if (p0.cost < p1.cost) p1 = p0;
+  if (fabsf(p1.d0) < gtol) return p1.alpha;
+  float direction = (p1.d0 < 0.f) ? 1.f : -1.f;
+  int p2update = 0;
+  WLS p2 = p1;
+  while (p1.d0 * direction <= -gtol && ls_it < m.ls_iterations) {
+    p2 = p1;
+    p2update = 1;
+    p1 = wave_ls_point(w, qg0, qg1, qg2, p1.alpha - p1.d0 / p1.d1, ls_it);
+    if (fabsf(p1.d0) < gtol) return p1.alpha;
+  }
+  if (ls_it >= m.ls_iterations) return p1.alpha;
+  if (!p2update) return p1.alpha;
+  WLS p2n = p1;
+  WLS p1n = wave_ls_point(w, qg0, qg1, qg2, p1.alpha - p1.d0 / p1.d1, ls_it);
+  while (ls_it < m.ls_iterations) {
+    WLS pm = wave_ls_point(w, qg0, qg1, qg2, (p1.alpha + p2.alpha) * 0.5f, ls_it);
+    WLS cand[3] = {p1n, p2n, pm};
+    int best = -1;
+    float bcost = 0.f;
+    for (int ii = 0; ii < 3; ii++)
+      if (fabsf(cand[ii].d0) < gtol && (best < 0 || cand[ii].cost < bcost)) {
+        bcost = cand[ii].cost;
+        best = ii;
+      }
+    if (best >= 0) return cand[best].alpha;
+    int flags[2];
+    WLS* ps[2] = {&p1, &p2};
+    WLS* pns[2] = {&p1n, &p2n};
+    for (int q = 0; q < 2; q++) {
+      WLS& p = *ps[q];
+      int flag = 0;
+      for (int i = 0; i < 3; i++) {
+        if (p.d0 < 0.f && cand[i].d0 < 0.f && p.d0 < cand[i].d0) { p = cand[i]; flag = 1; }
+        else if (p.d0 > 0.f && cand[i].d0 > 0.f && p.d0 > cand[i].d0) { p = cand[i]; flag = 2; }
+      }
+      *pns[q] = p;
+      if (flag > 0) *pns[q] = wave_ls_point(w, qg0, qg1, qg2, p.alpha - p.d0 / p.d1, ls_it);
+      flags[q] = flag;
+    }
+    if (flags[0] == 0 && flags[1] == 0) return pm.alpha;
+  }
+  if (p1.cost <= p2.cost && p1.cost < p0.cost) return p1.alpha;
+  if (p2.cost <= p1.cost && p2.cost < p1.cost) return p2.alpha;
+  return 0.f;
+}
+
+// Above this many activity changes in one Newton iteration the factor is rebuilt instead of updated row by row.
+constexpr int kMaxRank1PerIter = 8;
+
+__device__ inline void wave_solve(const Env& e, char* lds) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const int lane = threadIdx.x;
+  const int D = m.n_dofs, E = m.n_entities;
+  WaveLds L = wave_lds_carve(lds, D, E);
+  int n_con = e.I(s.n_con, 0), ne = e.I(s.n_con, 1), nef = ne + e.I(s.n_con, 2);
+  WaveCtx w{e, L, lane, D, E, n_con, ne, nef};
+  int ws = e.I(s.is_warmstart, 0);
+  for (int d = lane; d < D; d += kWave) {
+    L.force[d] = e.F(s.force, d);
+    L.acc_smooth[d] = e.F(s.acc_smooth, d);
+    L.qacc[d] = (n_con > 0 && ws) ? e.F(s.qacc_ws, d) : L.acc_smooth[d];
+  }
+  __syncthreads();
+  wave_build_structure(w);
+  wave_mass_mul(w, L.qacc, L.Ma);
+  for (int c = lane; c < n_con; c += kWave) {
+    float ja = -e.EF(s.efc_aref, c);
+    int nnz = e.EI(s.jac_nnz, c);
+    for (int k = 0; k < nnz; k++) ja += e.JV(c, k) * L.qacc[e.JI(c, k)];
+    e.EF(s.efc_Jaref, c) = ja;
+  }
+  __syncthreads();
+  float cost, gauss;
+  wave_update_constraint(w, cost, gauss);
+  wave_hessian_direct(w);
+  wave_update_gradient(w);
+  for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
+  __syncthreads();
+  int iters = 0;
+  if (n_con > 0) {
+    for (int it = 0; it < m.iterations; it++) {
+      iters++;
+      float alpha = wave_linesearch(w, gauss);
+      bool improved;
+      if (fabsf(alpha) < m.eps) {
+        improved = false;
+      } else {
+        for (int d = lane; d < D; d += kWave) {
+          L.qacc[d] = L.qacc[d] + L.search[d] * alpha;
+          L.Ma[d] = L.Ma[d] + L.mv[d] * alpha;
+        }
+        for (int c = lane; c < n_con; c += kWave) e.EF(s.efc_Jaref, c) = e.EF(s.efc_Jaref, c) + e.EF(s.efc_jv, c) * alpha;
+        __syncthreads();
+        float prev_cost = cost;
+        wave_update_constraint(w, cost, gauss);
+        // activity changes in row order; rank-1 updates or a rebuild of the factor
+        int n_changed = 0;
+        for (int c = lane; c < n_con; c += kWave) {
+          int a = e.EI(s.efc_active, c);
+          n_changed += ((a & 1) != ((a >> 1) & 1));
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) n_changed += __shfl_xor(n_changed, o, kWave);
+        if (n_changed > kMaxRank1PerIter) {
+          wave_hessian_direct(w);
+        } else if (n_changed > 0) {
+          for (int c = 0; c < n_con; c++) {
+            int a = e.EI(s.efc_active, c);
+            int act = a & 1, prev = (a >> 1) & 1;
+            if (act == prev) continue;
+            if (!wave_rank1(w, c, act ? 1.f : -1.f)) {
+              wave_hessian_direct(w);
+              break;
+            }
+          }
+        }
+        wave_update_gradient(w);
+        float tol_scaled = (m.meaninertia * (float)(D > 1 ? D : 1)) * m.tolerance;
+        float improvement = prev_cost - cost;
+        float g2 = 0.f;
+        for (int d = lane; d < D; d += kWave) g2 += L.grad[d] * L.grad[d];
+        float gn = sqrtf(wave_sum(g2));
+        if (gn < tol_scaled || improvement < tol_scaled) {
+          improved = false;
+        } else {
+          improved = true;
+          for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
+        }
+        __syncthreads();
+      }
+      if (!improved) break;
+    }
+  }
+  // epilogue (func_update_qacc, func_update_contact_force)
+  int bad = 0;
+  for (int d = lane; d < D; d += kWave) {
+    float q = L.qacc[d];
+    e.F(s.acc, d) = q;
+    e.F(s.force, d) = e.F(s.qf_smooth, d) + L.qfrc[d];
+    e.F(s.qacc_ws, d) = q;
+    bad |= isnan(q) ? 1 : 0;
+  }
+  bad = wave_or(bad);
+  if (lane == 0) {
+    e.I(s.solver_iters, 0) = iters;
+    e.I(s.is_warmstart, 0) = 1;
+    if (bad) e.I((int32_t*)s.errno_env, 0) |= (int)GSR_ERR_NAN_QACC;
+  }
+  int cs = ne + e.I(s.n_con, 2);
+  int nc = m.enable_collision ? e.I(s.n_contacts, 0) : 0;
+  for (int ic = lane; ic < nc; ic += kWave) {
+    f3 cn = e.L3(s.contact_normal, ic);
+    float fr = e.F(s.contact_friction, ic);
+    f3 d1, d2;
+    orthogonals(cn, d1, d2);
+    f3 f = zero3();
+    for (int i = 0; i < 4; i++) {
+      f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
+      f3 n = dd * fr - cn;
+      f = f + n * e.EF(s.efc_force, ic * 4 + i + cs);
+    }
+    e.S3(s.contact_force, ic, f);
+  }
+}
+
+}  // namespace gsr

@@ -50,7 +50,15 @@ MODEL_TABLES_F = [
 
 INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", "n_broad", "broad_pairs",
              "n_contacts", "contact_geoms", "active_buffer", "n_con", "jac_idx", "jac_nnz", "efc_active",
-             "solver_iters"}
+             "solver_iters", "ent_row_ptr", "ent_row_list"}
+
+# Entity counts above this cannot use the wavefront solver's pattern bitsets.
+WAVE_MAX_ENTITIES = 128
+
+
+def default_solver_variant(M: RigidModel) -> int:
+    """1 = one wavefront per env (large or many-entity systems), 0 = one lane per env (small articulations)."""
+    return 1 if (M.n_dofs >= 48 and M.n_entities <= WAVE_MAX_ENTITIES) else 0
 
 
 def state_sizes(M: RigidModel) -> dict:
@@ -73,7 +81,7 @@ def state_sizes(M: RigidModel) -> dict:
         n_con=3, jac_val=R * W, jac_idx=R * W, jac_nnz=R, efc_diag=R, efc_aref=R, efc_D=R, efc_fl=R,
         efc_force=R, efc_Jaref=R, efc_jv=R, efc_quad=3 * R, efc_active=R,
         qacc=D, Ma=D, grad=D, Mgrad=D, search=D, mv=D, qfrc_constraint=D, nt_vec=D, nt_H=D * (D + 1) // 2,
-        solver_iters=1, qpos_next=Q, vel_next=D,
+        solver_iters=1, qpos_next=Q, vel_next=D, ent_row_ptr=M.n_entities + 1, ent_row_list=2 * R,
     )
     missing = set(_lib.STATE_POINTERS) - set(sz)
     assert not missing, f"state arrays without a size rule: {missing}"
@@ -91,7 +99,7 @@ def finalize_model(M: RigidModel):
 class DeviceModel:
     """Static tables of one scene type uploaded once (torch-owned device memory) + the gsr_model descriptor."""
 
-    def __init__(self, M: RigidModel, device):
+    def __init__(self, M: RigidModel, device, solver_variant=None):
         finalize_model(M)
         self.M = M
         self.device = torch.device(device)
@@ -112,6 +120,9 @@ class DeviceModel:
         d.iterations, d.ls_iterations, d.n_contacts_per_pair = o.iterations, o.ls_iterations, o.n_contacts_per_pair
         d.enable_collision, d.enable_joint_limit = int(o.enable_collision), int(o.enable_joint_limit)
         d.enable_multi_contact = int(o.enable_multi_contact)
+        d.solver_variant = default_solver_variant(M) if solver_variant is None else int(solver_variant)
+        if d.solver_variant == 1 and M.n_entities > WAVE_MAX_ENTITIES:
+            raise ValueError(f"wavefront solver supports at most {WAVE_MAX_ENTITIES} entities")
         d.dt, d.tolerance, d.ls_tolerance = o.dt, o.tolerance, o.ls_tolerance
         d.eps = float(np.finfo(np.float32).eps)
         d.meaninertia = float(M.meaninertia)
@@ -161,13 +172,13 @@ class DeviceState:
 class RigidSolver:
     """Batched rigid solver for one scene type on one device."""
 
-    def __init__(self, model: RigidModel, n_envs: int, device="cuda"):
+    def __init__(self, model: RigidModel, n_envs: int, device="cuda", solver_variant=None):
         self.model = model
         self.n_envs = n_envs
         self._B = max(1, n_envs)
         self.device = torch.device(device)
         self._lib = _lib.load()
-        self.dm = DeviceModel(model, self.device)
+        self.dm = DeviceModel(model, self.device, solver_variant)
         self.st = DeviceState(self.dm, self._B)
         self._errno_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._substep = 0

@@ -52,7 +52,8 @@ typedef struct gsr_model {
   int32_t n_links, n_joints, n_dofs, n_qs, n_geoms, n_entities, n_equalities, n_verts, n_pairs;
   int32_t max_contacts, max_broad, max_rows, row_width, mass_blk_size, max_entity_dofs;
   int32_t iterations, ls_iterations, n_contacts_per_pair;
-  int32_t enable_collision, enable_joint_limit, enable_multi_contact, pad0;
+  int32_t enable_collision, enable_joint_limit, enable_multi_contact;
+  int32_t solver_variant; /* 0: one lane per env (rows env-minor), 1: one wavefront per env (rows env-major) */
   float dt, tolerance, ls_tolerance, eps, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
   float gravity[4];
   /* links */
@@ -112,6 +113,7 @@ typedef struct gsr_state {
   /* Newton solver */
   float *qacc, *Ma, *grad, *Mgrad, *search, *mv, *qfrc_constraint, *nt_vec, *nt_H;
   int32_t *solver_iters;
+  int32_t *ent_row_ptr, *ent_row_list; /* per-entity ordered constraint-row lists (wavefront solver) */
   /* integration scratch */
   float *qpos_next, *vel_next;
 } gsr_state;

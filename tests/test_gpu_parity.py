@@ -21,9 +21,9 @@ def _need_gpu():
         pytest.skip("no HIP device")
 
 
-def _solver(M, B):
+def _solver(M, B, variant=None):
     from genesis_amd.rigid_solver import RigidSolver
-    return RigidSolver(M, B, "cuda")
+    return RigidSolver(M, B, "cuda", solver_variant=variant)
 
 
 def _oracle(M, B, prec=32):
@@ -52,6 +52,7 @@ CASES = {
     "arm": lambda: (W.arm_workload().entities, CompileOptions(dt=0.01, enable_self_collision=False)),
     "quadruped": lambda: (W.quadruped_workload().entities, CompileOptions(dt=0.005)),
     "pile8": lambda: (W.pile_entities(8, np.random.default_rng(1)), CompileOptions(dt=0.01, max_collision_pairs=200)),
+    "pile64": lambda: (W.pile_entities(64, np.random.default_rng(1)), CompileOptions(dt=0.01, max_collision_pairs=1000)),
 }
 
 
@@ -136,15 +137,20 @@ def _control(case, M, B, step, rng, S):
         S.control_dofs_position(torch.as_tensor(tgt, dtype=torch.float32), dofs_idx=np.arange(6, 18))
 
 
-@pytest.mark.parametrize("case,B,steps", [("arm", 64, 150), ("quadruped", 128, 150), ("pile8", 64, 150)])
-def test_step_parity_resynced(case, B, steps):
-    """Each step starts the oracle from the HIP state (full persistent state incl. caches), then compares."""
+@pytest.mark.parametrize("case,B,steps,variant", [
+    ("arm", 64, 150, 0), ("quadruped", 128, 150, 0), ("pile8", 64, 150, 0),
+    ("arm", 64, 150, 1), ("quadruped", 128, 150, 1), ("pile8", 64, 150, 1), ("pile64", 16, 60, 1),
+])
+def test_step_parity_resynced(case, B, steps, variant):
+    """Each step starts the oracle from the HIP state (full persistent state incl. caches), then compares.
+
+    variant 0 = one lane per env, 1 = one wavefront per env with entity-block sparse Newton."""
     _need_gpu()
     ents, opt = CASES[case]()
     M = compile_model(ents, opt)
     rng = np.random.default_rng(7)
-    S = _solver(M, B)
-    if case == "pile8":
+    S = _solver(M, B, variant)
+    if case.startswith("pile"):
         q = np.tile(M.qpos0.astype(np.float64), (B, 1))
         n = M.n_qs // 7
         q = q.reshape(B, n, 7)
@@ -156,15 +162,24 @@ def test_step_parity_resynced(case, B, steps):
     if case == "quadruped":
         S.set_qpos(torch.as_tensor(np.tile(np.concatenate([W.GO2_INIT_BASE, W.GO2_STANDING]), (B, 1)), dtype=torch.float32))
     O = _oracle(M, B)
+    # The wavefront solver reorders floating-point reductions, so it is held to the FP32 oracle within TOL except
+    # where the FP32 oracle itself is ill-conditioned: there it must stay within 4x the oracle's own FP32-vs-FP64
+    # spread on the same step (contact-rich Newton solves near an active-set switch; see the DESIGN notes).
+    O64 = _oracle(M, B, 64) if variant == 1 else None
     worst = 0.0
     mismatched_contacts = 0
     total_contacts = 0
+    n_conditioned = 0
     for step in range(steps):
         _control(case, M, B, step, rng, S)
         torch.cuda.synchronize()
         gpu_to_oracle(S, O)
+        if O64 is not None:
+            gpu_to_oracle(S, O64)
         S.substep()
         O.run(1)
+        if O64 is not None:
+            O64.run(1)
         torch.cuda.synchronize()
         qg, vg = S.st.t["qpos"].T.cpu().numpy(), S.st.t["vel"].T.cpu().numpy()
         qo, vo = O.get("qpos"), O.get("vel")
@@ -172,13 +187,17 @@ def test_step_parity_resynced(case, B, steps):
         mismatched_contacts += int((ncg != nco).sum())
         total_contacts += int(nco.sum())
         ok = ncg == nco
-        # compare envs whose contact sets agree; a disagreement is counted and reported
-        dq = np.abs(qg[ok] - qo[ok]) - TOL * np.abs(qo[ok])
-        dv = np.abs(vg[ok] - vo[ok]) - TOL * np.abs(vo[ok])
-        worst = max(worst, float(dq.max(initial=0)), float(dv.max(initial=0)))
-        np.testing.assert_allclose(qg[ok], qo[ok], rtol=TOL, atol=TOL, err_msg=f"{case} step {step} qpos")
-        np.testing.assert_allclose(vg[ok], vo[ok], rtol=TOL, atol=TOL, err_msg=f"{case} step {step} vel")
-    print(f"{case}: worst excess {worst:.3g}, contact-count mismatches {mismatched_contacts} / {B * steps} env-steps, "
-          f"{total_contacts} contacts")
+        for name, g_, o_ in (("qpos", qg, qo), ("vel", vg, vo)):
+            err = np.abs(g_[ok] - o_[ok])
+            bound = TOL + TOL * np.abs(o_[ok])
+            if O64 is not None:
+                o64 = (O64.get("qpos") if name == "qpos" else O64.get("vel"))[ok]
+                spread = np.abs(o_[ok] - o64)
+                n_conditioned += int(((err > bound) & (err <= 4 * spread + TOL)).sum())
+                bound = np.maximum(bound, 4 * spread + TOL)
+            worst = max(worst, float((err - bound).max(initial=-1.0)))
+            assert (err <= bound).all(), f"{case} step {step} {name}: max excess {(err - bound).max():.3g}"
+    print(f"{case}/v{variant}: worst margin {worst:.3g}, contact-count mismatches {mismatched_contacts} / "
+          f"{B * steps} env-steps, {total_contacts} contacts, {n_conditioned} entries held to the FP32/FP64 spread")
     assert mismatched_contacts == 0
     assert int(S.st.t["errno_env"].max().item()) & 0xF == 0
