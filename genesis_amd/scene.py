@@ -1,0 +1,323 @@
+"""gs.Scene / RigidEntity surface of the batched rigid step.
+
+Mirrors the reference's user API on this path: Scene (genesis/engine/scene.py: add_entity :326, build :740-804,
+step :916-936, reset :857-895, get_state :904) and RigidEntity (rigid_entity.py: control_dofs_position :2450,
+get_dofs_position :2571, get_dofs_velocity :2551, set_qpos :2259, local->global index mapping :1780-1825). Option
+classes keep the reference's names and defaults for the fields this path reads (genesis/options/solvers.py:15-51,
+281-407). The step itself runs in the HIP library through RigidSolver.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import loaders
+from .model import (CompileOptions, EntitySpec, GeomSpec, JOINT_FIXED, JOINT_FREE, JointSpec, LinkSpec, GEOM_BOX,
+                    GEOM_PLANE, GEOM_SPHERE, compile_model)
+from .rigid_solver import RigidSolver, RATE_CHECK_ERRNO
+
+ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "assets")
+# Reference asset paths whose compiled descriptions ship with the package.
+BUILTIN_ASSETS = {
+    "xml/franka_emika_panda/panda.xml": "panda",
+    "urdf/go2/urdf/go2.urdf": "go2",
+}
+
+
+# ------------------------------------------------------------------------------------------------------------
+# options
+# ------------------------------------------------------------------------------------------------------------
+@dataclass
+class SimOptions:
+    dt: float = 1e-2
+    substeps: int = 1
+    gravity: tuple = (0.0, 0.0, -9.81)
+    requires_grad: bool = False
+
+
+@dataclass
+class RigidOptions:
+    dt: Optional[float] = None
+    gravity: Optional[tuple] = None
+    enable_collision: bool = True
+    enable_joint_limit: bool = True
+    enable_self_collision: bool = True
+    enable_adjacent_collision: bool = False
+    disable_constraint: bool = False
+    max_collision_pairs: int = 150
+    multiplier_collision_broad_phase: int = 8
+    integrator: str = "approximate_implicitfast"
+    constraint_solver: str = "Newton"
+    iterations: int = 50
+    tolerance: float = 1e-6
+    ls_iterations: int = 50
+    ls_tolerance: float = 1e-2
+    constraint_timeconst: float = 0.01
+    enable_multi_contact: bool = True
+    enable_mujoco_compatibility: bool = False
+    box_box_detection: bool = False
+    use_gjk_collision: Optional[bool] = None
+    max_dynamic_constraints: int = 8
+
+
+class options:  # noqa: N801 - namespace mirroring gs.options
+    SimOptions = SimOptions
+    RigidOptions = RigidOptions
+
+
+# ------------------------------------------------------------------------------------------------------------
+# morphs and materials
+# ------------------------------------------------------------------------------------------------------------
+@dataclass
+class Morph:
+    pos: tuple = (0.0, 0.0, 0.0)
+    quat: tuple = (1.0, 0.0, 0.0, 0.0)
+    fixed: bool = False
+    collision: bool = True
+    contype: int = 1
+    conaffinity: int = 1
+
+
+@dataclass
+class Plane(Morph):
+    normal: tuple = (0.0, 0.0, 1.0)
+    fixed: bool = True
+
+
+@dataclass
+class Box(Morph):
+    size: tuple = (1.0, 1.0, 1.0)
+
+
+@dataclass
+class Sphere(Morph):
+    radius: float = 0.5
+
+
+@dataclass
+class URDF(Morph):
+    file: str = ""
+    merge_fixed_links: bool = True
+    links_to_keep: tuple = ()
+
+
+@dataclass
+class MJCF(Morph):
+    file: str = ""
+
+
+class morphs:  # noqa: N801 - namespace mirroring gs.morphs
+    Plane = Plane
+    Box = Box
+    Sphere = Sphere
+    URDF = URDF
+    MJCF = MJCF
+
+
+@dataclass
+class RigidMaterial:
+    rho: float = 200.0
+    friction: Optional[float] = None
+
+
+class materials:  # noqa: N801
+    Rigid = RigidMaterial
+
+
+def _spec_from_morph(morph: Morph, material: RigidMaterial) -> EntitySpec:
+    pos = np.asarray(morph.pos, np.float64)
+    quat = np.asarray(morph.quat, np.float64)
+    quat = quat / np.linalg.norm(quat)
+    if isinstance(morph, Plane):
+        link = LinkSpec("plane_baselink", -1, pos, quat, [JointSpec("plane_baselink_joint", JOINT_FIXED)],
+                        [GeomSpec(GEOM_PLANE, data=tuple(morph.normal), contype=morph.contype,
+                                  conaffinity=morph.conaffinity)] if morph.collision else [])
+        return EntitySpec([link], rho=material.rho, friction=material.friction, name="plane")
+    if isinstance(morph, (Box, Sphere)):
+        g = GeomSpec(GEOM_BOX, data=tuple(morph.size)) if isinstance(morph, Box) else GeomSpec(GEOM_SPHERE, data=(morph.radius,))
+        g.contype, g.conaffinity = morph.contype, morph.conaffinity
+        name = "box" if isinstance(morph, Box) else "sphere"
+        j = JointSpec(f"{name}_baselink_joint", JOINT_FIXED) if morph.fixed else JointSpec(
+            f"{name}_baselink_joint", JOINT_FREE, kp=0.0, kv=0.0, init_qpos=np.concatenate([pos, quat]))
+        link = LinkSpec(f"{name}_baselink", -1, pos, quat, [j], [g] if morph.collision else [])
+        return EntitySpec([link], rho=material.rho, friction=material.friction, name=name)
+    if isinstance(morph, (URDF, MJCF)):
+        key = morph.file.replace("\\", "/")
+        if key in BUILTIN_ASSETS and not os.path.exists(key):
+            ent = loaders.entity_from_json(__import__("json").load(open(os.path.join(ASSETS, BUILTIN_ASSETS[key] + ".json"))))
+        elif isinstance(morph, URDF):
+            ent = loaders.parse_urdf(morph.file, fixed=morph.fixed, merge_fixed_links=morph.merge_fixed_links,
+                                     links_to_keep=morph.links_to_keep)
+        else:
+            ent = loaders.parse_mjcf(morph.file)
+        # place the root link (and free-joint initial pose) at the morph pose
+        root = ent.links[0]
+        for j in root.joints:
+            if j.type == JOINT_FREE:
+                j.init_qpos = np.concatenate([pos + (j.init_qpos[:3] if j.init_qpos is not None else 0),
+                                              quat if j.init_qpos is None else j.init_qpos[3:]])
+        if not any(j.type == JOINT_FREE for j in root.joints):
+            root.pos = pos + np.asarray(root.pos)
+            root.quat = quat
+        if isinstance(morph, URDF) and morph.fixed:
+            for j in root.joints:
+                if j.type == JOINT_FREE:
+                    j.type = JOINT_FIXED
+        ent.rho = material.rho if material.rho != 200.0 else ent.rho
+        ent.friction = material.friction
+        return ent
+    raise TypeError(f"unsupported morph {type(morph).__name__}")
+
+
+# ------------------------------------------------------------------------------------------------------------
+# entity
+# ------------------------------------------------------------------------------------------------------------
+class RigidEntity:
+    def __init__(self, scene: "Scene", idx: int, morph, material, spec: EntitySpec):
+        self._scene = scene
+        self.idx = idx
+        self.morph = morph
+        self.material = material
+        self.spec = spec
+
+    # filled at build
+    def _bind(self, M):
+        self.link_start, self.link_end = int(M.ent_link_start[self.idx]), int(M.ent_link_end[self.idx])
+        self.dof_start, self.dof_end = int(M.ent_dof_start[self.idx]), int(M.ent_dof_end[self.idx])
+        qs = [M.link_q_start[l] for l in range(self.link_start, self.link_end)]
+        qe = [M.link_q_end[l] for l in range(self.link_start, self.link_end)]
+        self.q_start, self.q_end = (int(min(qs)), int(max(qe))) if qs else (0, 0)
+        self.n_dofs = self.dof_end - self.dof_start
+        self.n_qs = self.q_end - self.q_start
+        self.n_links = self.link_end - self.link_start
+
+    @property
+    def _solver(self) -> RigidSolver:
+        return self._scene.rigid_solver
+
+    def _global(self, idx_local, n, start):
+        if idx_local is None:
+            return torch.arange(start, start + n)
+        t = torch.as_tensor(idx_local).long().reshape(-1)
+        if ((t < 0) | (t >= n)).any():
+            raise IndexError("local index out of range")
+        return t + start
+
+    def control_dofs_position(self, position, dofs_idx_local=None, envs_idx=None):
+        self._solver.control_dofs_position(position, self._global(dofs_idx_local, self.n_dofs, self.dof_start), envs_idx)
+
+    def control_dofs_velocity(self, velocity, dofs_idx_local=None, envs_idx=None):
+        self._solver.control_dofs_velocity(velocity, self._global(dofs_idx_local, self.n_dofs, self.dof_start), envs_idx)
+
+    def control_dofs_force(self, force, dofs_idx_local=None, envs_idx=None):
+        self._solver.control_dofs_force(force, self._global(dofs_idx_local, self.n_dofs, self.dof_start), envs_idx)
+
+    def get_dofs_position(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_position(self._global(dofs_idx_local, self.n_dofs, self.dof_start), envs_idx)
+
+    def get_dofs_velocity(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_velocity(self._global(dofs_idx_local, self.n_dofs, self.dof_start), envs_idx)
+
+    def get_qpos(self, qs_idx_local=None, envs_idx=None):
+        return self._solver.get_qpos(self._global(qs_idx_local, self.n_qs, self.q_start), envs_idx)
+
+    def set_qpos(self, qpos, qs_idx_local=None, envs_idx=None, zero_velocity=True):
+        qidx = self._global(qs_idx_local, self.n_qs, self.q_start)
+        if zero_velocity:
+            self._solver.set_dofs_velocity(torch.zeros((self._solver._idx(envs_idx, self._solver._B).numel(), self.n_dofs)),
+                                           self._global(None, self.n_dofs, self.dof_start), envs_idx, skip_forward=True)
+        self._solver.set_qpos(qpos, qidx, envs_idx)
+
+    def set_dofs_velocity(self, velocity, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_velocity(velocity, self._global(dofs_idx_local, self.n_dofs, self.dof_start), envs_idx)
+
+    def get_links_pos(self, envs_idx=None):
+        return self._solver.get_links_pos(envs_idx)[..., self.link_start:self.link_end, :]
+
+    def get_links_quat(self, envs_idx=None):
+        return self._solver.get_links_quat(envs_idx)[..., self.link_start:self.link_end, :]
+
+    def get_pos(self, envs_idx=None):
+        return self.get_links_pos(envs_idx)[..., 0, :]
+
+    def get_quat(self, envs_idx=None):
+        return self.get_links_quat(envs_idx)[..., 0, :]
+
+
+# ------------------------------------------------------------------------------------------------------------
+# scene
+# ------------------------------------------------------------------------------------------------------------
+class Scene:
+    def __init__(self, sim_options: Optional[SimOptions] = None, rigid_options: Optional[RigidOptions] = None,
+                 show_viewer: bool = False, **_ignored):
+        self.sim_options = sim_options or SimOptions()
+        self.rigid_options = rigid_options or RigidOptions()
+        self.entities: List[RigidEntity] = []
+        self.is_built = False
+        self.rigid_solver: Optional[RigidSolver] = None
+        self.t = 0
+
+    def add_entity(self, morph, material: Optional[RigidMaterial] = None, **_ignored) -> RigidEntity:
+        if self.is_built:
+            raise RuntimeError("Scene is already built")
+        material = material or RigidMaterial()
+        ent = RigidEntity(self, len(self.entities), morph, material, _spec_from_morph(morph, material))
+        self.entities.append(ent)
+        return ent
+
+    def _compile_options(self) -> CompileOptions:
+        so, ro = self.sim_options, self.rigid_options
+        dt = (ro.dt if ro.dt is not None else so.dt) / max(1, so.substeps)
+        return CompileOptions(
+            dt=dt, gravity=tuple(ro.gravity if ro.gravity is not None else so.gravity), iterations=ro.iterations,
+            tolerance=ro.tolerance, ls_iterations=ro.ls_iterations, ls_tolerance=ro.ls_tolerance,
+            constraint_timeconst=ro.constraint_timeconst, max_collision_pairs=ro.max_collision_pairs,
+            multiplier_collision_broad_phase=ro.multiplier_collision_broad_phase,
+            enable_collision=ro.enable_collision and not ro.disable_constraint, enable_joint_limit=ro.enable_joint_limit,
+            enable_self_collision=ro.enable_self_collision, enable_adjacent_collision=ro.enable_adjacent_collision,
+            enable_multi_contact=ro.enable_multi_contact, max_dynamic_constraints=ro.max_dynamic_constraints)
+
+    def build(self, n_envs: int = 0, device="cuda", solver_variant=None):
+        ro = self.rigid_options
+        if ro.integrator != "approximate_implicitfast" or ro.constraint_solver != "Newton":
+            raise NotImplementedError("only the default approximate_implicitfast integrator and Newton solver are built")
+        if ro.enable_mujoco_compatibility:
+            raise NotImplementedError("enable_mujoco_compatibility=True is not built")
+        self.model = compile_model([e.spec for e in self.entities], self._compile_options())
+        self.n_envs = n_envs
+        self.rigid_solver = RigidSolver(self.model, n_envs, device, solver_variant=solver_variant)
+        for e in self.entities:
+            e._bind(self.model)
+        self._init_qpos = self.rigid_solver.st.t["qpos"].clone()
+        self.is_built = True
+
+    @property
+    def envs_idx(self):
+        return torch.arange(max(1, self.n_envs))
+
+    def step(self):
+        for _ in range(max(1, self.sim_options.substeps)):
+            self.rigid_solver.substep()
+        self.t += 1
+
+    def get_state(self):
+        return self.rigid_solver.get_state()
+
+    def reset(self, state=None, envs_idx=None):
+        """Back to the build-time state (or `state`), clearing contact caches and the solver warm start."""
+        S = self.rigid_solver
+        if state is None:
+            e = S._idx(envs_idx, S._B)
+            S.st.init_state(e)
+            S.st.t["qpos"][:, e] = self._init_qpos[:, e]
+            ei = e.to(torch.int32)
+            S.reset_caches(ei)
+            S.forward_kinematics(ei)
+        else:
+            S.set_state(state, envs_idx)
+        if envs_idx is None:
+            self.t = 0

@@ -1655,6 +1655,17 @@ static void get_set_env(Env<T>& e, const Model<T>& m, const char* name, double* 
   if (n == "n_contacts") { if (set) e.n_contacts = iv[0]; else iv[0] = e.n_contacts; return; }
   if (n == "n_broad") { if (set) e.n_broad = iv[0]; else iv[0] = e.n_broad; return; }
   if (n == "n_con") { iv[0] = e.n_con; return; }
+  if (n == "n_con_eq") { iv[0] = e.n_con_eq; return; }
+  if (n == "n_con_fl") { iv[0] = e.n_con_fl; return; }
+  // constraint rows (reads only): dense Jacobian rows [n_con][D] and per-row scalars [n_con]
+  if (n == "efc_jac") { for (size_t i = 0; i < (size_t)e.n_con * m.D; i++) f[i] = double(e.jac[i]); return; }
+  if (n == "efc_D") { for (int i = 0; i < e.n_con; i++) f[i] = double(e.efc_D[i]); return; }
+  if (n == "efc_aref") { for (int i = 0; i < e.n_con; i++) f[i] = double(e.aref[i]); return; }
+  if (n == "efc_diag") { for (int i = 0; i < e.n_con; i++) f[i] = double(e.diag[i]); return; }
+  if (n == "efc_fl") { for (int i = 0; i < e.n_con; i++) f[i] = double(e.efc_fl[i]); return; }
+  if (n == "efc_force") { for (int i = 0; i < e.n_con; i++) f[i] = double(e.efc_force[i]); return; }
+  if (n == "qf_smooth") { for (int i = 0; i < m.D; i++) f[i] = double(e.qf_smooth[i]); return; }
+  if (n == "qacc") { for (int i = 0; i < m.D; i++) f[i] = double(e.qacc[i]); return; }
   if (n == "n_newton_iters") { iv[0] = e.n_newton_iters; return; }
   if (n == "cache_normal") {
     for (size_t i = 0; i < e.cache_normal.size(); i++)
@@ -1694,7 +1705,6 @@ static void get_set_env(Env<T>& e, const Model<T>& m, const char* name, double* 
     }
     return;
   }
-  (void)m;
 }
 
 // Read (set=0) or write (set=1) field `name` of env b.
@@ -1707,7 +1717,7 @@ extern "C" int orc_env_io(void* hp, int b, const char* name, double* f, int32_t*
 }
 
 // Stages for tests: 0 = full step, 1 = kinematics only, 2 = step_1 (kinematics + smooth dynamics),
-// 3 = step_1 + collision detection (AABB/broad/narrow).
+// 3 = step_1 + collision detection (AABB/broad/narrow), 4 = everything before integration (rows + Newton solve).
 template <class T>
 static void run_stage(const Model<T>& m, Env<T>& e, int stage) {
   if (stage == 0) { step(m, e); return; }
@@ -1722,6 +1732,13 @@ static void run_stage(const Model<T>& m, Env<T>& e, int stage) {
     update_aabbs(m, e);
     if (m.P > 0) { broad_phase(m, e); narrow_phase(m, e); }
   }
+  if (stage == 3) return;
+  add_frictionloss(m, e);
+  if (m.enable_collision) add_collision_constraints(m, e);
+  if (m.enable_joint_limit) add_joint_limits(m, e);
+  init_solver(m, e);
+  solve(m, e);
+  update_qacc_and_forces(m, e);
 }
 
 // Advance envs [b0, b1) by n_steps with n_threads host threads (env-chunked).

@@ -59,7 +59,9 @@ FLOAT_TABLES = [
 FIELDS_F = {"qpos": "Q", "vel": "D", "qacc_ws": "D", "ctrl_pos": "D", "ctrl_vel": "D", "ctrl_force": "D", "acc": "D",
             "acc_smooth": "D", "dofs_pos": "D", "sort_value": "2G"}
 FIELDS_I = {"ctrl_mode": "D", "sort_ig": "2G", "sort_ismax": "2G"}
-SCALAR_I = ["is_warmstart", "first_time", "errno", "n_contacts", "n_broad", "n_con", "n_newton_iters"]
+SCALAR_I = ["is_warmstart", "first_time", "errno", "n_contacts", "n_broad", "n_con", "n_newton_iters", "n_con_eq",
+            "n_con_fl"]
+ROW_F = ["efc_D", "efc_aref", "efc_diag", "efc_fl", "efc_force"]
 
 
 class Oracle:
@@ -151,6 +153,16 @@ class Oracle:
                 a = np.zeros((max(int(n[0]), 1), 2), np.int32)
                 L.orc_env_io(self.h, b, b"broad_pairs", None, a.ctypes.data, 0)
                 a = a[: int(n[0])]
+            elif name in ROW_F or name == "efc_jac":
+                n = np.zeros(1, np.int32)
+                L.orc_env_io(self.h, b, b"n_con", None, n.ctypes.data, 0)
+                shape = (int(n[0]), m.n_dofs) if name == "efc_jac" else (int(n[0]),)
+                a = np.zeros(max(int(np.prod(shape)), 1), np.float64)
+                L.orc_env_io(self.h, b, name.encode(), a.ctypes.data, None, 0)
+                a = a[: int(np.prod(shape))].reshape(shape)
+            elif name in ("qf_smooth", "qacc"):
+                a = np.zeros(m.n_dofs, np.float64)
+                L.orc_env_io(self.h, b, name.encode(), a.ctypes.data, None, 0)
             elif name == "contacts":
                 n = np.zeros(1, np.int32)
                 L.orc_env_io(self.h, b, b"n_contacts", None, n.ctypes.data, 0)
@@ -161,7 +173,7 @@ class Oracle:
                 a = np.zeros(shapes[name], np.float64)
                 L.orc_env_io(self.h, b, name.encode(), a.ctypes.data, None, 0)
             out.append(a)
-        if name in ("broad_pairs", "contacts"):
+        if name in ("broad_pairs", "contacts", "efc_jac") or name in ROW_F:
             return out
         return np.stack(out)
 
