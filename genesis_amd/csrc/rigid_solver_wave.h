@@ -41,10 +41,17 @@ struct WaveLds {
   float* acc_smooth;
   unsigned long long* pat;  // [E][2] entity pattern bits (row e: entities e' with block (e, e') nonzero)
   int* ent_of_dof;          // [D]
+  int* lvl_order;           // [E] entities sorted by forward level (elimination dependencies)
+  int* lvl_start;           // [E + 2]
+  int* rlvl_order;          // [E] entities sorted by backward level
+  int* rlvl_start;          // [E + 2]
+  int* cursor;              // [E] row-list fill cursors
+  int* n_lvl;               // [2] number of forward / backward levels
 };
 
 __host__ __device__ inline size_t wave_lds_bytes(int D, int E) {
-  return sizeof(float) * 10 * (size_t)D + sizeof(unsigned long long) * 2 * (size_t)E + sizeof(int) * (size_t)D + 64;
+  return sizeof(float) * 10 * (size_t)D + sizeof(unsigned long long) * 2 * (size_t)E + sizeof(int) * (size_t)D +
+         sizeof(int) * (5 * (size_t)E + 6) + 64;
 }
 
 __device__ inline WaveLds wave_lds_carve(char* base, int D, int E) {
@@ -64,6 +71,12 @@ __device__ inline WaveLds wave_lds_carve(char* base, int D, int E) {
   p = (p + 7) & ~(uintptr_t)7;
   w.pat = reinterpret_cast<unsigned long long*>(p);
   w.ent_of_dof = reinterpret_cast<int*>(w.pat + 2 * E);
+  w.lvl_order = w.ent_of_dof + D;
+  w.lvl_start = w.lvl_order + E;
+  w.rlvl_order = w.lvl_start + E + 2;
+  w.rlvl_start = w.rlvl_order + E;
+  w.cursor = w.rlvl_start + E + 2;
+  w.n_lvl = w.cursor + E;
   return w;
 }
 
@@ -133,32 +146,116 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
       }
     }
   }
-  // ordered row lists per entity: lane handles entities lane, lane + 64, ...
-  int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
-  int* lst = s.ent_row_list + (size_t)e.b * 2 * m.max_rows;
-  // counts (order-free), prefix on lane 0
-  for (int ie = w.lane; ie < w.E; ie += kWave) ptr[ie + 1] = 0;
-  __syncthreads();
+  // entity pair of every row, once (env-major scratch)
+  int* rent = s.efc_ent + (size_t)e.b * m.max_rows;
   for (int c = w.lane; c < w.n_con; c += kWave) {
     int ea, eb;
     row_entities(w, c, ea, eb);
-    atomicAdd(&ptr[ea + 1], 1);
-    if (eb != ea) atomicAdd(&ptr[eb + 1], 1);
+    rent[c] = ea | (eb << 16);
+  }
+  // ordered row lists per entity (CSR): counts, prefix, then a stable placement chunk by chunk
+  int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
+  int* lst = s.ent_row_list + (size_t)e.b * 2 * m.max_rows;
+  for (int ie = w.lane; ie < w.E; ie += kWave) L.cursor[ie] = 0;
+  __syncthreads();
+  for (int c = w.lane; c < w.n_con; c += kWave) {
+    int r = rent[c];
+    int ea = r & 0xffff, eb = r >> 16;
+    atomicAdd(&L.cursor[ea], 1);
+    if (eb != ea) atomicAdd(&L.cursor[eb], 1);
   }
   __syncthreads();
   if (w.lane == 0) {
-    ptr[0] = 0;
-    for (int ie = 0; ie < w.E; ie++) ptr[ie + 1] += ptr[ie];
+    int acc = 0;
+    for (int ie = 0; ie < w.E; ie++) {
+      int n = L.cursor[ie];
+      ptr[ie] = acc;
+      L.cursor[ie] = acc;
+      acc += n;
+    }
+    ptr[w.E] = acc;
   }
   __syncthreads();
-  for (int ie = w.lane; ie < w.E; ie += kWave) {
-    int pos = ptr[ie];
-    if (ptr[ie + 1] == pos) continue;
-    for (int c = 0; c < w.n_con; c++) {
-      int ea, eb;
-      row_entities(w, c, ea, eb);
-      if (ea == ie || eb == ie) lst[pos++] = c;
+  for (int c0 = 0; c0 < w.n_con; c0 += kWave) {
+    int c = c0 + w.lane;
+    bool valid = c < w.n_con;
+    int r = valid ? rent[c] : -1;
+    int ea = valid ? (r & 0xffff) : -1, eb = valid ? (r >> 16) : -2;
+    int rank_a = 0, rank_b = 0, cnt_a = 0, cnt_b = 0;
+    for (int j = 0; j < kWave; j++) {
+      int ra = __shfl(ea, j, kWave), rb = __shfl(eb, j, kWave);
+      bool ta = valid && (ra == ea || rb == ea);
+      bool tb = valid && (ra == eb || rb == eb);
+      if (j < w.lane) { rank_a += ta; rank_b += tb; }
+      cnt_a += ta;
+      cnt_b += tb;
     }
+    if (valid) {
+      lst[L.cursor[ea] + rank_a] = c;
+      if (eb != ea) lst[L.cursor[eb] + rank_b] = c;
+    }
+    __syncthreads();
+    // the last lane touching an entity in this chunk advances its cursor
+    if (valid) {
+      if (rank_a == cnt_a - 1) L.cursor[ea] += cnt_a;
+      if (eb != ea && rank_b == cnt_b - 1) L.cursor[eb] += cnt_b;
+    }
+    __syncthreads();
+  }
+  // level schedules of the pattern: forward (predecessors first) and backward (successors first)
+  if (w.lane == 0) {
+    int* lvl = L.cursor;  // reuse as level scratch
+    int maxl = -1;
+    for (int ie = 0; ie < w.E; ie++) {
+      if (m.ent_dof_end[ie] <= m.ent_dof_start[ie]) { lvl[ie] = -1; continue; }
+      int lv = 0;
+      for (int word = 0; word < 2; word++) {
+        unsigned long long t = L.pat[2 * ie + word];
+        while (t) {
+          int k = word * 64 + __ffsll((long long)t) - 1;
+          t &= t - 1;
+          if (k >= ie) break;
+          if (lvl[k] + 1 > lv) lv = lvl[k] + 1;
+        }
+      }
+      lvl[ie] = lv;
+      if (lv > maxl) maxl = lv;
+    }
+    int nl = maxl + 1;
+    for (int l = 0; l <= nl; l++) L.lvl_start[l] = 0;
+    for (int ie = 0; ie < w.E; ie++) if (lvl[ie] >= 0) L.lvl_start[lvl[ie] + 1]++;
+    for (int l = 0; l < nl; l++) L.lvl_start[l + 1] += L.lvl_start[l];
+    for (int l = 0; l < nl; l++) L.rlvl_start[l] = L.lvl_start[l];  // temp cursor
+    for (int ie = 0; ie < w.E; ie++) if (lvl[ie] >= 0) L.lvl_order[L.rlvl_start[lvl[ie]]++] = ie;
+    L.n_lvl[0] = nl;
+    // backward levels
+    maxl = -1;
+    for (int ie = w.E - 1; ie >= 0; ie--) {
+      if (m.ent_dof_end[ie] <= m.ent_dof_start[ie]) { lvl[ie] = -1; continue; }
+      int lv = 0;
+      for (int word = 0; word < 2; word++) {
+        unsigned long long t = L.pat[2 * ie + word];
+        while (t) {
+          int k = word * 64 + __ffsll((long long)t) - 1;
+          t &= t - 1;
+          if (k <= ie) continue;
+          if (lvl[k] + 1 > lv) lv = lvl[k] + 1;
+        }
+      }
+      lvl[ie] = lv;
+      if (lv > maxl) maxl = lv;
+    }
+    nl = maxl + 1;
+    for (int l = 0; l <= nl; l++) L.rlvl_start[l] = 0;
+    for (int ie = 0; ie < w.E; ie++) if (lvl[ie] >= 0) L.rlvl_start[lvl[ie] + 1]++;
+    for (int l = 0; l < nl; l++) L.rlvl_start[l + 1] += L.rlvl_start[l];
+    // stable fill through a second pass (lvl_start space beyond nl is free scratch only when E is large; use
+    // an O(E * nl) scan instead)
+    int pos = 0;
+    for (int l = 0; l < nl; l++)
+      for (int ie = 0; ie < w.E; ie++)
+        if (lvl[ie] == l) L.rlvl_order[pos++] = ie;
+    L.n_lvl[1] = nl;
   }
   __syncthreads();
 }
@@ -234,11 +331,17 @@ __device__ __forceinline__ int wave_col_candidate(const WaveCtx& w, int i, int e
   int rest = m.ent_dof_end[ei] - i - 1;
   if (t < rest) return i + 1 + t;
   t -= rest;
-  for (int ej = ei + 1; ej < w.E; ej++) {
-    if (!pat_has(w.L, ej, ei)) continue;
-    int n = m.ent_dof_end[ej] - m.ent_dof_start[ej];
-    if (t < n) return m.ent_dof_start[ej] + t;
-    t -= n;
+  for (int word = (ei + 1) >> 6; word < 2; word++) {
+    unsigned long long bits = w.L.pat[2 * ei + word];
+    int lo = ei + 1 - word * 64;
+    if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
+    while (bits) {
+      int ej = word * 64 + __ffsll((long long)bits) - 1;
+      bits &= bits - 1;
+      int n = m.ent_dof_end[ej] - m.ent_dof_start[ej];
+      if (t < n) return m.ent_dof_start[ej] + t;
+      t -= n;
+    }
   }
   return -1;
 }
@@ -264,45 +367,47 @@ __device__ __forceinline__ float wave_row_dot(const WaveCtx& w, int a, int bb, i
   return acc;
 }
 
-// left-looking Cholesky over the pattern, row by row like func_nt_chol_factor
+// Left-looking Cholesky over the pattern with the entries and summation orders of func_nt_chol_factor. Entities of
+// one forward level have no predecessor in common level, so one lane factorises a whole entity block column.
 __device__ inline void wave_chol_factor(const WaveCtx& w) {
   const Env& e = w.e;
-  const float EPS = e.m.eps;
-  for (int i = 0; i < w.D; i++) {
-    int ei = w.L.ent_of_dof[i];
-    // every lane evaluates the diagonal identically (uniform control flow, no communication)
-    float tmp = e.H(i, i) - 0.f;
-    {
-      const gsr_model& m = e.m;
-      unsigned long long m0 = w.L.pat[2 * ei], m1 = w.L.pat[2 * ei + 1];
-      float acc = tmp;
-      for (int word = 0; word < 2; word++) {
-        unsigned long long t = word ? m1 : m0;
-        bool stop = false;
-        while (t && !stop) {
-          int ek = word * 64 + __ffsll((long long)t) - 1;
-          t &= t - 1;
-          int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
-          if (k0 >= i) { stop = true; break; }
-          if (k1 > i) k1 = i;
-          for (int k = k0; k < k1; k++) {
-            float h = e.H(i, k);
-            acc -= h * h;
+  const gsr_model& m = e.m;
+  const float EPS = m.eps;
+  const WaveLds& L = w.L;
+  int nl = L.n_lvl[0];
+  for (int lv = 0; lv < nl; lv++) {
+    for (int idx = L.lvl_start[lv] + w.lane; idx < L.lvl_start[lv + 1]; idx += kWave) {
+      int ei = L.lvl_order[idx];
+      int s0 = m.ent_dof_start[ei], s1 = m.ent_dof_end[ei];
+      for (int i = s0; i < s1; i++) {
+        // diagonal: tmp = H_ii, then tmp -= H_ik^2 for k < i in ascending order, like the reference
+        float tmp = e.H(i, i);
+        for (int word = 0; word < 2; word++) {
+          unsigned long long t = L.pat[2 * ei + word];
+          bool stop = false;
+          while (t) {
+            int ek = word * 64 + __ffsll((long long)t) - 1;
+            t &= t - 1;
+            int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+            if (k0 >= i) { stop = true; break; }
+            if (k1 > i) k1 = i;
+            for (int k = k0; k < k1; k++) {
+              float h = e.H(i, k);
+              tmp -= h * h;
+            }
           }
+          if (stop) break;
         }
-        if (stop) break;
+        float hii = sqrtf(fmaxf(tmp, EPS));
+        e.H(i, i) = hii;
+        float inv = 1.f / hii;
+        for (int t = 0;; t++) {
+          int j = wave_col_candidate(w, i, ei, t);
+          if (j < 0) break;
+          float dt = wave_row_dot(w, j, i, i);
+          e.H(j, i) = (e.H(j, i) - dt) * inv;
+        }
       }
-      tmp = acc;
-    }
-    float hii = sqrtf(fmaxf(tmp, EPS));
-    float inv = 1.f / hii;
-    __syncthreads();
-    if (w.lane == 0) e.H(i, i) = hii;
-    for (int t = w.lane;; t += kWave) {
-      int j = wave_col_candidate(w, i, ei, t);
-      if (j < 0) break;
-      float dt = wave_row_dot(w, j, i, i);
-      e.H(j, i) = (e.H(j, i) - dt) * inv;
     }
     __syncthreads();
   }
@@ -394,45 +499,51 @@ __device__ inline bool wave_rank1(const WaveCtx& w, int c, float sgn) {
   return rank_ok;
 }
 
-// Mgrad = H^-1 grad through the factor: every lane runs the substitutions redundantly (no communication)
+// Mgrad = H^-1 grad through the factor (func_nt_chol_solve): forward over forward levels, backward over backward
+// levels; one lane per entity, each entry with the reference's summation order.
 __device__ inline void wave_chol_solve(const WaveCtx& w) {
   const Env& e = w.e;
   const gsr_model& m = e.m;
   const WaveLds& L = w.L;
   __syncthreads();
-  for (int i = 0; i < w.D; i++) {
-    int ei = L.ent_of_dof[i];
-    float g = L.grad[i];
-    for (int word = 0; word < 2; word++) {
-      unsigned long long t = L.pat[2 * ei + word];
-      bool stop = false;
-      while (t) {
-        int ek = word * 64 + __ffsll((long long)t) - 1;
-        t &= t - 1;
-        int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
-        if (k0 >= i) { stop = true; break; }
-        if (k1 > i) k1 = i;
-        for (int j = k0; j < k1; j++) g = g - e.H(i, j) * L.Mgrad[j];
+  int nl = L.n_lvl[0];
+  for (int lv = 0; lv < nl; lv++) {
+    for (int idx = L.lvl_start[lv] + w.lane; idx < L.lvl_start[lv + 1]; idx += kWave) {
+      int ei = L.lvl_order[idx];
+      for (int i = m.ent_dof_start[ei]; i < m.ent_dof_end[ei]; i++) {
+        float g = L.grad[i];
+        for (int word = 0; word < 2; word++) {
+          unsigned long long t = L.pat[2 * ei + word];
+          bool stop = false;
+          while (t) {
+            int ek = word * 64 + __ffsll((long long)t) - 1;
+            t &= t - 1;
+            int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+            if (k0 >= i) { stop = true; break; }
+            if (k1 > i) k1 = i;
+            for (int j = k0; j < k1; j++) g = g - e.H(i, j) * L.Mgrad[j];
+          }
+          if (stop) break;
+        }
+        L.Mgrad[i] = g / e.H(i, i);
       }
-      if (stop) break;
     }
-    float v = g / e.H(i, i);
-    if (w.lane == 0) L.Mgrad[i] = v;
-    // keep the value visible to this lane's later reads without a barrier: every lane wrote nothing else
     __syncthreads();
   }
-  for (int ii = 0; ii < w.D; ii++) {
-    int i = w.D - 1 - ii;
-    int ei = L.ent_of_dof[i];
-    float g = L.Mgrad[i];
-    for (int t = 0;; t++) {
-      int j = wave_col_candidate(w, i, ei, t);
-      if (j < 0) break;
-      g = g - e.H(j, i) * L.Mgrad[j];
+  nl = L.n_lvl[1];
+  for (int lv = 0; lv < nl; lv++) {
+    for (int idx = L.rlvl_start[lv] + w.lane; idx < L.rlvl_start[lv + 1]; idx += kWave) {
+      int ei = L.rlvl_order[idx];
+      for (int i = m.ent_dof_end[ei] - 1; i >= m.ent_dof_start[ei]; i--) {
+        float g = L.Mgrad[i];
+        for (int t = 0;; t++) {
+          int j = wave_col_candidate(w, i, ei, t);
+          if (j < 0) break;
+          g = g - e.H(j, i) * L.Mgrad[j];
+        }
+        L.Mgrad[i] = g / e.H(i, i);
+      }
     }
-    float v = g / e.H(i, i);
-    __syncthreads();
-    if (w.lane == 0) L.Mgrad[i] = v;
     __syncthreads();
   }
 }

@@ -50,7 +50,7 @@ MODEL_TABLES_F = [
 
 INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", "n_broad", "broad_pairs",
              "n_contacts", "contact_geoms", "active_buffer", "n_con", "jac_idx", "jac_nnz", "efc_active",
-             "solver_iters", "ent_row_ptr", "ent_row_list"}
+             "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent"}
 
 # Entity counts above this cannot use the wavefront solver's pattern bitsets.
 WAVE_MAX_ENTITIES = 128
@@ -81,7 +81,7 @@ def state_sizes(M: RigidModel) -> dict:
         n_con=3, jac_val=R * W, jac_idx=R * W, jac_nnz=R, efc_diag=R, efc_aref=R, efc_D=R, efc_fl=R,
         efc_force=R, efc_Jaref=R, efc_jv=R, efc_quad=3 * R, efc_active=R,
         qacc=D, Ma=D, grad=D, Mgrad=D, search=D, mv=D, qfrc_constraint=D, nt_vec=D, nt_H=D * (D + 1) // 2,
-        solver_iters=1, qpos_next=Q, vel_next=D, ent_row_ptr=M.n_entities + 1, ent_row_list=2 * R,
+        solver_iters=1, qpos_next=Q, vel_next=D, ent_row_ptr=M.n_entities + 1, ent_row_list=2 * R, efc_ent=R,
     )
     missing = set(_lib.STATE_POINTERS) - set(sz)
     assert not missing, f"state arrays without a size rule: {missing}"
@@ -177,6 +177,8 @@ class RigidSolver:
         self.n_envs = n_envs
         self._B = max(1, n_envs)
         self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("the rigid step runs on a HIP device; there is no CPU execution path")
         self._lib = _lib.load()
         self.dm = DeviceModel(model, self.device, solver_variant)
         self.st = DeviceState(self.dm, self._B)

@@ -114,6 +114,7 @@ typedef struct gsr_state {
   float *qacc, *Ma, *grad, *Mgrad, *search, *mv, *qfrc_constraint, *nt_vec, *nt_H;
   int32_t *solver_iters;
   int32_t *ent_row_ptr, *ent_row_list; /* per-entity ordered constraint-row lists (wavefront solver) */
+  int32_t *efc_ent;                    /* entity pair of each row (wavefront solver) */
   /* integration scratch */
   float *qpos_next, *vel_next;
 } gsr_state;
@@ -137,6 +138,44 @@ int gsr_launch_kernel(const gsr_model* model, const gsr_state* state, int32_t ke
 
 /* Stage runner for parity tests: 1 = kinematics, 2 = + smooth dynamics, 3 = + collision detection, 0 = full step. */
 int gsr_run_stage(const gsr_model* model, const gsr_state* state, int32_t stage, hipStream_t stream);
+
+/* ---- Batching form (no counterpart in the reference, whose batches are homogeneous) ----------------------------
+ * A heterogeneous batch is a list of groups: each group is one scene type (model) and a contiguous range of envs of
+ * one state. One persistent launch per substep steps every group: workgroups (one wavefront each) pull tiles of
+ * `envs_per_tile` envs from a device work queue, heaviest group first, and run the whole substep (step_1, collision,
+ * constraint rows, Newton solve, step_2) for their tile. The per-env arithmetic is the plain path's, so a result
+ * does not depend on what else is in the batch. */
+#define GSR_MAX_GROUPS 64
+
+typedef struct gsr_group {
+  gsr_model model;         /* by value; model.solver_variant selects the solver for this group */
+  gsr_state state;         /* by value */
+  int32_t env_begin;       /* first env of the group inside `state` */
+  int32_t n_envs;          /* envs of the group: [env_begin, env_begin + n_envs) */
+  int32_t envs_per_tile;   /* 1..64; the lane solver (variant 0) uses one lane per env of the tile */
+  int32_t tile_begin;      /* written by gsr_batch_upload: first tile index of the group */
+} gsr_group;
+
+typedef struct gsr_batch_info {
+  int32_t n_groups, n_tiles; /* groups and tiles in the batch */
+  int32_t grid;              /* persistent workgroups per launch (CUs x resident workgroups) */
+  int32_t lds_bytes;         /* dynamic LDS per workgroup (largest wavefront-solver group) */
+} gsr_batch_info;
+
+/* Device bytes the batch table of n_groups needs (the caller allocates it, 256-byte aligned, zero-filled once). */
+int64_t gsr_batch_table_bytes(int32_t n_groups);
+/* Validate groups, assign tiles and copy the table to `dev_table` (synchronously); fills `info`. Groups are
+ * stepped in the given order, so list the most expensive first. */
+int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table, int64_t table_bytes,
+                     gsr_batch_info* info);
+/* n_substeps fused launches of the uploaded batch on `stream`. */
+int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream);
+/* Capture n_substeps fused launches into a hipGraph (stream must not be the legacy default stream) and return the
+ * instantiated executable; launch it with gsr_graph_launch, free it with gsr_graph_destroy. */
+int gsr_graph_capture(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream,
+                      void** graph_exec);
+int gsr_graph_launch(void* graph_exec, hipStream_t stream);
+int gsr_graph_destroy(void* graph_exec);
 
 #ifdef __cplusplus
 }
