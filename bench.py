@@ -6,6 +6,11 @@ Workloads (seeded synthetic scenes, random-init state; no datasets):
              plane, 4096 envs, dt = 0.01                                    [default: the headline configuration]
   arm        9-DoF Panda arm + plane, 64 envs, dt = 0.01, PD targets U(limits) resampled every 100 steps
   quadruped  go2 quadruped + plane, 32768 envs, dt = 0.005, PD targets standing + U(-0.3, 0.3) every step
+  mixed      batching form: 16384 envs split (seed 3) among arm / quadruped / 20-box pile, dt = 0.01; times the plain
+             drop-in (one gsr_step per scene type) and the fused hipGraph-captured launch from the same state
+  serve      serving form: Poisson rollout requests (scene uniform over arm / quadruped / 20-box pile, n_envs
+             log-uniform in [1, 4096], horizon U{50..500}) served by continuous batching under a memory budget;
+             capacity measured saturated, then offered load swept 20-100 % of it; p50/p99 step and request latency
 """
 import argparse
 import json
@@ -41,45 +46,6 @@ def make(workload, n_envs):
     return w, compile_model(w.entities, w.options)
 
 
-class Control:
-    """Per-step PD targets of the workload, generated on the device from a seeded generator."""
-
-    def __init__(self, w, M, S, seed):
-        import torch
-        self.w, self.M, self.S = w, M, S
-        self.gen = torch.Generator(device=S.device).manual_seed(seed)
-        self.n_ctrl = 0
-        if w.name == "arm":
-            self.lo = torch.as_tensor(M.dof_limit[:, 0], device=S.device)
-            self.hi = torch.as_tensor(M.dof_limit[:, 1], device=S.device)
-            self.n_ctrl = M.n_dofs
-        elif w.name == "quadruped":
-            from genesis_amd.workloads import GO2_STANDING
-            self.stand = torch.as_tensor(GO2_STANDING, dtype=torch.float32, device=S.device)
-            self.dofs = torch.arange(6, 18, device=S.device)
-            self.n_ctrl = 12
-
-    def __call__(self, step):
-        import torch
-        B = self.S._B
-        if self.w.name == "arm" and step % 100 == 0:
-            u = torch.rand((B, self.M.n_dofs), generator=self.gen, device=self.S.device)
-            self.S.control_dofs_position(self.lo + (self.hi - self.lo) * u)
-        elif self.w.name == "quadruped":
-            u = torch.rand((B, 12), generator=self.gen, device=self.S.device)
-            self.S.control_dofs_position(self.stand + (u * 0.6 - 0.3), dofs_idx=self.dofs)
-
-
-def init_state(w, M, S):
-    import torch
-    from genesis_amd.workloads import GO2_INIT_BASE, GO2_STANDING
-    if w.name == "pile":
-        S.set_qpos(torch.as_tensor(w.init_qpos(S._B, M.n_qs), dtype=torch.float32))
-    elif w.name == "quadruped":
-        q = np.tile(np.concatenate([GO2_INIT_BASE, GO2_STANDING]), (S._B, 1))
-        S.set_qpos(torch.as_tensor(q, dtype=torch.float32))
-
-
 def cpu_baseline(w, M, seconds=15.0):
     """The CPU oracle (scalar restatement of the reference step, FP32) on the host cores, env-chunked threads."""
     from oracle.oracle import Oracle
@@ -101,12 +67,150 @@ def cpu_baseline(w, M, seconds=15.0):
             "sample": f"{n} envs x {steps} steps of the same workload from step 0 on the CPU oracle, {dt:.1f} s"}
 
 
+def bench_mixed(args):
+    import torch
+    from genesis_amd import batch as FB
+    from genesis_amd.model import compile_model
+    from genesis_amd.rigid_solver import RigidSolver
+    from genesis_amd.workloads import Control, init_state, mixed_workloads
+
+    n_total = args.n_envs or 16384
+    parts = []
+    for i, (w, n) in enumerate(mixed_workloads(n_total)):
+        if n == 0:
+            continue
+        M = compile_model(w.entities, w.options)
+        S = RigidSolver(M, n, "cuda")
+        init_state(w, M, S)
+        parts.append((w, M, S, Control(w, M, S, args.seed + i)))
+    step = 0
+
+    def controls():
+        for w, M, S, c in parts:
+            c(step)
+
+    for _ in range(args.warmup):
+        controls()
+        for w, M, S, c in parts:
+            S.substep()
+        step += 1
+    torch.cuda.synchronize()
+    snap = [FB._snapshot(S) for _, _, S, _ in parts]
+    step0 = step
+
+    def timed(run):
+        nonlocal step
+        step = step0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(args.steps):
+            controls()
+            run()
+            step += 1
+            if step % 10 == 0:
+                for _, _, S, _ in parts:
+                    S.check_errno()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def plain():
+        for _, _, S, _ in parts:
+            S._lib.gsr_step(S.dm.desc, S.st.desc, 1, S._stream())
+
+    t_plain = timed(plain)
+    for (_, _, S, _), sn in zip(parts, snap):
+        FB._restore(S, sn)
+    tuned, launch, report = FB.autotune([FB.Group(S) for _, _, S, _ in parts])
+    for (_, _, S, _), sn in zip(parts, snap):
+        FB._restore(S, sn)
+    fb = FB.FusedBatch(tuned, **launch)
+    fb.capture(1)
+    t_fused = timed(fb.replay)
+    B = sum(S._B for _, _, S, _ in parts)
+    v_plain, v_fused = B * args.steps / t_plain, B * args.steps / t_fused
+    names = {id(S): w.name for w, _, S, _ in parts}
+    out = {
+        "metric": "env-steps/s (fused batched vs plain drop-in, 1 GPU)",
+        "value": round(v_fused, 1), "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_fused / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scenes, random-init state)",
+        "config": {"workload": "mixed batch arm/quadruped/20-box pile", "n_envs": B, "dt": 0.01, "form": "fused",
+                   "split": {w.name: S._B for w, _, S, _ in parts},
+                   "tiles": [{"scene": names[id(g.solver)], "solver": ["lane-per-env", "wavefront-per-env"][g.solver_variant],
+                              "envs_per_tile": g.envs_per_tile} for g in tuned],
+                   "grid": fb.info.grid, "lds_bytes": fb.info.lds_bytes, "launch": launch},
+        "plain": {"value": round(v_plain, 1), "ms_per_step": round(1e3 * t_plain / args.steps, 4)},
+        "speedup_vs_plain": round(v_fused / v_plain, 3),
+        "autotune": [dict(r, scene=parts[r["group"]][0].name if r["group"] != "all" else "all", ms=round(r["ms"], 4))
+                     for r in report],
+    }
+    print(json.dumps(out))
+
+
+def bench_serve(args):
+    import numpy as np
+    import torch
+    from genesis_amd.batch import Group, autotune
+    from genesis_amd.serving import Pool, Server, poisson_requests
+
+    cap_envs = args.n_envs or 8192
+    pools = {s: Pool(s, cap_envs, dt=0.01) for s in ("arm", "quadruped", "pile20")}
+    # execution choices per scene type, tuned once on a 1024-env slice of each pool
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    log("serve: tuning tiles")
+    tuned, launch, _ = autotune([Group(p.S, 0, min(1024, cap_envs)) for p in pools.values()], steps=2, warmup=1)
+    for g in tuned:
+        for p in pools.values():
+            if p.S is g.solver:
+                p.envs_per_tile = g.envs_per_tile
+                p.S.dm.desc.solver_variant = g.solver_variant
+    budget = int(args.serve_budget_gb * 2**30)
+    server = Server(pools, budget, launch=launch)
+    sample = poisson_requests(1000.0, 100.0, seed=99)
+    mean_work = float(np.mean([r.n_envs * r.horizon for r in sample]))
+    # capacity: a saturated queue (every request waiting at t = 0)
+    sat = poisson_requests(1e9, 1.0, seed=5)[:args.serve_saturate]
+    for r in sat:
+        r.arrival = 0.0
+    log(f"serve: capacity run with {len(sat)} queued requests")
+    cap = server.run(sat)
+    capacity = cap["env_steps_per_s"]
+    log(f"serve: capacity {capacity:.0f} env-steps/s")
+    sweep = []
+    for load in (0.2, 0.4, 0.6, 0.8, 1.0):
+        rate = load * capacity / mean_work
+        reqs = poisson_requests(rate, args.serve_seconds, seed=4, start_id=0)
+        st = server.run(reqs)
+        st.update(load=load, offered_requests_per_s=rate, n_requests=len(reqs))
+        log(f"serve: load {load:.1f}: {len(reqs)} requests, p50 {st['step_ms_p50']:.3f} ms, p99 {st['step_ms_p99']:.3f} ms")
+        sweep.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()})
+    at80 = next(x for x in sweep if x["load"] == 0.8)
+    out = {
+        "metric": "served p50/p99 step latency and env-steps/s (1 GPU)",
+        "value": at80["env_steps_per_s"], "unit": "env-steps/s", "n_gpus": 1, "steps": at80["ticks"],
+        "warmup": 0, "ms_per_step": at80["step_ms_p50"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Poisson request stream, random-init state)",
+        "config": {"workload": "serving arm/quadruped/20-box pile requests", "pool_envs_per_scene": cap_envs,
+                   "memory_budget_gb": args.serve_budget_gb, "dt": 0.01, "form": "serving",
+                   "tiles": {p.scene: {"envs_per_tile": p.envs_per_tile, "solver": int(p.S.dm.desc.solver_variant)}
+                             for p in pools.values()}},
+        "step_ms_p50": at80["step_ms_p50"], "step_ms_p99": at80["step_ms_p99"],
+        "request_ms_p50": at80["request_ms_p50"], "request_ms_p99": at80["request_ms_p99"],
+        "capacity_env_steps_per_s": round(capacity, 1), "mean_request_env_steps": round(mean_work, 1),
+        "sweep": sweep,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="pile", choices=["pile", "arm", "quadruped"])
+    ap.add_argument("--workload", default="pile", choices=["pile", "arm", "quadruped", "mixed", "serve"])
+    ap.add_argument("--serve-seconds", type=float, default=6.0, help="arrival window per offered-load level")
+    ap.add_argument("--serve-budget-gb", type=float, default=12.0, help="device memory budget for admission")
+    ap.add_argument("--serve-saturate", type=int, default=24, help="requests queued at t=0 to measure capacity")
     ap.add_argument("--n-envs", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -116,8 +220,14 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
+    if args.workload == "mixed":
+        return bench_mixed(args)
+    if args.workload == "serve":
+        return bench_serve(args)
+
     import torch
     from genesis_amd.rigid_solver import RigidSolver
+    from genesis_amd.workloads import Control, init_state
 
     w, M = make(args.workload, args.n_envs)
     S = RigidSolver(M, w.n_envs if not args.n_envs else args.n_envs, "cuda", solver_variant=args.solver_variant)

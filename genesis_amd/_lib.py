@@ -14,15 +14,23 @@ ROOT = os.path.dirname(HERE)
 HEADER = os.path.join(ROOT, "include", "genesis_rigid.h")
 LIB_PATH = os.path.join(HERE, "_build", "libgenesis_rigid.so")
 
-EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_step", "gsr_forward_kinematics",
-           "gsr_reset_envs", "gsr_get_errno", "gsr_run_stage", "gsr_launch_kernel"]
+EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_group_size", "gsr_batch_info_size",
+           "gsr_step", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno", "gsr_run_stage", "gsr_launch_kernel", "gsr_batch_table_bytes",
+           "gsr_batch_upload", "gsr_step_fused", "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy"]
 
-_CTYPE = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "float": ctypes.c_float}
+_CTYPE = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "float": ctypes.c_float,
+          "int64_t": ctypes.c_int64}
+
+
+def _defines():
+    src = open(HEADER).read()
+    return {k: int(v) for k, v in re.findall(r"#define (\w+) (\d+)\b", src)}
 
 
 def _parse_struct(name: str):
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    consts = _defines()
     m = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), src, flags=re.S)
     body = m.group(1)
     fields = []
@@ -37,11 +45,15 @@ def _parse_struct(name: str):
             if var.startswith("*"):
                 fields.append((var[1:].strip(), ctypes.c_void_p))
             elif "[" in var:
-                vname, n = re.match(r"(\w+)\[(\d+)\]", var).groups()
-                fields.append((vname, _CTYPE[base] * int(n)))
+                vname, n = re.match(r"(\w+)\[(\w+)\]", var).groups()
+                n = int(n) if n.isdigit() else consts[n]
+                fields.append((vname, (_CTYPE[base] if base in _CTYPE else _STRUCTS[base]) * n))
             else:
-                fields.append((var, _CTYPE[base]))
+                fields.append((var, _CTYPE[base] if base in _CTYPE else _STRUCTS[base]))
     return fields
+
+
+_STRUCTS = {}
 
 
 class GsrModel(ctypes.Structure):
@@ -50,6 +62,17 @@ class GsrModel(ctypes.Structure):
 
 class GsrState(ctypes.Structure):
     _fields_ = _parse_struct("gsr_state")
+
+
+_STRUCTS.update(gsr_model=GsrModel, gsr_state=GsrState)
+
+
+class GsrGroup(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_group")
+
+
+class GsrBatchInfo(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_batch_info")
 
 
 MODEL_POINTERS = [n for n, t in GsrModel._fields_ if t is ctypes.c_void_p]
@@ -82,7 +105,17 @@ def load(path: str = LIB_PATH):
     L.gsr_get_errno.argtypes = [P(GsrState), ctypes.c_void_p, ctypes.c_void_p]
     L.gsr_run_stage.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
     L.gsr_launch_kernel.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
-    if L.gsr_model_size() != ctypes.sizeof(GsrModel) or L.gsr_state_size() != ctypes.sizeof(GsrState):
+    L.gsr_batch_table_bytes.restype = ctypes.c_int64
+    L.gsr_batch_table_bytes.argtypes = [ctypes.c_int32]
+    L.gsr_batch_upload.argtypes = [P(GsrGroup), ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, P(GsrBatchInfo)]
+    L.gsr_step_fused.argtypes = [ctypes.c_void_p, P(GsrBatchInfo), ctypes.c_int32, ctypes.c_void_p]
+    L.gsr_graph_capture.argtypes = [ctypes.c_void_p, P(GsrBatchInfo), ctypes.c_int32, ctypes.c_void_p,
+                                    P(ctypes.c_void_p)]
+    L.gsr_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.gsr_graph_destroy.argtypes = [ctypes.c_void_p]
+    if (L.gsr_model_size() != ctypes.sizeof(GsrModel) or L.gsr_state_size() != ctypes.sizeof(GsrState)
+            or L.gsr_group_size() != ctypes.sizeof(GsrGroup)
+            or L.gsr_batch_info_size() != ctypes.sizeof(GsrBatchInfo)):
         raise RuntimeError("gsr_model/gsr_state layout mismatch between include/genesis_rigid.h and the built library")
     _lib = L
     return L

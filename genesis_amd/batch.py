@@ -1,0 +1,191 @@
+"""Batching form: heterogeneous envs (different robots and DoF counts) stepped by one fused launch per substep.
+
+The reference batches only homogeneous envs (one model per scene; rigid_solver_decomp.py:80-352), so this form has
+no reference counterpart. It changes execution, not arithmetic: every env runs the same device code as in the plain
+drop-in (`gsr_step`), so with the same solver variant a fused result is bitwise the plain result.
+
+* `FusedBatch(groups)` uploads one device table (`gsr_batch_upload`) and steps every group with `gsr_step_fused`:
+  a persistent grid of one-wavefront workgroups pulls tiles of `envs_per_tile` envs, heaviest group first.
+* `capture()` records n substeps into a hipGraph (`gsr_graph_capture`); `replay()` launches it.
+* `autotune()` times each group alone under each candidate (solver variant, collision variant, envs per tile) on a
+  copy of its state, orders the winners by per-tile cost (longest first), then times the launch forms.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from .rigid_solver import WAVE_MAX_ENTITIES, RigidSolver
+
+
+@dataclass
+class Group:
+    """Envs [env_begin, env_begin + n_envs) of one solver, with the execution choices for the fused launch."""
+    solver: RigidSolver
+    env_begin: int = 0
+    n_envs: Optional[int] = None
+    envs_per_tile: int = 64
+    solver_variant: Optional[int] = None
+    collide_variant: Optional[int] = None
+
+    def resolved(self) -> "Group":
+        n = self.solver._B - self.env_begin if self.n_envs is None else self.n_envs
+        d = self.solver.dm.desc
+        v = int(d.solver_variant) if self.solver_variant is None else int(self.solver_variant)
+        c = int(d.collide_variant) if self.collide_variant is None else int(self.collide_variant)
+        return Group(self.solver, self.env_begin, n, self.envs_per_tile, v, c)
+
+
+class FusedBatch:
+    """One device table over several groups; `step()` runs fused substeps, `capture()`/`replay()` a hipGraph."""
+
+    def __init__(self, groups: Sequence[Group], persistent: bool = True, waves_per_simd: int = 4):
+        if not groups:
+            raise ValueError("empty batch")
+        self.lib = _lib.load()
+        self.groups = [g.resolved() for g in groups]
+        self.device = self.groups[0].solver.device
+        n = len(self.groups)
+        nbytes = int(self.lib.gsr_batch_table_bytes(n))
+        if nbytes < 0:
+            raise ValueError(f"a batch holds 1..64 groups, got {n}")
+        self.table = torch.zeros(nbytes, dtype=torch.uint8, device=self.device)
+        arr = (_lib.GsrGroup * n)()
+        for i, g in enumerate(self.groups):
+            m = _lib.GsrModel.from_buffer_copy(g.solver.dm.desc)
+            m.solver_variant = g.solver_variant
+            m.collide_variant = g.collide_variant
+            arr[i].model = m
+            arr[i].state = g.solver.st.desc
+            arr[i].env_begin, arr[i].n_envs, arr[i].envs_per_tile = g.env_begin, g.n_envs, g.envs_per_tile
+        self._arr = arr
+        self.info = _lib.GsrBatchInfo()
+        self.info.persistent = int(bool(persistent))
+        self.info.waves_per_simd = int(waves_per_simd)
+        torch.cuda.synchronize(self.device)  # the table's zero fill precedes the synchronous upload
+        _lib.check(self.lib.gsr_batch_upload(arr, n, ctypes.c_void_p(self.table.data_ptr()), nbytes,
+                                             ctypes.byref(self.info)), "gsr_batch_upload")
+        self._graphs = []
+        self._side = None
+
+    @property
+    def n_envs(self) -> int:
+        return sum(g.n_envs for g in self.groups)
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def step(self, n_substeps: int = 1):
+        _lib.check(self.lib.gsr_step_fused(ctypes.c_void_p(self.table.data_ptr()), ctypes.byref(self.info),
+                                           n_substeps, self._stream()), "gsr_step_fused")
+
+    def capture(self, n_substeps: int = 1):
+        """Record n_substeps fused launches into a hipGraph on a side stream; `replay()` launches the latest."""
+        if self._side is None:
+            self._side = torch.cuda.Stream(self.device)  # default priority
+        self._side.wait_stream(torch.cuda.current_stream(self.device))
+        exe = ctypes.c_void_p()
+        rc = self.lib.gsr_graph_capture(ctypes.c_void_p(self.table.data_ptr()), ctypes.byref(self.info), n_substeps,
+                                        ctypes.c_void_p(self._side.cuda_stream), ctypes.byref(exe))
+        _lib.check(rc, "gsr_graph_capture")
+        self._graphs.append(exe)
+        return exe
+
+    def replay(self, graph=None):
+        g = graph if graph is not None else self._graphs[-1]
+        _lib.check(self.lib.gsr_graph_launch(g, self._stream()), "gsr_graph_launch")
+
+    def check_errno(self):
+        for g in self.groups:
+            g.solver.check_errno()
+
+    def close(self):
+        for g in self._graphs:
+            self.lib.gsr_graph_destroy(g)
+        self._graphs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def candidates(solver: RigidSolver) -> List[Tuple[int, int, int]]:
+    """(solver_variant, collide_variant, envs_per_tile) triples worth timing for one scene type."""
+    M = solver.model
+    out = [(0, 0, 64), (0, 0, 32), (0, 1, 8)]
+    if M.n_entities <= WAVE_MAX_ENTITIES:
+        out += [(1, 1, t) for t in (1, 2, 4, 8)] + [(1, 0, 4)]
+    return out
+
+
+def _snapshot(solver: RigidSolver):
+    return {k: v.clone() for k, v in solver.st.t.items()}
+
+
+def _restore(solver: RigidSolver, snap):
+    for k, v in snap.items():
+        solver.st.t[k].copy_(v)
+
+
+def time_batch(groups: Sequence[Group], steps: int = 3, warmup: int = 1, **launch) -> float:
+    """Milliseconds per fused substep of `groups` (their solvers' states are restored afterwards)."""
+    solvers = list({id(g.solver): g.solver for g in groups}.values())
+    snaps = [_snapshot(S) for S in solvers]
+    fb = FusedBatch(groups, **launch)
+    for _ in range(warmup):
+        fb.step()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    fb.step(steps)
+    e1.record()
+    torch.cuda.synchronize(solvers[0].device)
+    for S, sn in zip(solvers, snaps):
+        _restore(S, sn)
+    return e0.elapsed_time(e1) / steps
+
+
+def time_group(group: Group, steps: int = 3, warmup: int = 1, **launch) -> float:
+    return time_batch([group], steps, warmup, **launch)
+
+
+LAUNCH_CANDIDATES = [dict(persistent=True, waves_per_simd=4), dict(persistent=False, waves_per_simd=4),
+                     dict(persistent=True, waves_per_simd=2), dict(persistent=False, waves_per_simd=2)]
+
+
+def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: bool = False):
+    """Pick (solver variant, envs per tile) per group by timing each group alone, order the groups by per-tile cost
+    (descending, so the longest tiles start first), then pick the launch form (persistent queue or one workgroup per
+    tile; 128 or 256 VGPRs) by timing the whole batch.
+
+    Returns (tuned groups, launch kwargs for FusedBatch, report of every candidate's ms per substep)."""
+    tuned, report = [], []
+    for gi, g in enumerate(groups):
+        g = g.resolved()
+        best = None
+        for v, c, t in candidates(g.solver):
+            ms = time_group(Group(g.solver, g.env_begin, g.n_envs, t, v, c), steps, warmup, **LAUNCH_CANDIDATES[0])
+            report.append({"group": gi, "solver": v, "collide": c, "envs_per_tile": t, "ms": ms})
+            if verbose:
+                print(f"  group {gi} solver {v} collide {c} tile {t}: {ms:.3f} ms", flush=True)
+            if best is None or ms < best[0]:
+                best = (ms, v, c, t)
+        ms, v, c, t = best
+        n_tiles = (g.n_envs + t - 1) // t
+        tuned.append((ms / n_tiles, Group(g.solver, g.env_begin, g.n_envs, t, v, c)))
+    tuned.sort(key=lambda x: -x[0])
+    tuned = [g for _, g in tuned]
+    best = None
+    for lc in LAUNCH_CANDIDATES:
+        ms = time_batch(tuned, steps, warmup, **lc)
+        report.append(dict(group="all", launch=lc, ms=ms))
+        if verbose:
+            print(f"  batch {lc}: {ms:.3f} ms", flush=True)
+        if best is None or ms < best[0]:
+            best = (ms, lc)
+    return tuned, best[1], report

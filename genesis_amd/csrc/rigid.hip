@@ -8,11 +8,16 @@
 //   k_step2      integration, NaN-guarded commit, end-of-step kinematics (kernel_step_2)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "genesis_rigid.h"
 #include "rigid_kin.h"
 #include "rigid_collide.h"
 #include "rigid_solver.h"
 #include "rigid_solver_wave.h"
+#include "rigid_collide_wave.h"
 
 using namespace gsr;
 
@@ -47,6 +52,14 @@ __global__ void __launch_bounds__(kBlock) k_collide(gsr_model m, gsr_state s) {
   dev_equality_rows(e);
   dev_collision(e);
   dev_inequality_rows(e);
+}
+
+__global__ void __launch_bounds__(kWave) k_collide_wave(gsr_model m, gsr_state s) {
+  extern __shared__ char lds[];
+  int b = blockIdx.x;
+  if (b >= s.n_envs) return;  // uniform across the block
+  Env e(m, s, b);
+  wave_collide(e, lds);
 }
 
 __global__ void __launch_bounds__(kBlock) k_solve(gsr_model m, gsr_state s) {
@@ -90,6 +103,139 @@ __global__ void __launch_bounds__(kBlock) k_reset(gsr_model m, gsr_state s, cons
   e.I((int32_t*)s.errno_env, 0) = 0;
 }
 
+
+// ---- batching form ----------------------------------------------------------------------------------------------
+// Device table of a heterogeneous batch: a header (work queue) then one record per group (env range). The model and
+// state descriptors of the batch's scene types are kernel arguments, so every field read is a scalar load from the
+// kernel-argument segment, as in the plain kernels.
+struct BatchHeader {
+  int32_t n_groups, n_tiles;
+  uint32_t counter, done;  // work queue: next tile, finished workgroups (the last one resets both)
+};
+struct BatchRange {
+  int32_t type, env_begin, n_envs, envs_per_tile, tile_begin, tile_end, pad0, pad1;
+};
+struct FusedTypes {
+  gsr_model m[GSR_MAX_TYPES];
+  gsr_state s[GSR_MAX_TYPES];
+};
+constexpr size_t kTypesOffset = 256;
+constexpr size_t kRangesOffset = (kTypesOffset + sizeof(FusedTypes) + 255) & ~(size_t)255;
+// The descriptors are read through the constant address space: every field read with a uniform type index is a
+// scalar, invariant load, like a kernel argument, without copying the descriptors anywhere.
+typedef const FusedTypes __attribute__((address_space(4))) ConstTypes;
+
+// Stages are separate (not inlined) functions so each gets its own register allocation.
+__device__ __noinline__ void tile_lane_stages_1(const Env& e) {
+  dev_smooth_dynamics(e);
+  dev_equality_rows(e);
+  dev_collision(e);
+  dev_inequality_rows(e);
+}
+__device__ __forceinline__ void tile_lane_dynamics(const Env& e) { dev_smooth_dynamics(e); }
+__device__ __forceinline__ void tile_lane_solve(const Env& e) { dev_solve(e); }
+__device__ __forceinline__ void tile_wave_collide(const Env& e, char* lds) { wave_collide(e, lds); }
+__device__ __forceinline__ void tile_wave_solve(const Env& e, char* lds) { wave_solve(e, lds); }
+__device__ __forceinline__ void tile_lane_stages_2(const Env& e) {
+  dev_integrate(e);
+  dev_kinematics(e);
+}
+
+// One substep of envs [first, first + n_here) of a group; the workgroup is one wavefront. Per-env stages run one
+// lane per env; the wavefront collision and solver variants take the tile's envs one after the other.
+__device__ __forceinline__ void run_tile(const gsr_model& m, const gsr_state& s, int env0, int n_here, char* lds) {
+  const int lane = threadIdx.x;
+  const bool mine = lane < n_here;
+  const bool cw = m.collide_variant == 1, sw = m.solver_variant == 1;
+  if (!cw && !sw) {
+    if (mine) {
+      Env e(m, s, env0 + lane);
+      dev_smooth_dynamics(e);
+      dev_equality_rows(e);
+      dev_collision(e);
+      dev_inequality_rows(e);
+      dev_solve(e);
+      dev_integrate(e);
+      dev_kinematics(e);
+    }
+    return;
+  }
+  if (mine) {
+    Env e(m, s, env0 + lane);
+    dev_smooth_dynamics(e);
+    if (!cw) {
+      dev_equality_rows(e);
+      dev_collision(e);
+      dev_inequality_rows(e);
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < n_here; j++) {
+    Env e(m, s, env0 + j);
+    if (cw) wave_collide(e, lds);
+    if (sw) wave_solve(e, lds);
+    __syncthreads();
+  }
+  if (mine) {
+    Env e(m, s, env0 + lane);
+    if (!sw) dev_solve(e);
+    dev_integrate(e);
+    dev_kinematics(e);
+  }
+}
+
+__device__ __forceinline__ void fused_tile(ConstTypes* types, const BatchRange* __restrict__ R, int n_groups,
+                                           int tile, char* lds) {
+  // the range holding the tile: binary search over the cumulative tile ends
+  int lo = 0, hi = n_groups - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (tile < R[mid].tile_end) hi = mid;
+    else lo = mid + 1;
+  }
+  const BatchRange g = R[lo];
+  const int T = g.envs_per_tile;
+  const int first = (tile - g.tile_begin) * T;
+  const int n_here = min(T, g.n_envs - first);
+  const int env0 = g.env_begin + first;
+  // the type index is uniform, so the descriptor fields are scalar loads from the kernel-argument segment
+  const int k = __builtin_amdgcn_readfirstlane(g.type);
+  const FusedTypes* ty = (const FusedTypes*)types;
+  run_tile(ty->m[k], ty->s[k], env0, n_here, lds);
+}
+
+// W = resident wavefronts per SIMD the register allocation must allow (2: 256 VGPRs, 4: 128 VGPRs)
+template <int W>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(W, 8)))
+k_fused(const char* __restrict__ table, uint32_t* queue, int persistent) {
+  extern __shared__ char lds[];
+  const BatchHeader* __restrict__ H = reinterpret_cast<const BatchHeader*>(table);
+  ConstTypes* types = (ConstTypes*)(table + kTypesOffset);
+  const BatchRange* __restrict__ R = reinterpret_cast<const BatchRange*>(table + kRangesOffset);
+  const int n_tiles = H->n_tiles, n_groups = H->n_groups;
+  if (!persistent) {
+    if ((int)blockIdx.x < n_tiles) fused_tile(types, R, n_groups, blockIdx.x, lds);
+    return;
+  }
+  for (;;) {
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(&queue[0], 1u);
+    const int tile = __builtin_amdgcn_readfirstlane((int)__shfl((int)t, 0, kWave));
+    if (tile >= n_tiles) break;
+    fused_tile(types, R, n_groups, tile, lds);
+    __syncthreads();
+  }
+  // the last workgroup out resets the queue for the next launch (no host memset, so graph replays stay valid)
+  if (threadIdx.x == 0) {
+    __threadfence();
+    uint32_t d = atomicAdd(&queue[1], 1u);
+    if (d == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
+    }
+  }
+}
+
 __global__ void k_errno_reduce(gsr_state s, uint32_t* out) {
   // one block of 256 threads ORs all env words
   __shared__ uint32_t part[256];
@@ -113,6 +259,11 @@ int check(const gsr_model* m, const gsr_state* s) {
     if (m->n_entities > kMaxPatternEntities) return GSR_E_SHAPE;
     if (wave_lds_bytes(m->n_dofs, m->n_entities) > 160 * 1024) return GSR_E_SHAPE;
   }
+  if (m->collide_variant == 1) {
+    if (m->n_geoms > kWaveMaxGeoms || m->n_contacts_per_pair > kMaxPairContacts) return GSR_E_SHAPE;
+  }
+  if (m->solver_variant < 0 || m->solver_variant > 1 || m->collide_variant < 0 || m->collide_variant > 1)
+    return GSR_E_ARG;
   return GSR_OK;
 }
 
@@ -129,7 +280,13 @@ void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
     // kinematics of the current pose are refreshed at the end of every step (and by gsr_forward_kinematics after a
     // pose write), so kernel_step_1 never needs to recompute them (enable_mujoco_compatibility = False)
     case 0: hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, st, *m, *s, 0); break;
-    case 1: hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, st, *m, *s); break;
+    case 1:
+      if (m->collide_variant == 1)
+        hipLaunchKernelGGL(k_collide_wave, dim3(s->n_envs), dim3(kWave), wave_collide_lds_bytes(m->n_geoms), st, *m,
+                           *s);
+      else
+        hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, st, *m, *s);
+      break;
     case 2:
       if (m->solver_variant == 1)
         hipLaunchKernelGGL(k_solve_wave, dim3(s->n_envs), dim3(kWave), wave_lds_bytes(m->n_dofs, m->n_entities), st,
@@ -153,6 +310,8 @@ extern "C" {
 int gsr_abi_version(void) { return GSR_ABI_VERSION; }
 int gsr_model_size(void) { return (int)sizeof(gsr_model); }
 int gsr_state_size(void) { return (int)sizeof(gsr_state); }
+int gsr_group_size(void) { return (int)sizeof(gsr_group); }
+int gsr_batch_info_size(void) { return (int)sizeof(gsr_batch_info); }
 
 int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream) {
   int rc = check(model, state);
@@ -206,8 +365,127 @@ int gsr_run_stage(const gsr_model* model, const gsr_state* state, int32_t stage,
   if (stage == 0) return run_substep(model, state, stream);
   hipLaunchKernelGGL(k_kinematics, g, dim3(kBlock), 0, stream, *model, *state, (const int32_t*)nullptr, state->n_envs);
   if (stage >= 2) hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, stream, *model, *state, 0);
-  if (stage >= 3) hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, stream, *model, *state);
+  if (stage >= 3) launch_one(model, state, 1, stream);
   return launch_status();
+}
+
+int64_t gsr_batch_table_bytes(int32_t n_groups) {
+  if (n_groups < 1 || n_groups > GSR_MAX_GROUPS) return GSR_E_ARG;
+  return (int64_t)(kRangesOffset + sizeof(BatchRange) * (size_t)n_groups);
+}
+
+int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table, int64_t table_bytes,
+                     gsr_batch_info* info) {
+  if (!groups || !dev_table || !info || n_groups < 1 || n_groups > GSR_MAX_GROUPS) return GSR_E_ARG;
+  if (table_bytes < gsr_batch_table_bytes(n_groups)) return GSR_E_ARG;
+  std::vector<BatchRange> ranges(n_groups);
+  gsr_batch_info out;
+  std::memset(&out, 0, sizeof(out));
+  int tiles = 0;
+  size_t lds = 0;
+  for (int g = 0; g < n_groups; g++) {
+    const gsr_group& G = groups[g];
+    int rc = check(&G.model, &G.state);
+    if (rc) return rc;
+    if (G.n_envs < 1 || G.env_begin < 0 || G.env_begin + G.n_envs > G.state.n_envs) return GSR_E_ARG;
+    if (G.envs_per_tile < 1 || G.envs_per_tile > kWave) return GSR_E_ARG;
+    int type = -1;
+    for (int k = 0; k < out.n_types; k++)
+      if (!std::memcmp(&out.models[k], &G.model, sizeof(gsr_model)) &&
+          !std::memcmp(&out.states[k], &G.state, sizeof(gsr_state)))
+        type = k;
+    if (type < 0) {
+      if (out.n_types == GSR_MAX_TYPES) return GSR_E_SHAPE;
+      type = out.n_types++;
+      out.models[type] = G.model;
+      out.states[type] = G.state;
+    }
+    if (G.model.solver_variant == 1) lds = std::max(lds, wave_lds_bytes(G.model.n_dofs, G.model.n_entities));
+    if (G.model.collide_variant == 1) lds = std::max(lds, wave_collide_lds_bytes(G.model.n_geoms));
+    int n_t = (G.n_envs + G.envs_per_tile - 1) / G.envs_per_tile;
+    ranges[g] = BatchRange{type, G.env_begin, G.n_envs, G.envs_per_tile, tiles, tiles + n_t, 0, 0};
+    tiles += n_t;
+  }
+  BatchHeader h{n_groups, tiles, 0u, 0u};
+  FusedTypes ty;
+  std::memset(&ty, 0, sizeof(ty));
+  for (int k = 0; k < out.n_types; k++) {
+    ty.m[k] = out.models[k];
+    ty.s[k] = out.states[k];
+  }
+  // the queue words stay as they are on the device: zero-filled by the caller once, reset by every launch
+  if (hipMemcpy(static_cast<char*>(dev_table) + kRangesOffset, ranges.data(), sizeof(BatchRange) * ranges.size(),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return GSR_E_LAUNCH;
+  if (hipMemcpy(static_cast<char*>(dev_table) + kTypesOffset, &ty, sizeof(ty), hipMemcpyHostToDevice) != hipSuccess)
+    return GSR_E_LAUNCH;
+  if (hipMemcpy(dev_table, &h, 2 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) return GSR_E_LAUNCH;
+  int dev = 0, n_cu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return GSR_E_LAUNCH;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return GSR_E_LAUNCH;
+  const int W = info->waves_per_simd == 4 ? 4 : 2;
+  const void* fn = W == 4 ? (const void*)k_fused<4> : (const void*)k_fused<2>;
+  if (lds > 0 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return GSR_E_LAUNCH;
+  hipError_t oe = W == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused<4>, kWave, lds)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused<2>, kWave, lds);
+  if (oe != hipSuccess) return GSR_E_LAUNCH;
+  out.n_groups = n_groups;
+  out.n_tiles = tiles;
+  out.persistent = info->persistent ? 1 : 0;
+  out.waves_per_simd = W;
+  out.grid = out.persistent ? std::max(1, std::min(tiles, n_cu * std::max(per_cu, 1))) : tiles;
+  out.lds_bytes = (int32_t)lds;
+  *info = out;
+  return GSR_OK;
+}
+
+int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream) {
+  if (!dev_table || !info || n_substeps < 1 || info->grid < 1 || info->n_tiles < 1 || info->n_types < 1 ||
+      info->n_types > GSR_MAX_TYPES)
+    return GSR_E_ARG;
+  const char* t = static_cast<const char*>(dev_table);
+  uint32_t* queue = reinterpret_cast<uint32_t*>(const_cast<char*>(t) + 2 * sizeof(int32_t));
+  for (int i = 0; i < n_substeps; i++) {
+    if (info->waves_per_simd == 4)
+      hipLaunchKernelGGL(k_fused<4>, dim3(info->grid), dim3(kWave), info->lds_bytes, stream, t, queue,
+                         (int)info->persistent);
+    else
+      hipLaunchKernelGGL(k_fused<2>, dim3(info->grid), dim3(kWave), info->lds_bytes, stream, t, queue,
+                         (int)info->persistent);
+  }
+  return launch_status();
+}
+
+int gsr_graph_capture(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream,
+                      void** graph_exec) {
+  if (!graph_exec || !stream) return GSR_E_ARG;
+  *graph_exec = nullptr;
+  if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return GSR_E_LAUNCH;
+  int rc = gsr_step_fused(dev_table, info, n_substeps, stream);
+  hipGraph_t graph = nullptr;
+  hipError_t err = hipStreamEndCapture(stream, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (err != hipSuccess || !graph) return GSR_E_LAUNCH;
+  hipGraphExec_t exec = nullptr;
+  err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (err != hipSuccess) return GSR_E_LAUNCH;
+  *graph_exec = exec;
+  return GSR_OK;
+}
+
+int gsr_graph_launch(void* graph_exec, hipStream_t stream) {
+  if (!graph_exec) return GSR_E_ARG;
+  return hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), stream) == hipSuccess ? GSR_OK : GSR_E_LAUNCH;
+}
+
+int gsr_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return GSR_E_ARG;
+  return hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)) == hipSuccess ? GSR_OK : GSR_E_LAUNCH;
 }
 
 }  // extern "C"

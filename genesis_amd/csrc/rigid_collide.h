@@ -126,6 +126,35 @@ __device__ inline void dev_add_contact(const Env& e, int ga, int gb, f3 normal, 
   }
 }
 
+// Where the narrowphase puts the contacts of one pair. GlobalSink appends to the env's contact list at once (one
+// lane per env); LocalSink keeps them in the lane until the wavefront has counted every pair's contacts.
+struct GlobalSink {
+  const Env& e;
+  __device__ void add(int ga, int gb, f3 n, f3 p, float pen) { dev_add_contact(e, ga, gb, n, p, pen); }
+  // position of the k-th most recent contact (k = 0: the last one)
+  __device__ f3 recent_pos(int k) const { return e.L3(e.s.contact_pos, e.I(e.s.n_contacts, 0) - 1 - k); }
+  __device__ void flag(uint32_t bits) { e.I((int32_t*)e.s.errno_env, 0) |= (int)bits; }
+};
+
+constexpr int kMaxPairContacts = 8;
+
+struct LocalSink {
+  f3 n[kMaxPairContacts], p[kMaxPairContacts];
+  float pen[kMaxPairContacts];
+  int cnt;
+  uint32_t err;
+  __device__ void add(int, int, f3 nn, f3 pp, float pe) {
+    if (cnt < kMaxPairContacts) {
+      n[cnt] = nn;
+      p[cnt] = pp;
+      pen[cnt] = pe;
+      cnt++;
+    }
+  }
+  __device__ f3 recent_pos(int k) const { return p[cnt - 1 - k]; }
+  __device__ void flag(uint32_t bits) { err |= bits; }
+};
+
 __device__ inline float dev_tolerance(const gsr_model& m, int ga, int gb, float tol) {
   float sb = norm(T3(m.geom_init_aabb, 8 * gb + 7) - T3(m.geom_init_aabb, 8 * gb));
   float sz = sb;
@@ -367,7 +396,8 @@ __device__ inline void dev_contact_orthogonals(const Env& e, int ga, int gb, f3 
 }
 
 // func_convex_convex_contact (collider_decomp.py:2328-2725); MPR path with multi-contact
-__device__ inline void dev_convex_convex(const Env& e, int ga, int gb) {
+template <class Sink>
+__device__ inline void dev_convex_convex(const Env& e, int ga, int gb, Sink& snk) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const float EPS = m.eps;
@@ -416,14 +446,14 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb) {
         }
         if (pen > tol) {
           bool prefer_gjk = !guess || (m.mc_tolerance * pen >= m.mpr_to_gjk * tol);
-          if (prefer_gjk) e.I((int32_t*)s.errno_env, 0) |= (int)GSR_ERR_DEEP_PENETRATION;
+          if (prefer_gjk) snk.flag(GSR_ERR_DEEP_PENETRATION);
         }
       }
     }
     if (det == 0) {
       is_col0 = is_col; n0 = normal; pen0 = pen; cp0 = cpos;
       if (is_col0) {
-        dev_add_contact(e, ga, gb, n0, cp0, pen0);
+        snk.add(ga, gb, n0, cp0, pen0);
         if (multi) {
           dev_contact_orthogonals(e, ga, gb, normal, ax0, ax1);
           n_con = 1;
@@ -442,23 +472,22 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb) {
       normal = normal + cross(tw, normal);
       pen = dot(normal, cpb - cpa);
       bool repeated = false;
-      int nc = e.I(s.n_contacts, 0);
       for (int ic = 0; ic < n_con; ic++) {
         if (!repeated) {
-          int idx = nc - 1 - ic;
-          if (norm(cpos - e.L3(s.contact_pos, idx)) < tol) repeated = true;
+          if (norm(cpos - snk.recent_pos(ic)) < tol) repeated = true;
         }
       }
       if (!repeated && pen > -tol) {
         pen = fmaxf(pen, 0.f);
-        dev_add_contact(e, ga, gb, normal, cpos, pen);
+        snk.add(ga, gb, normal, cpos, pen);
         n_con++;
       }
     }
   }
 }
 
-__device__ inline void dev_plane_box(const Env& e, int ga, int gb) {
+template <class Sink>
+__device__ inline void dev_plane_box(const Env& e, int ga, int gb, Sink& snk) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   f3 gap = e.L3(s.geoms_pos, ga);
@@ -470,7 +499,7 @@ __device__ inline void dev_plane_box(const Env& e, int ga, int gb) {
   float pen = dot(normal, v1 - gap);
   if (pen > 0.f) {
     f3 cpos = v1 - 0.5f * pen * normal;
-    dev_add_contact(e, ga, gb, normal, cpos, pen);
+    snk.add(ga, gb, normal, cpos, pen);
     if (m.enable_multi_contact) {
       int n_con = 1;
       f3 cp0 = cpos;
@@ -482,7 +511,7 @@ __device__ inline void dev_plane_box(const Env& e, int ga, int gb) {
           if (pen > 0.f) {
             cpos = pc - 0.5f * pen * normal;
             if (norm(cpos - cp0) > tol) {
-              dev_add_contact(e, ga, gb, normal, cpos, pen);
+              snk.add(ga, gb, normal, cpos, pen);
               n_con++;
             }
           }
@@ -500,13 +529,15 @@ __device__ inline void dev_narrow_phase(const Env& e) {
     int p = e.I(s.broad_pairs, ip);
     int ga = p >> 16, gb = p & 0xffff;
     if (m.geom_type[ga] > m.geom_type[gb]) { int t = ga; ga = gb; gb = t; }
-    if (!(m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX)) dev_convex_convex(e, ga, gb);
+    GlobalSink snk{e};
+    if (!(m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX)) dev_convex_convex(e, ga, gb, snk);
   }
   for (int ip = 0; ip < nb; ip++) {
     int p = e.I(s.broad_pairs, ip);
     int ga = p >> 16, gb = p & 0xffff;
     if (m.geom_type[ga] > m.geom_type[gb]) { int t = ga; ga = gb; gb = t; }
-    if (m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX) dev_plane_box(e, ga, gb);
+    GlobalSink snk{e};
+    if (m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX) dev_plane_box(e, ga, gb, snk);
   }
 }
 

@@ -96,12 +96,11 @@ __device__ inline void dev_equality_rows(const Env& e) {
   e.I(s.n_con, 1) = n_eq;
 }
 
-__device__ inline void dev_inequality_rows(const Env& e) {
+// frictionloss rows (add_frictionloss_constraints), appended from row n_con; returns the new row count
+__device__ inline int dev_frictionloss_rows(const Env& e, int n_con, int& n_fl) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
-  int n_con = e.I(s.n_con, 0);
-  int n_fl = 0;
-  // frictionloss
+  n_fl = 0;
   for (int l = 0; l < m.n_links; l++)
     for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++)
       for (int d = m.joint_dof_start[j]; d < m.joint_dof_end[j]; d++) {
@@ -120,81 +119,99 @@ __device__ inline void dev_inequality_rows(const Env& e) {
           row_store(e, c, r);
         }
       }
-  // contacts: 4 pyramid rows each
-  if (m.enable_collision) {
-    int nc = e.I(s.n_contacts, 0);
-    for (int ic = 0; ic < nc; ic++) {
-      int ga = e.I(s.contact_geoms, 2 * ic), gb = e.I(s.contact_geoms, 2 * ic + 1);
-      int la = m.geom_link[ga], lb = m.geom_link[gb];
-      f3 cn = e.L3(s.contact_normal, ic), cp = e.L3(s.contact_pos, ic);
-      float fr = e.F(s.contact_friction, ic), pen = e.F(s.contact_pen, ic);
-      float sp[7];
-      for (int k = 0; k < 7; k++) sp[k] = 0.5f * (m.geom_sol_params[7 * ga + k] + m.geom_sol_params[7 * gb + k]);
-      f3 d1, d2;
-      orthogonals(cn, d1, d2);
-      float invw = m.link_invweight[2 * la];
-      if (lb > -1) invw = invw + m.link_invweight[2 * lb];
-      for (int i = 0; i < 4; i++) {
-        f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
-        f3 n = dd * fr - cn;
-        int c = n_con++;
-        RowBuf r;
-        r.n = 0;
-        float jac_qvel = 0.f;
-        for (int iab = 0; iab < 2; iab++) {
-          float sign = iab == 0 ? -1.f : 1.f;
-          int link = iab == 0 ? la : lb;
-          while (link > -1) {
-            int de = m.link_dof_end[link], nd = de - m.link_dof_start[link];
-            f3 tpos = cp - e.L3(s.root_com, link);
-            for (int k = 0; k < nd; k++) {
-              int d = de - 1 - k;
-              f3 vel = qrot(e.L3(s.cdof_vel, d) - cross(tpos, e.L3(s.cdof_ang, d)), qid());
-              float jac = dot(sign * vel, n);
-              jac_qvel = jac_qvel + jac * e.F(s.vel, d);
-              row_add(r, d, jac);
-            }
-            link = m.link_parent[link];
-          }
+  return n_con;
+}
+
+// the 4 pyramid rows of contact ic, stored at rows c0 .. c0 + 3 (add_collision_constraints)
+__device__ inline void dev_contact_rows(const Env& e, int ic, int c0) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  int ga = e.I(s.contact_geoms, 2 * ic), gb = e.I(s.contact_geoms, 2 * ic + 1);
+  int la = m.geom_link[ga], lb = m.geom_link[gb];
+  f3 cn = e.L3(s.contact_normal, ic), cp = e.L3(s.contact_pos, ic);
+  float fr = e.F(s.contact_friction, ic), pen = e.F(s.contact_pen, ic);
+  float sp[7];
+  for (int k = 0; k < 7; k++) sp[k] = 0.5f * (m.geom_sol_params[7 * ga + k] + m.geom_sol_params[7 * gb + k]);
+  f3 d1, d2;
+  orthogonals(cn, d1, d2);
+  float invw = m.link_invweight[2 * la];
+  if (lb > -1) invw = invw + m.link_invweight[2 * lb];
+  for (int i = 0; i < 4; i++) {
+    f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
+    f3 n = dd * fr - cn;
+    int c = c0 + i;
+    RowBuf r;
+    r.n = 0;
+    float jac_qvel = 0.f;
+    for (int iab = 0; iab < 2; iab++) {
+      float sign = iab == 0 ? -1.f : 1.f;
+      int link = iab == 0 ? la : lb;
+      while (link > -1) {
+        int de = m.link_dof_end[link], nd = de - m.link_dof_start[link];
+        f3 tpos = cp - e.L3(s.root_com, link);
+        for (int k = 0; k < nd; k++) {
+          int d = de - 1 - k;
+          f3 vel = qrot(e.L3(s.cdof_vel, d) - cross(tpos, e.L3(s.cdof_ang, d)), qid());
+          float jac = dot(sign * vel, n);
+          jac_qvel = jac_qvel + jac * e.F(s.vel, d);
+          row_add(r, d, jac);
         }
-        row_sort(r);
-        row_store(e, c, r);
+        link = m.link_parent[link];
+      }
+    }
+    row_sort(r);
+    row_store(e, c, r);
+    float imp, aref;
+    imp_aref(sp, -pen, jac_qvel, -pen, imp, aref);
+    float diag = invw + fr * fr * invw;
+    diag *= 2.f * fr * fr * (1.f - imp) / imp;
+    diag = fmaxf(diag, m.eps);
+    e.EF(s.efc_diag, c) = diag;
+    e.EF(s.efc_aref, c) = aref;
+    e.EF(s.efc_D, c) = 1.f / diag;
+  }
+}
+
+// joint-limit rows (add_joint_limit_constraints), appended from row n_con; returns the new row count
+__device__ inline int dev_limit_rows(const Env& e, int n_con) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  for (int l = 0; l < m.n_links; l++)
+    for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
+      int jt = m.joint_type[j];
+      if (jt != JOINT_REVOLUTE && jt != JOINT_PRISMATIC) continue;
+      int q = m.joint_q_start[j], d = m.joint_dof_start[j];
+      float qv = e.F(s.qpos, q);
+      float dmin = qv - m.dof_limit[2 * d], dmax = m.dof_limit[2 * d + 1] - qv;
+      float pd = fminf(dmin, dmax);
+      if (pd < 0.f) {
+        float jac = (dmin < dmax) ? 1.f : -1.f;
         float imp, aref;
-        imp_aref(sp, -pen, jac_qvel, -pen, imp, aref);
-        float diag = invw + fr * fr * invw;
-        diag *= 2.f * fr * fr * (1.f - imp) / imp;
-        diag = fmaxf(diag, m.eps);
+        imp_aref(m.joint_sol_params + 7 * j, pd, jac * e.F(s.vel, d), pd, imp, aref);
+        float diag = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
+        int c = n_con++;
         e.EF(s.efc_diag, c) = diag;
         e.EF(s.efc_aref, c) = aref;
         e.EF(s.efc_D, c) = 1.f / diag;
+        RowBuf r;
+        r.n = 1; r.idx[0] = d; r.val[0] = jac;
+        row_store(e, c, r);
       }
     }
+  return n_con;
+}
+
+__device__ inline void dev_inequality_rows(const Env& e) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  int n_fl = 0;
+  int n_con = dev_frictionloss_rows(e, e.I(s.n_con, 0), n_fl);
+  if (m.enable_collision) {
+    int nc = e.I(s.n_contacts, 0);
+    for (int ic = 0; ic < nc; ic++) dev_contact_rows(e, ic, n_con + 4 * ic);
+    n_con += 4 * nc;
   }
-  // joint limits
-  if (m.enable_joint_limit) {
-    for (int l = 0; l < m.n_links; l++)
-      for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
-        int jt = m.joint_type[j];
-        if (jt != JOINT_REVOLUTE && jt != JOINT_PRISMATIC) continue;
-        int q = m.joint_q_start[j], d = m.joint_dof_start[j];
-        float qv = e.F(s.qpos, q);
-        float dmin = qv - m.dof_limit[2 * d], dmax = m.dof_limit[2 * d + 1] - qv;
-        float pd = fminf(dmin, dmax);
-        if (pd < 0.f) {
-          float jac = (dmin < dmax) ? 1.f : -1.f;
-          float imp, aref;
-          imp_aref(m.joint_sol_params + 7 * j, pd, jac * e.F(s.vel, d), pd, imp, aref);
-          float diag = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
-          int c = n_con++;
-          e.EF(s.efc_diag, c) = diag;
-          e.EF(s.efc_aref, c) = aref;
-          e.EF(s.efc_D, c) = 1.f / diag;
-          RowBuf r;
-          r.n = 1; r.idx[0] = d; r.val[0] = jac;
-          row_store(e, c, r);
-        }
-      }
-  }
+  if (m.enable_joint_limit) n_con = dev_limit_rows(e, n_con);
   e.I(s.n_con, 0) = n_con;
   e.I(s.n_con, 2) = n_fl;
 }

@@ -85,8 +85,8 @@ def arm_workload(n_envs=64, n_steps=1000, seed=0) -> Workload:
     return Workload("arm", [plane_entity(), panda_entity()], opts, n_envs, n_steps)
 
 
-def quadruped_workload(n_envs=32768, n_steps=1000, seed=2) -> Workload:
-    opts = CompileOptions(dt=0.005)
+def quadruped_workload(n_envs=32768, n_steps=1000, seed=2, dt=0.005) -> Workload:
+    opts = CompileOptions(dt=dt)
     return Workload("quadruped", [plane_entity(), go2_entity()], opts, n_envs, n_steps)
 
 
@@ -114,3 +114,79 @@ def pile_workload(n_boxes=64, n_envs=4096, n_steps=500, seed=1, max_collision_pa
         return q
 
     return Workload("pile", ents, opts, n_envs, n_steps, init_qpos=init_qpos)
+
+
+MIXED_TYPES = ("arm", "quadruped", "pile20")
+
+
+def mixed_workloads(n_envs=16384, seed=3, dt=0.01):
+    """Heterogeneous batch: a seeded multinomial split of n_envs among the arm, the quadruped and a 20-box pile,
+    all at the same dt. Returns [(Workload, count)] in MIXED_TYPES order."""
+    import torch
+    g = torch.Generator().manual_seed(seed)
+    pick = torch.multinomial(torch.ones(3), n_envs, replacement=True, generator=g)
+    counts = torch.bincount(pick, minlength=3).tolist()
+    ws = [arm_workload(n_envs=counts[0]), quadruped_workload(n_envs=counts[1], dt=dt),
+          pile_workload(n_boxes=20, n_envs=counts[2])]
+    ws[2].name = "pile20"
+    return list(zip(ws, counts))
+
+
+def workload_by_name(name: str, n_envs: int, dt: float = None) -> Workload:
+    if name == "arm":
+        return arm_workload(n_envs=n_envs)
+    if name == "quadruped":
+        return quadruped_workload(n_envs=n_envs, **({} if dt is None else {"dt": dt}))
+    if name == "pile":
+        return pile_workload(n_envs=n_envs)
+    if name == "pile20":
+        w = pile_workload(n_boxes=20, n_envs=n_envs)
+        w.name = "pile20"
+        return w
+    raise ValueError(f"unknown workload {name}")
+
+
+class Control:
+    """Per-step PD targets of a workload for envs [env_begin, env_begin + n_envs) of a solver, drawn on the device
+    from a seeded generator: arm targets U(joint limits) resampled every 100 steps; quadruped targets = standing
+    pose + U(-0.3, 0.3) every step; piles are uncontrolled."""
+
+    def __init__(self, w: Workload, M, S, seed, env_begin=0, n_envs=None):
+        import torch
+        self.w, self.M, self.S = w, M, S
+        self.b0 = env_begin
+        self.n = S._B - env_begin if n_envs is None else n_envs
+        self.envs = torch.arange(self.b0, self.b0 + self.n, device=S.device)
+        self.gen = torch.Generator(device=S.device).manual_seed(seed)
+        self.n_ctrl = 0
+        if w.name == "arm":
+            self.lo = torch.as_tensor(M.dof_limit[:, 0], device=S.device)
+            self.hi = torch.as_tensor(M.dof_limit[:, 1], device=S.device)
+            self.n_ctrl = M.n_dofs
+        elif w.name == "quadruped":
+            self.stand = torch.as_tensor(GO2_STANDING, dtype=torch.float32, device=S.device)
+            self.dofs = torch.arange(6, 18, device=S.device)
+            self.n_ctrl = 12
+
+    def __call__(self, step):
+        import torch
+        if self.w.name == "arm" and step % 100 == 0:
+            u = torch.rand((self.n, self.M.n_dofs), generator=self.gen, device=self.S.device)
+            self.S.control_dofs_position(self.lo + (self.hi - self.lo) * u, envs_idx=self.envs)
+        elif self.w.name == "quadruped":
+            u = torch.rand((self.n, 12), generator=self.gen, device=self.S.device)
+            self.S.control_dofs_position(self.stand + (u * 0.6 - 0.3), dofs_idx=self.dofs, envs_idx=self.envs)
+
+
+def init_state(w: Workload, M, S, env_begin=0, n_envs=None):
+    """Initial poses of the workload for envs [env_begin, env_begin + n_envs) (qpos0 unless the workload says)."""
+    import torch
+    n = S._B - env_begin if n_envs is None else n_envs
+    envs = torch.arange(env_begin, env_begin + n, device=S.device)
+    if w.init_qpos is not None:
+        S.set_qpos(torch.as_tensor(w.init_qpos(n, M.n_qs), dtype=torch.float32), envs_idx=envs)
+    elif w.name == "quadruped":
+        q = np.tile(np.concatenate([GO2_INIT_BASE, GO2_STANDING]), (n, 1))
+        S.set_qpos(torch.as_tensor(q, dtype=torch.float32), envs_idx=envs)
+    else:
+        S.set_qpos(torch.as_tensor(np.tile(M.qpos0, (n, 1)), dtype=torch.float32), envs_idx=envs)

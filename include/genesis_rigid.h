@@ -53,7 +53,8 @@ typedef struct gsr_model {
   int32_t max_contacts, max_broad, max_rows, row_width, mass_blk_size, max_entity_dofs;
   int32_t iterations, ls_iterations, n_contacts_per_pair;
   int32_t enable_collision, enable_joint_limit, enable_multi_contact;
-  int32_t solver_variant; /* 0: one lane per env (rows env-minor), 1: one wavefront per env (rows env-major) */
+  int32_t solver_variant;  /* 0: one lane per env (rows env-minor), 1: one wavefront per env (rows env-major) */
+  int32_t collide_variant; /* 0: one lane per env, 1: one wavefront per env (pairs across lanes) */
   float dt, tolerance, ls_tolerance, eps, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
   float gravity[4];
   /* links */
@@ -120,9 +121,11 @@ typedef struct gsr_state {
 } gsr_state;
 
 int gsr_abi_version(void);
-/* sizeof(gsr_model) and sizeof(gsr_state), for binding checks */
+/* sizeof(gsr_model), sizeof(gsr_state), sizeof(gsr_group), sizeof(gsr_batch_info), for binding checks */
 int gsr_model_size(void);
 int gsr_state_size(void);
+int gsr_group_size(void);
+int gsr_batch_info_size(void);
 
 int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream);
 int gsr_forward_kinematics(const gsr_model* model, const gsr_state* state, const int32_t* envs_idx, int32_t n_idx,
@@ -145,27 +148,35 @@ int gsr_run_stage(const gsr_model* model, const gsr_state* state, int32_t stage,
  * `envs_per_tile` envs from a device work queue, heaviest group first, and run the whole substep (step_1, collision,
  * constraint rows, Newton solve, step_2) for their tile. The per-env arithmetic is the plain path's, so a result
  * does not depend on what else is in the batch. */
-#define GSR_MAX_GROUPS 64
+#define GSR_MAX_GROUPS 4096 /* env ranges per batch */
+#define GSR_MAX_TYPES 4     /* distinct (model, state) pairs per batch: their descriptors travel as kernel arguments */
 
 typedef struct gsr_group {
-  gsr_model model;         /* by value; model.solver_variant selects the solver for this group */
+  gsr_model model;         /* by value; model.solver_variant / collide_variant select the per-env code paths */
   gsr_state state;         /* by value */
   int32_t env_begin;       /* first env of the group inside `state` */
   int32_t n_envs;          /* envs of the group: [env_begin, env_begin + n_envs) */
-  int32_t envs_per_tile;   /* 1..64; the lane solver (variant 0) uses one lane per env of the tile */
+  int32_t envs_per_tile;   /* 1..64; the per-lane stages use one lane per env of the tile */
   int32_t tile_begin;      /* written by gsr_batch_upload: first tile index of the group */
 } gsr_group;
 
 typedef struct gsr_batch_info {
   int32_t n_groups, n_tiles; /* groups and tiles in the batch */
-  int32_t grid;              /* persistent workgroups per launch (CUs x resident workgroups) */
-  int32_t lds_bytes;         /* dynamic LDS per workgroup (largest wavefront-solver group) */
+  int32_t grid;              /* workgroups per launch */
+  int32_t lds_bytes;         /* dynamic LDS per workgroup (largest wavefront-variant group) */
+  int32_t n_types;           /* distinct (model, state) pairs, in first-appearance order */
+  int32_t persistent;        /* input: 1 = CUs x resident workgroups pull tiles from a queue, 0 = one per tile */
+  int32_t waves_per_simd;    /* input: register budget of the fused kernel, 2 (256 VGPRs) or 4 (128 VGPRs) */
+  int32_t pad;
+  gsr_model models[GSR_MAX_TYPES];
+  gsr_state states[GSR_MAX_TYPES];
 } gsr_batch_info;
 
 /* Device bytes the batch table of n_groups needs (the caller allocates it, 256-byte aligned, zero-filled once). */
 int64_t gsr_batch_table_bytes(int32_t n_groups);
-/* Validate groups, assign tiles and copy the table to `dev_table` (synchronously); fills `info`. Groups are
- * stepped in the given order, so list the most expensive first. */
+/* Validate groups, assign tiles and copy the table to `dev_table` (synchronously); fills `info` (whose `persistent`
+ * and `waves_per_simd` are read as inputs), which the launch functions need. Groups are stepped in the given order,
+ * so list the most expensive first. No launch that uses `dev_table` may be in flight. */
 int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table, int64_t table_bytes,
                      gsr_batch_info* info);
 /* n_substeps fused launches of the uploaded batch on `stream`. */

@@ -201,3 +201,76 @@ def test_step_parity_resynced(case, B, steps, variant):
           f"{B * steps} env-steps, {total_contacts} contacts, {n_conditioned} entries held to the FP32/FP64 spread")
     assert mismatched_contacts == 0
     assert int(S.st.t["errno_env"].max().item()) & 0xF == 0
+
+
+@pytest.mark.parametrize("case", ["arm", "quadruped", "pile8", "pile64"])
+def test_wave_collide_equals_lane_collide(case):
+    """The wavefront collision kernel (pairs across lanes, prefix-sum placement) writes bitwise what the one-lane-per-env
+    kernel writes: sort buffers, broadphase pairs in order, contacts in order, normal cache and constraint rows, over
+    two consecutive steps (so the warm-started sort and the cached normals are exercised too)."""
+    _need_gpu()
+    from genesis_amd.rigid_solver import RigidSolver
+    ents, opt = CASES[case]()
+    M = compile_model(ents, opt)
+    B = 32
+    rng = np.random.default_rng(5)
+    q = torch.as_tensor(_random_qpos(M, B, rng, base_z=(0.05, 0.35)), dtype=torch.float32)
+    lane = RigidSolver(M, B, "cuda", solver_variant=1, collide_variant=0)
+    wave = RigidSolver(M, B, "cuda", solver_variant=1, collide_variant=1)
+    for S in (lane, wave):
+        S.set_qpos(q)
+    fields = ["sort_value", "sort_ig", "first_time", "n_broad", "broad_pairs", "n_contacts", "contact_geoms",
+              "contact_pos", "contact_normal", "contact_pen", "contact_friction", "cache_normal", "n_con", "jac_val",
+              "jac_idx", "jac_nnz", "efc_diag", "efc_aref", "efc_D", "efc_fl", "aabb_min", "aabb_max", "errno_env"]
+    for step in range(2):
+        for S in (lane, wave):
+            S.launch_kernel(0)
+            S.launch_kernel(1)
+        torch.cuda.synchronize()
+        for f in fields:
+            a, b = lane.st.t[f], wave.st.t[f]
+            assert torch.equal(a, b), f"{case} step {step}: {f} differs"
+        if case != "arm":
+            assert int(wave.st.t["n_contacts"].sum()) > 0
+        for S in (lane, wave):
+            S.launch_kernel(2)
+            S.launch_kernel(3)
+
+
+@pytest.mark.parametrize("case,B,steps", [("arm", 8, 1000), ("quadruped", 8, 1000)])
+def test_fp64_drift_over_1000_steps(case, B, steps):
+    """Free runs (no resync) of the HIP step, the FP32 oracle and the FP64 oracle from one state under the same controls:
+    the HIP step's drift from FP64 must be of the same order as the FP32 oracle's own drift (both are FP32 roundings of
+    the same algorithm). The drifts are printed: they are the stated FP64 drift of both implementations."""
+    _need_gpu()
+    from tests.sync import gpu_to_oracle as g2o
+    ents, opt = CASES[case]()
+    M = compile_model(ents, opt)
+    S = _solver(M, B)
+    if case == "quadruped":
+        S.set_qpos(torch.as_tensor(np.tile(np.concatenate([W.GO2_INIT_BASE, W.GO2_STANDING]), (B, 1)),
+                                   dtype=torch.float32))
+    O32, O64 = _oracle(M, B, 32), _oracle(M, B, 64)
+    torch.cuda.synchronize()
+    g2o(S, O32)
+    g2o(S, O64)
+    rng = np.random.default_rng(3)
+    drift_gpu, drift_o32 = [], []
+    for step in range(steps):
+        _control(case, M, B, step, rng, S)
+        torch.cuda.synchronize()
+        for O in (O32, O64):  # same targets and modes as the GPU
+            O.set("ctrl_pos", S.st.t["ctrl_pos"].T.cpu().numpy().astype(np.float64))
+            O.set("ctrl_mode", S.st.t["ctrl_mode"].T.cpu().numpy())
+        S.substep()
+        O32.run(1)
+        O64.run(1)
+        if (step + 1) % 100 == 0:
+            torch.cuda.synchronize()
+            q64 = O64.get("qpos")
+            drift_gpu.append(float(np.abs(S.st.t["qpos"].T.cpu().numpy() - q64).max()))
+            drift_o32.append(float(np.abs(O32.get("qpos") - q64).max()))
+    print(f"{case}: max |qpos - FP64| every 100 steps, HIP: {['%.2e' % d for d in drift_gpu]}")
+    print(f"{case}: max |qpos - FP64| every 100 steps, FP32 oracle: {['%.2e' % d for d in drift_o32]}")
+    assert np.isfinite(drift_gpu).all()
+    assert max(drift_gpu) <= 10 * max(drift_o32) + 1e-3
