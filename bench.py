@@ -152,13 +152,18 @@ def bench_serve(args):
     import torch
     from genesis_amd.batch import Group, autotune
     from genesis_amd.serving import Pool, Server, poisson_requests
+    from genesis_amd.workloads import init_state
 
     cap_envs = args.n_envs or 8192
     pools = {s: Pool(s, cap_envs, dt=0.01) for s in ("arm", "quadruped", "pile20")}
     # execution choices per scene type, tuned once on a 1024-env slice of each pool
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     log("serve: tuning tiles")
-    tuned, launch, _ = autotune([Group(p.S, 0, min(1024, cap_envs)) for p in pools.values()], steps=2, warmup=1)
+    n_tune = min(512, cap_envs)
+    for p in pools.values():
+        init_state(p.w, p.M, p.S, 0, n_tune)
+    tuned, launch, _ = autotune([Group(p.S, 0, n_tune) for p in pools.values()], steps=2, warmup=1, verbose=True,
+                                log=log)
     for g in tuned:
         for p in pools.values():
             if p.S is g.solver:
@@ -173,14 +178,14 @@ def bench_serve(args):
     for r in sat:
         r.arrival = 0.0
     log(f"serve: capacity run with {len(sat)} queued requests")
-    cap = server.run(sat)
+    cap = server.run(sat, log=log)
     capacity = cap["env_steps_per_s"]
     log(f"serve: capacity {capacity:.0f} env-steps/s")
     sweep = []
     for load in (0.2, 0.4, 0.6, 0.8, 1.0):
         rate = load * capacity / mean_work
         reqs = poisson_requests(rate, args.serve_seconds, seed=4, start_id=0)
-        st = server.run(reqs)
+        st = server.run(reqs, log=log)
         st.update(load=load, offered_requests_per_s=rate, n_requests=len(reqs))
         log(f"serve: load {load:.1f}: {len(reqs)} requests, p50 {st['step_ms_p50']:.3f} ms, p99 {st['step_ms_p99']:.3f} ms")
         sweep.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()})
@@ -202,6 +207,19 @@ def bench_serve(args):
     print(json.dumps(out))
 
 
+def calibrate_copy(mb):
+    """A plain device copy of known size, profiled beside the step with the same HBM counters (FETCH_SIZE,
+    WRITE_SIZE) to turn their readings into bytes: bytes per KiB reported = (mb MiB) / (counter reading)."""
+    import torch
+    n = int(mb * (1 << 20)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,7 +233,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--solver-variant", type=int, default=None, help="0 lane/env, 1 wavefront/env (default: by model)")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, help="only run a device copy of this many MiB")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)

@@ -158,7 +158,7 @@ LAUNCH_CANDIDATES = [dict(persistent=True, waves_per_simd=4), dict(persistent=Fa
                      dict(persistent=True, waves_per_simd=2), dict(persistent=False, waves_per_simd=2)]
 
 
-def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: bool = False):
+def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: bool = False, log=print):
     """Pick (solver variant, envs per tile) per group by timing each group alone, order the groups by per-tile cost
     (descending, so the longest tiles start first), then pick the launch form (persistent queue or one workgroup per
     tile; 128 or 256 VGPRs) by timing the whole batch.
@@ -172,7 +172,7 @@ def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: 
             ms = time_group(Group(g.solver, g.env_begin, g.n_envs, t, v, c), steps, warmup, **LAUNCH_CANDIDATES[0])
             report.append({"group": gi, "solver": v, "collide": c, "envs_per_tile": t, "ms": ms})
             if verbose:
-                print(f"  group {gi} solver {v} collide {c} tile {t}: {ms:.3f} ms", flush=True)
+                log(f"  group {gi} solver {v} collide {c} tile {t}: {ms:.3f} ms")
             if best is None or ms < best[0]:
                 best = (ms, v, c, t)
         ms, v, c, t = best
@@ -185,7 +185,7 @@ def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: 
         ms = time_batch(tuned, steps, warmup, **lc)
         report.append(dict(group="all", launch=lc, ms=ms))
         if verbose:
-            print(f"  batch {lc}: {ms:.3f} ms", flush=True)
+            log(f"  batch {lc}: {ms:.3f} ms")
         if best is None or ms < best[0]:
             best = (ms, lc)
     return tuned, best[1], report

@@ -45,6 +45,37 @@ __global__ void __launch_bounds__(kBlock) k_step1(gsr_model m, gsr_state s, int 
   dev_smooth_dynamics(e);
 }
 
+// step_1 / step_2 of one env on one wavefront: the env's entities are spread over the lanes (no entity reads another's
+// state); only the geom poses, split by geom index, need the link poses of every lane, hence the barrier.
+__device__ __forceinline__ void wave_step1(const Env& e) { dev_smooth_dynamics(e, threadIdx.x, kWave); }
+
+__device__ __forceinline__ void wave_step2(const Env& e) {
+  const int lane = threadIdx.x;
+  bool valid = dev_integrate_next(e, lane, kWave);
+  valid = __all(valid);
+  dev_integrate_commit(e, lane, kWave, valid, lane == 0);
+  dev_forward_kinematics(e, lane, kWave);
+  dev_com_links(e, lane, kWave);
+  __syncthreads();
+  dev_update_geoms(e, lane, kWave);
+  dev_forward_velocity(e, lane, kWave);
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kWave) k_step1_wave(gsr_model m, gsr_state s) {
+  int b = blockIdx.x;
+  if (b >= s.n_envs) return;  // uniform across the block
+  Env e(m, s, b);
+  wave_step1(e);
+}
+
+__global__ void __launch_bounds__(kWave) k_step2_wave(gsr_model m, gsr_state s) {
+  int b = blockIdx.x;
+  if (b >= s.n_envs) return;
+  Env e(m, s, b);
+  wave_step2(e);
+}
+
 __global__ void __launch_bounds__(kBlock) k_collide(gsr_model m, gsr_state s) {
   int b = env_id();
   if (b >= s.n_envs) return;
@@ -146,8 +177,8 @@ __device__ __forceinline__ void tile_lane_stages_2(const Env& e) {
 __device__ __forceinline__ void run_tile(const gsr_model& m, const gsr_state& s, int env0, int n_here, char* lds) {
   const int lane = threadIdx.x;
   const bool mine = lane < n_here;
-  const bool cw = m.collide_variant == 1, sw = m.solver_variant == 1;
-  if (!cw && !sw) {
+  const bool cw = m.collide_variant == 1, sw = m.solver_variant == 1, dv = m.dyn_variant == 1;
+  if (!cw && !sw && !dv) {
     if (mine) {
       Env e(m, s, env0 + lane);
       dev_smooth_dynamics(e);
@@ -160,9 +191,10 @@ __device__ __forceinline__ void run_tile(const gsr_model& m, const gsr_state& s,
     }
     return;
   }
-  if (mine) {
+  // collision reads only the pose state of the previous step, so it may run before or after the dynamics
+  if (mine && (!dv || !cw)) {
     Env e(m, s, env0 + lane);
-    dev_smooth_dynamics(e);
+    if (!dv) dev_smooth_dynamics(e);
     if (!cw) {
       dev_equality_rows(e);
       dev_collision(e);
@@ -172,15 +204,26 @@ __device__ __forceinline__ void run_tile(const gsr_model& m, const gsr_state& s,
   __syncthreads();
   for (int j = 0; j < n_here; j++) {
     Env e(m, s, env0 + j);
+    if (dv) wave_step1(e);
     if (cw) wave_collide(e, lds);
+    __syncthreads();
     if (sw) wave_solve(e, lds);
     __syncthreads();
   }
-  if (mine) {
+  if (mine && (!sw || !dv)) {
     Env e(m, s, env0 + lane);
     if (!sw) dev_solve(e);
-    dev_integrate(e);
-    dev_kinematics(e);
+    if (!dv) {
+      dev_integrate(e);
+      dev_kinematics(e);
+    }
+  }
+  if (dv) {
+    __syncthreads();
+    for (int j = 0; j < n_here; j++) {
+      Env e(m, s, env0 + j);
+      wave_step2(e);
+    }
   }
 }
 
@@ -262,7 +305,8 @@ int check(const gsr_model* m, const gsr_state* s) {
   if (m->collide_variant == 1) {
     if (m->n_geoms > kWaveMaxGeoms || m->n_contacts_per_pair > kMaxPairContacts) return GSR_E_SHAPE;
   }
-  if (m->solver_variant < 0 || m->solver_variant > 1 || m->collide_variant < 0 || m->collide_variant > 1)
+  if (m->solver_variant < 0 || m->solver_variant > 1 || m->collide_variant < 0 || m->collide_variant > 1 ||
+      m->dyn_variant < 0 || m->dyn_variant > 1)
     return GSR_E_ARG;
   return GSR_OK;
 }
@@ -279,7 +323,10 @@ void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
   switch (k) {
     // kinematics of the current pose are refreshed at the end of every step (and by gsr_forward_kinematics after a
     // pose write), so kernel_step_1 never needs to recompute them (enable_mujoco_compatibility = False)
-    case 0: hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, st, *m, *s, 0); break;
+    case 0:
+      if (m->dyn_variant == 1) hipLaunchKernelGGL(k_step1_wave, dim3(s->n_envs), dim3(kWave), 0, st, *m, *s);
+      else hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, st, *m, *s, 0);
+      break;
     case 1:
       if (m->collide_variant == 1)
         hipLaunchKernelGGL(k_collide_wave, dim3(s->n_envs), dim3(kWave), wave_collide_lds_bytes(m->n_geoms), st, *m,
@@ -294,7 +341,10 @@ void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
       else
         hipLaunchKernelGGL(k_solve, g, dim3(kBlock), 0, st, *m, *s);
       break;
-    default: hipLaunchKernelGGL(k_step2, g, dim3(kBlock), 0, st, *m, *s); break;
+    default:
+      if (m->dyn_variant == 1) hipLaunchKernelGGL(k_step2_wave, dim3(s->n_envs), dim3(kWave), 0, st, *m, *s);
+      else hipLaunchKernelGGL(k_step2, g, dim3(kBlock), 0, st, *m, *s);
+      break;
   }
 }
 
@@ -364,7 +414,7 @@ int gsr_run_stage(const gsr_model* model, const gsr_state* state, int32_t stage,
   dim3 g = grid_for(state->n_envs);
   if (stage == 0) return run_substep(model, state, stream);
   hipLaunchKernelGGL(k_kinematics, g, dim3(kBlock), 0, stream, *model, *state, (const int32_t*)nullptr, state->n_envs);
-  if (stage >= 2) hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, stream, *model, *state, 0);
+  if (stage >= 2) launch_one(model, state, 0, stream);
   if (stage >= 3) launch_one(model, state, 1, stream);
   return launch_status();
 }

@@ -199,7 +199,7 @@ __device__ inline void wave_narrow_phase(const Env& e, int lane) {
 }
 
 // equality rows, collision detection and inequality rows of one env (k_collide's work) on one wavefront
-__device__ inline void wave_collide(const Env& e, char* lds) {
+__device__ __forceinline__ void wave_collide(const Env& e, char* lds) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const int lane = threadIdx.x;

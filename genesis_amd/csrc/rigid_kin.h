@@ -10,11 +10,20 @@
 
 namespace gsr {
 
-__device__ inline void dev_forward_kinematics(const Env& e) {
+// Every function below takes the entities it works on as (first, stride): (0, 1) is the whole env on one lane,
+// (lane, 64) splits the env's entities over a wavefront. Entities share no state, so the split changes no result.
+#define GSR_FOR_ENT_LINKS(ie, l)                                      \
+  for (int ie = e0; ie < m.n_entities; ie += es)                      \
+    for (int l = m.ent_link_start[ie]; l < m.ent_link_end[ie]; l++)
+#define GSR_FOR_ENT_DOFS(ie, d)                                       \
+  for (int ie = e0; ie < m.n_entities; ie += es)                      \
+    for (int d = m.ent_dof_start[ie]; d < m.ent_dof_end[ie]; d++)
+
+__device__ inline void dev_forward_kinematics(const Env& e, int e0 = 0, int es = 1) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const float EPS = m.eps;
-  for (int l = 0; l < m.n_links; l++) {
+  GSR_FOR_ENT_LINKS(ie, l) {
     f3 pos = T3(m.link_pos, l);
     q4 quat = T4(m.link_quat, l);
     int par = m.link_parent[l];
@@ -71,17 +80,16 @@ __device__ inline void dev_forward_kinematics(const Env& e) {
   }
 }
 
-__device__ inline void dev_com_links(const Env& e) {
+__device__ inline void dev_com_links(const Env& e, int e0 = 0, int es = 1) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const float EPS = m.eps;
-  const int L = m.n_links;
   // subtree-root mass and mass-weighted COM, accumulated in link order (crb_* used as scratch)
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie, l) {
     e.S3(s.root_com, l, zero3());
     e.F(s.crb_mass, l) = 0.f;
   }
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie, l) {
     float mass = m.link_inertial_mass[l];
     f3 lp = e.L3(s.links_pos, l);
     q4 lq = e.L4(s.links_quat, l);
@@ -92,13 +100,13 @@ __device__ inline void dev_com_links(const Env& e) {
     e.F(s.crb_mass, r) = e.F(s.crb_mass, r) + mass;
     e.S3(s.root_com, r, e.L3(s.root_com, r) + mass * ipos_bw);
   }
-  for (int l = 0; l < L; l++)
+  GSR_FOR_ENT_LINKS(ie, l)
     if (m.link_root[l] == l) e.S3(s.root_com, l, e.L3(s.root_com, l) / e.F(s.crb_mass, l));
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie, l) {
     int r = m.link_root[l];
     if (r != l) e.S3(s.root_com, l, e.L3(s.root_com, r));
   }
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie, l) {
     f3 t = e.L3(s.crb_pos, l) - e.L3(s.root_com, l);  // i_pos
     float mass = m.link_inertial_mass[l];
     float x = t.x, y = t.y, z = t.z;
@@ -117,7 +125,7 @@ __device__ inline void dev_com_links(const Env& e) {
     e.S3(s.cinr_pos, l, t * mass);
     e.F(s.cinr_mass, l) = mass;
   }
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie, l) {
     if (m.link_dof_end[l] - m.link_dof_start[l] <= 0) continue;
     for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
       int jt = m.joint_type[j], ds = m.joint_dof_start[j];
@@ -153,10 +161,10 @@ __device__ inline void dev_com_links(const Env& e) {
   }
 }
 
-__device__ inline void dev_update_geoms(const Env& e) {
+__device__ inline void dev_update_geoms(const Env& e, int e0 = 0, int es = 1) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
-  for (int g = 0; g < m.n_geoms; g++) {
+  for (int g = e0; g < m.n_geoms; g += es) {
     int l = m.geom_link[g];
     f3 lp = e.L3(s.links_pos, l);
     q4 lq = e.L4(s.links_quat, l);
@@ -165,10 +173,10 @@ __device__ inline void dev_update_geoms(const Env& e) {
   }
 }
 
-__device__ inline void dev_forward_velocity(const Env& e) {
+__device__ inline void dev_forward_velocity(const Env& e, int e0 = 0, int es = 1) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
-  for (int l = 0; l < m.n_links; l++) {
+  GSR_FOR_ENT_LINKS(ie, l) {
     f3 cv = zero3(), ca = zero3();
     int par = m.link_parent[l];
     if (par != -1) {
@@ -215,6 +223,7 @@ __device__ inline void dev_forward_velocity(const Env& e) {
   }
 }
 
+// geoms are split by index, so a split run needs a barrier between link poses and geom poses
 __device__ inline void dev_kinematics(const Env& e) {
   dev_forward_kinematics(e);
   dev_com_links(e);
@@ -223,10 +232,10 @@ __device__ inline void dev_kinematics(const Env& e) {
 }
 
 // M^-1 vec per entity block through the LDL^T factors (rsd:3932-4042)
-__device__ inline void dev_solve_mass(const Env& e, const float* vec, float* out) {
+__device__ inline void dev_solve_mass(const Env& e, const float* vec, float* out, int e0 = 0, int es = 1) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
-  for (int ie = 0; ie < m.n_entities; ie++) {
+  for (int ie = e0; ie < m.n_entities; ie += es) {
     int st = m.ent_dof_start[ie], en = m.ent_dof_end[ie], n = en - st;
     for (int ii = 0; ii < n; ii++) {
       int i = en - ii - 1;
@@ -243,18 +252,18 @@ __device__ inline void dev_solve_mass(const Env& e, const float* vec, float* out
   }
 }
 
-__device__ inline void dev_smooth_dynamics(const Env& e) {
+__device__ inline void dev_smooth_dynamics(const Env& e, int e0 = 0, int es = 1) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
-  const int L = m.n_links, D = m.n_dofs, E = m.n_entities;
+  const int D = m.n_dofs, E = m.n_entities;
   const float EPS = m.eps, dt = m.dt;
   // --- CRBA (rsd:3422-3678) ---
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie0, l) {
     for (int k = 0; k < 9; k++) e.F(s.crb_inertial, 9 * l + k) = e.F(s.cinr_inertial, 9 * l + k);
     e.S3(s.crb_pos, l, e.L3(s.cinr_pos, l));
     e.F(s.crb_mass, l) = e.F(s.cinr_mass, l);
   }
-  for (int ie = 0; ie < E; ie++) {
+  for (int ie = e0; ie < E; ie += es) {
     int ls = m.ent_link_start[ie], le = m.ent_link_end[ie];
     for (int l = le - 1; l >= ls; l--) {
       int p = m.link_parent[l];
@@ -265,7 +274,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
       }
     }
   }
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie1, l) {
     int d0 = m.link_dof_start[l], d1 = m.link_dof_end[l];
     if (d1 <= d0) continue;
     float I[9];
@@ -279,7 +288,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
       e.S3(s.f_vel, d, fv);
     }
   }
-  for (int ie = 0; ie < E; ie++) {
+  for (int ie = e0; ie < E; ie += es) {
     int st = m.ent_dof_start[ie], en = m.ent_dof_end[ie];
     for (int i = st; i < en; i++) {
       f3 fa = e.L3(s.f_ang, i), fv = e.L3(s.f_vel, i);
@@ -316,7 +325,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
     }
   }
   // --- actuation and passive forces (rsd:5960-6177) ---
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie2, l) {
     int dstart = m.link_dof_start[l], dend = m.link_dof_end[l];
     if (dend <= dstart) continue;
     int jt = m.joint_type[m.link_joint_start[l]];
@@ -342,8 +351,8 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
       }
     }
   }
-  for (int d = 0; d < D; d++) e.F(s.qf_passive, d) = -m.dof_damping[d] * e.F(s.vel, d);
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_DOFS(ie3, d) e.F(s.qf_passive, d) = -m.dof_damping[d] * e.F(s.vel, d);
+  GSR_FOR_ENT_LINKS(ie4, l) {
     if (m.link_dof_end[l] - m.link_dof_start[l] <= 0) continue;
     int jt = m.joint_type[m.link_joint_start[l]];
     if (jt != JOINT_FREE && jt != JOINT_FIXED)
@@ -351,7 +360,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
         e.F(s.qf_passive, d) = e.F(s.qf_passive, d) + -e.F(s.dofs_pos, d) * m.dof_stiffness[d];
   }
   // --- bias accelerations (rsd:6180-6281) ---
-  for (int ie = 0; ie < E; ie++) {
+  for (int ie = e0; ie < E; ie += es) {
     for (int l = m.ent_link_start[ie]; l < m.ent_link_end[ie]; l++) {
       int p = m.link_parent[l];
       f3 cv, ca;
@@ -372,7 +381,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
     }
   }
   // --- link forces and backward accumulation (rsd:6284-6398) ---
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie5, l) {
     float I[9];
     for (int k = 0; k < 9; k++) I[k] = e.F(s.cinr_inertial, 9 * l + k);
     f3 cp = e.L3(s.cinr_pos, l);
@@ -386,7 +395,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
     e.S3(s.cfrc_vel, l, f1v + f3v + zero3() + zero3());
     e.S3(s.cfrc_ang, l, f1a + f3a + zero3() + zero3());
   }
-  for (int ie = 0; ie < E; ie++) {
+  for (int ie = e0; ie < E; ie += es) {
     int ls = m.ent_link_start[ie], le = m.ent_link_end[ie];
     for (int l = le - 1; l >= ls; l--) {
       int p = m.link_parent[l];
@@ -397,7 +406,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
     }
   }
   // --- generalized bias force, total force, smooth acceleration (rsd:6425-6563) ---
-  for (int l = 0; l < L; l++) {
+  GSR_FOR_ENT_LINKS(ie6, l) {
     f3 fa = e.L3(s.cfrc_ang, l), fv = e.L3(s.cfrc_vel, l);
     for (int d = m.link_dof_start[l]; d < m.link_dof_end[l]; d++) {
       float qb = dot(e.L3(s.cdof_ang, d), fa) + dot(e.L3(s.cdof_vel, d), fv);
@@ -407,8 +416,8 @@ __device__ inline void dev_smooth_dynamics(const Env& e) {
       e.F(s.qf_smooth, d) = f;
     }
   }
-  dev_solve_mass(e, s.force, s.acc_smooth);
-  for (int d = 0; d < D; d++) e.F(s.acc, d) = e.F(s.acc_smooth, d);
+  dev_solve_mass(e, s.force, s.acc_smooth, e0, es);
+  GSR_FOR_ENT_DOFS(ie7, d) e.F(s.acc, d) = e.F(s.acc_smooth, d);
 }
 
 }  // namespace gsr

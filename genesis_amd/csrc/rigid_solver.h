@@ -606,43 +606,62 @@ __device__ inline void dev_solve(const Env& e) {
 }
 
 // func_integrate (rsd:6566-6708) and the NaN-guarded commit func_copy_next_to_curr (rsd:6711-6740)
-__device__ inline void dev_integrate(const Env& e) {
+// semi-implicit Euler into vel_next / qpos_next for the entities (first, stride); returns false on a NaN
+__device__ inline bool dev_integrate_next(const Env& e, int e0, int es) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const float dt = m.dt;
-  for (int d = 0; d < m.n_dofs; d++) e.F(s.vel_next, d) = e.F(s.vel, d) + e.F(s.acc, d) * dt;
-  for (int l = 0; l < m.n_links; l++) {
-    int ds = m.link_dof_start[l], qs = m.link_q_start[l], qe = m.link_q_end[l];
-    if (m.link_dof_end[l] - ds <= 0) continue;
-    int jt = m.joint_type[m.link_joint_start[l]];
-    if (jt == JOINT_FREE) {
-      for (int j = 0; j < 3; j++) e.F(s.qpos_next, qs + j) = e.F(s.qpos, qs + j) + e.F(s.vel_next, ds + j) * dt;
-    }
-    if (jt == JOINT_SPHERICAL || jt == JOINT_FREE) {
-      int ro = jt == JOINT_FREE ? 3 : 0;
-      q4 rot0{e.F(s.qpos, qs + ro), e.F(s.qpos, qs + ro + 1), e.F(s.qpos, qs + ro + 2), e.F(s.qpos, qs + ro + 3)};
-      f3 ang = f3{e.F(s.vel_next, ds + ro), e.F(s.vel_next, ds + ro + 1), e.F(s.vel_next, ds + ro + 2)} * dt;
-      q4 rot = qrot_q(rotvec_to_quat(ang, m.eps), rot0);  // normalize(rot0 * qrot)
-      e.F(s.qpos_next, qs + ro) = rot.w;
-      e.F(s.qpos_next, qs + ro + 1) = rot.x;
-      e.F(s.qpos_next, qs + ro + 2) = rot.y;
-      e.F(s.qpos_next, qs + ro + 3) = rot.z;
-    } else {
-      for (int j = 0; j < qe - qs; j++) e.F(s.qpos_next, qs + j) = e.F(s.qpos, qs + j) + e.F(s.vel_next, ds + j) * dt;
-    }
-  }
   bool valid = true;
-  for (int d = 0; d < m.n_dofs; d++) valid &= !isnan(e.F(s.vel_next, d));
-  for (int q = 0; q < m.n_qs; q++) {
-    // q slots of dof-less links are never written by the integrator: keep their current value
-    valid &= !isnan(e.F(s.qpos_next, q));
+  for (int ie = e0; ie < m.n_entities; ie += es) {
+    for (int d = m.ent_dof_start[ie]; d < m.ent_dof_end[ie]; d++) {
+      float v = e.F(s.vel, d) + e.F(s.acc, d) * dt;
+      e.F(s.vel_next, d) = v;
+      valid &= !isnan(v);
+    }
+    for (int l = m.ent_link_start[ie]; l < m.ent_link_end[ie]; l++) {
+      int ds = m.link_dof_start[l], qs = m.link_q_start[l], qe = m.link_q_end[l];
+      if (m.link_dof_end[l] - ds <= 0) continue;
+      int jt = m.joint_type[m.link_joint_start[l]];
+      if (jt == JOINT_FREE) {
+        for (int j = 0; j < 3; j++) e.F(s.qpos_next, qs + j) = e.F(s.qpos, qs + j) + e.F(s.vel_next, ds + j) * dt;
+      }
+      if (jt == JOINT_SPHERICAL || jt == JOINT_FREE) {
+        int ro = jt == JOINT_FREE ? 3 : 0;
+        q4 rot0{e.F(s.qpos, qs + ro), e.F(s.qpos, qs + ro + 1), e.F(s.qpos, qs + ro + 2), e.F(s.qpos, qs + ro + 3)};
+        f3 ang = f3{e.F(s.vel_next, ds + ro), e.F(s.vel_next, ds + ro + 1), e.F(s.vel_next, ds + ro + 2)} * dt;
+        q4 rot = qrot_q(rotvec_to_quat(ang, m.eps), rot0);  // normalize(rot0 * qrot)
+        e.F(s.qpos_next, qs + ro) = rot.w;
+        e.F(s.qpos_next, qs + ro + 1) = rot.x;
+        e.F(s.qpos_next, qs + ro + 2) = rot.y;
+        e.F(s.qpos_next, qs + ro + 3) = rot.z;
+      } else {
+        for (int j = 0; j < qe - qs; j++) e.F(s.qpos_next, qs + j) = e.F(s.qpos, qs + j) + e.F(s.vel_next, ds + j) * dt;
+      }
+      for (int q = qs; q < qe; q++) valid &= !isnan(e.F(s.qpos_next, q));
+    }
   }
+  return valid;
+}
+
+// NaN-guarded commit (func_copy_next_to_curr): the env's whole state moves only if no entity produced a NaN
+__device__ inline void dev_integrate_commit(const Env& e, int e0, int es, bool valid, bool flag_lane) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
   if (valid) {
-    for (int d = 0; d < m.n_dofs; d++) e.F(s.vel, d) = e.F(s.vel_next, d);
-    for (int q = 0; q < m.n_qs; q++) e.F(s.qpos, q) = e.F(s.qpos_next, q);
-  } else {
+    for (int ie = e0; ie < m.n_entities; ie += es) {
+      for (int d = m.ent_dof_start[ie]; d < m.ent_dof_end[ie]; d++) e.F(s.vel, d) = e.F(s.vel_next, d);
+      for (int l = m.ent_link_start[ie]; l < m.ent_link_end[ie]; l++)
+        for (int q = m.link_q_start[l]; q < m.link_q_end[l]; q++) e.F(s.qpos, q) = e.F(s.qpos_next, q);
+    }
+  } else if (flag_lane) {
     e.I((int32_t*)s.errno_env, 0) |= (int)GSR_ERR_NAN_STATE;
   }
+}
+
+// func_integrate (rsd:6566-6708) + func_copy_next_to_curr (rsd:6711-6740)
+__device__ inline void dev_integrate(const Env& e) {
+  bool valid = dev_integrate_next(e, 0, 1);
+  dev_integrate_commit(e, 0, 1, valid, true);
 }
 
 }  // namespace gsr

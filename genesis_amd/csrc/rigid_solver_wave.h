@@ -6,16 +6,25 @@
 //     wave-shuffle tree reductions (every lane holds the identical result, so control flow stays uniform);
 //   * per-dof work (M x, J^T f, Hessian rows) is spread over lanes by dof; every per-dof sum keeps the reference's
 //     row order through per-entity row lists;
-//   * the Hessian H = M + J^T D J is stored dense (packed lower triangle) but only its entity-block pattern is
-//     touched. The pattern is the symbolic Cholesky fill of the entity graph of all constraint rows, so every entry
-//     outside it is an exact zero in the dense factorisation too and skipping it changes no bit of the factors.
-// Vectors over dofs live in LDS; rows and H live in global memory, env-major.
+//   * the Hessian H = M + J^T D J lives only on its entity-block pattern: the symbolic Cholesky fill of the entity
+//     graph of all constraint rows, so every entry outside it is an exact zero in the dense factorisation too and
+//     skipping it changes no bit of the factors. When the pattern fits (kWaveHBudget floats) H is kept in LDS,
+//     row-compressed per entity; otherwise in the env's packed dense lower triangle in global memory.
+// Vectors over dofs live in LDS; constraint rows live in global memory, env-major.
 #pragma once
 #include "rigid_env.h"
 
 namespace gsr {
 
 constexpr int kWave = 64;
+
+// LDS pointers carry their address space so every access compiles to ds_* instructions (not flat), and the compiler
+// knows they never alias global memory.
+#define GSR_LDS __attribute__((address_space(3)))
+typedef GSR_LDS float lds_f;
+typedef GSR_LDS int lds_i;
+typedef GSR_LDS unsigned long long lds_u64;
+typedef GSR_LDS char lds_c;
 constexpr int kMaxPatternEntities = 128;  // two 64-bit words per entity row of the pattern
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -29,34 +38,50 @@ __device__ __forceinline__ int wave_or(int v) {
 
 // LDS layout of one env (floats / ints), sized on the host by wave_lds_bytes()
 struct WaveLds {
-  float* qacc;
-  float* Ma;
-  float* grad;
-  float* Mgrad;
-  float* search;
-  float* mv;
-  float* qfrc;
-  float* vec;
-  float* force;
-  float* acc_smooth;
-  unsigned long long* pat;  // [E][2] entity pattern bits (row e: entities e' with block (e, e') nonzero)
-  int* ent_of_dof;          // [D]
-  int* lvl_order;           // [E] entities sorted by forward level (elimination dependencies)
-  int* lvl_start;           // [E + 2]
-  int* rlvl_order;          // [E] entities sorted by backward level
-  int* rlvl_start;          // [E + 2]
-  int* cursor;              // [E] row-list fill cursors
-  int* n_lvl;               // [2] number of forward / backward levels
+  lds_f* qacc;
+  lds_f* Ma;
+  lds_f* grad;
+  lds_f* Mgrad;
+  lds_f* search;
+  lds_f* mv;
+  lds_f* qfrc;
+  lds_f* vec;
+  lds_f* force;
+  lds_f* acc_smooth;
+  lds_u64* pat;  // [E][2] entity pattern bits (row e: entities e' with block (e, e') nonzero)
+  lds_i* ent_of_dof;          // [D]
+  lds_i* lvl_order;           // [E] entities sorted by forward level (elimination dependencies)
+  lds_i* lvl_start;           // [E + 2]
+  lds_i* rlvl_order;          // [E] entities sorted by backward level
+  lds_i* rlvl_start;          // [E + 2]
+  lds_i* cursor;              // [E] row-list fill cursors
+  lds_i* n_lvl;               // [2] number of forward / backward levels
+  // Hessian in LDS (when the pattern fits): row i of entity e starts at hoff[e] + r * below[e] + r * (r + 1) / 2
+  // (r = i - es[e]) and holds the dofs of the pattern entities below e (entity k at pre[pstart[e] + rank of k in
+  // the row]) followed by e's own dofs up to i.
+  lds_f* hl;                // [kWaveHBudget]
+  lds_i* es;                  // [E] first dof of each entity
+  lds_i* ee;                  // [E] end dof of each entity
+  lds_i* hoff;                // [E]
+  lds_i* below;               // [E]
+  lds_i* pstart;              // [E]
+  lds_i* pre;                 // [kWavePreSlots]
+  lds_i* hmode;               // [1] 1 = H in LDS, 0 = H in global memory
+  lds_i* moff;                // [E] offset of each entity's mass block (copy of ent_mass_offset)
 };
 
+constexpr int kWaveHBudget = 5120;   // floats of LDS for the Hessian of one env
+constexpr int kWavePreSlots = 512;   // off-diagonal pattern blocks addressable in LDS mode
+
 __host__ __device__ inline size_t wave_lds_bytes(int D, int E) {
-  return sizeof(float) * 10 * (size_t)D + sizeof(unsigned long long) * 2 * (size_t)E + sizeof(int) * (size_t)D +
-         sizeof(int) * (5 * (size_t)E + 6) + 64;
+  return sizeof(float) * (10 * (size_t)D + kWaveHBudget) + sizeof(unsigned long long) * 2 * (size_t)E +
+         sizeof(int) * (size_t)D + sizeof(int) * (5 * (size_t)E + 6) + sizeof(int) * (6 * (size_t)E + kWavePreSlots + 1) +
+         64;
 }
 
-__device__ inline WaveLds wave_lds_carve(char* base, int D, int E) {
+__device__ inline WaveLds wave_lds_carve(lds_c* base, int D, int E) {
   WaveLds w;
-  float* f = reinterpret_cast<float*>(base);
+  lds_f* f = reinterpret_cast<lds_f*>(base);
   w.qacc = f; f += D;
   w.Ma = f; f += D;
   w.grad = f; f += D;
@@ -67,16 +92,25 @@ __device__ inline WaveLds wave_lds_carve(char* base, int D, int E) {
   w.vec = f; f += D;
   w.force = f; f += D;
   w.acc_smooth = f; f += D;
-  uintptr_t p = reinterpret_cast<uintptr_t>(f);
-  p = (p + 7) & ~(uintptr_t)7;
-  w.pat = reinterpret_cast<unsigned long long*>(p);
-  w.ent_of_dof = reinterpret_cast<int*>(w.pat + 2 * E);
+  w.hl = f; f += kWaveHBudget;
+  uint32_t p = (uint32_t)reinterpret_cast<uintptr_t>(f);
+  lds_c* pc = reinterpret_cast<lds_c*>(f) + (((p + 7u) & ~7u) - p);
+  w.pat = reinterpret_cast<lds_u64*>(pc);
+  w.ent_of_dof = reinterpret_cast<lds_i*>(w.pat + 2 * E);
   w.lvl_order = w.ent_of_dof + D;
   w.lvl_start = w.lvl_order + E;
   w.rlvl_order = w.lvl_start + E + 2;
   w.rlvl_start = w.rlvl_order + E;
   w.cursor = w.rlvl_start + E + 2;
   w.n_lvl = w.cursor + E;
+  w.es = w.n_lvl + 2;
+  w.ee = w.es + E;
+  w.hoff = w.ee + E;
+  w.below = w.hoff + E;
+  w.pstart = w.below + E;
+  w.pre = w.pstart + E;
+  w.hmode = w.pre + kWavePreSlots;
+  w.moff = w.hmode + 1;
   return w;
 }
 
@@ -88,6 +122,11 @@ struct WaveCtx {
   const Env& e;
   const WaveLds& L;
   int lane, D, E, n_con, ne, nef;
+  // entity-block mass matrix entry (i, j) of entity ie, with the block geometry read from LDS
+  __device__ __forceinline__ float mm(int ie, int i, int j) const {
+    int st = L.es[ie], n = L.ee[ie] - st;
+    return e.s.mass_blk[e.ix(L.moff[ie] + (i - st) * n + (j - st))];
+  }
 };
 
 // entities touched by constraint row c (first and last stored dof; rows touch at most two entities)
@@ -104,19 +143,24 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
   const gsr_state& s = e.s;
   const WaveLds& L = w.L;
   for (int d = w.lane; d < w.D; d += kWave) L.ent_of_dof[d] = m.dof_entity[d];
+  for (int i = w.lane; i < w.E; i += kWave) {
+    L.es[i] = m.ent_dof_start[i];
+    L.ee[i] = m.ent_dof_end[i];
+    L.moff[i] = m.ent_mass_offset[i];
+  }
   for (int i = w.lane; i < 2 * w.E; i += kWave) L.pat[i] = 0ull;
   __syncthreads();
   if (w.lane == 0) {
     for (int ie = 0; ie < w.E; ie++)
-      if (m.ent_dof_end[ie] > m.ent_dof_start[ie]) L.pat[2 * ie + (ie >> 6)] |= 1ull << (ie & 63);
+      if (L.ee[ie] > L.es[ie]) L.pat[2 * ie + (ie >> 6)] |= 1ull << (ie & 63);
   }
   __syncthreads();
   for (int c = w.lane; c < w.n_con; c += kWave) {
     int ea, eb;
     row_entities(w, c, ea, eb);
     if (ea != eb) {
-      atomicOr(&L.pat[2 * ea + (eb >> 6)], 1ull << (eb & 63));
-      atomicOr(&L.pat[2 * eb + (ea >> 6)], 1ull << (ea & 63));
+      atomicOr((unsigned long long*)&L.pat[2 * ea + (eb >> 6)], 1ull << (eb & 63));
+      atomicOr((unsigned long long*)&L.pat[2 * eb + (ea >> 6)], 1ull << (ea & 63));
     }
   }
   __syncthreads();
@@ -161,8 +205,8 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
   for (int c = w.lane; c < w.n_con; c += kWave) {
     int r = rent[c];
     int ea = r & 0xffff, eb = r >> 16;
-    atomicAdd(&L.cursor[ea], 1);
-    if (eb != ea) atomicAdd(&L.cursor[eb], 1);
+    atomicAdd((int*)&L.cursor[ea], 1);
+    if (eb != ea) atomicAdd((int*)&L.cursor[eb], 1);
   }
   __syncthreads();
   if (w.lane == 0) {
@@ -204,10 +248,10 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
   }
   // level schedules of the pattern: forward (predecessors first) and backward (successors first)
   if (w.lane == 0) {
-    int* lvl = L.cursor;  // reuse as level scratch
+    lds_i* lvl = L.cursor;  // reuse as level scratch
     int maxl = -1;
     for (int ie = 0; ie < w.E; ie++) {
-      if (m.ent_dof_end[ie] <= m.ent_dof_start[ie]) { lvl[ie] = -1; continue; }
+      if (L.ee[ie] <= L.es[ie]) { lvl[ie] = -1; continue; }
       int lv = 0;
       for (int word = 0; word < 2; word++) {
         unsigned long long t = L.pat[2 * ie + word];
@@ -231,7 +275,7 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
     // backward levels
     maxl = -1;
     for (int ie = w.E - 1; ie >= 0; ie--) {
-      if (m.ent_dof_end[ie] <= m.ent_dof_start[ie]) { lvl[ie] = -1; continue; }
+      if (L.ee[ie] <= L.es[ie]) { lvl[ie] = -1; continue; }
       int lv = 0;
       for (int word = 0; word < 2; word++) {
         unsigned long long t = L.pat[2 * ie + word];
@@ -256,18 +300,63 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
       for (int ie = 0; ie < w.E; ie++)
         if (lvl[ie] == l) L.rlvl_order[pos++] = ie;
     L.n_lvl[1] = nl;
+    // LDS layout of the Hessian on this pattern, if it fits
+    int total = 0, slots = 0;
+    bool ok = true;
+    for (int ie = 0; ie < w.E; ie++) {
+      int n = L.ee[ie] - L.es[ie];
+      if (n <= 0) continue;
+      L.hoff[ie] = total;
+      L.pstart[ie] = slots;
+      int acc = 0;
+      for (int word = 0; word < 2; word++) {
+        unsigned long long t = L.pat[2 * ie + word];
+        while (t) {
+          int k = word * 64 + __ffsll((long long)t) - 1;
+          t &= t - 1;
+          if (k >= ie) break;
+          if (slots < kWavePreSlots) L.pre[slots] = acc;
+          else ok = false;
+          slots++;
+          acc += L.ee[k] - L.es[k];
+        }
+      }
+      L.below[ie] = acc;
+      total += n * acc + n * (n + 1) / 2;
+    }
+    L.hmode[0] = (ok && total <= kWaveHBudget) ? 1 : 0;
   }
   __syncthreads();
 }
 
+// ---- Hessian storage ----------------------------------------------------------------------------------------------
+// Both stores hand out a pointer p with H(i, k) = p[k - es[ek]] for the dofs k of entity ek <= ent(i) in the pattern
+// row of ent(i).
+struct HGlobal {
+  const Env& e;
+  const lds_i* es;
+  __device__ __forceinline__ float* blk(int i, int ei, int ek) const { return &e.H(i, es[ek]); }
+};
+
+struct HLds {
+  const WaveLds& L;
+  __device__ __forceinline__ float* blk(int i, int ei, int ek) const {
+    int r = i - L.es[ei];
+    lds_f* row = L.hl + L.hoff[ei] + r * L.below[ei] + r * (r + 1) / 2;
+    if (ek == ei) return (float*)(row + L.below[ei]);
+    unsigned long long w0 = L.pat[2 * ei], w1 = L.pat[2 * ei + 1];
+    int pc = ek < 64 ? __popcll(w0 & ((1ull << ek) - 1ull)) : __popcll(w0) + __popcll(w1 & ((1ull << (ek - 64)) - 1ull));
+    return (float*)(row + L.pre[L.pstart[ei] + pc]);
+  }
+};
+
 // M x (entity blocks), one dof per lane, serial inner sum in the reference order
-__device__ inline void wave_mass_mul(const WaveCtx& w, const float* x, float* out) {
+__device__ inline void wave_mass_mul(const WaveCtx& w, const lds_f* x, lds_f* out) {
   const Env& e = w.e;
-  const gsr_model& m = e.m;
   for (int i = w.lane; i < w.D; i += kWave) {
     int ie = w.L.ent_of_dof[i];
     float v = 0.f;
-    for (int j = m.ent_dof_start[ie]; j < m.ent_dof_end[ie]; j++) v += e.Mm(ie, i, j) * x[j];
+    for (int j = w.L.es[ie]; j < w.L.ee[ie]; j++) v += w.mm(ie, i, j) * x[j];
     out[i] = v;
   }
 }
@@ -300,26 +389,31 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, float& cost, flo
     ci += 0.5f * (ja * ja * Dc * (float)act);
   }
   __syncthreads();
-  // J^T f per dof, rows in ascending order through the entity row list
+  // J^T f per dof, rows in ascending order through the entity row list; one lane per entity reads each of its
+  // rows once and accumulates into the entity's own dofs (no other lane writes them)
   const int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
   const int* lst = s.ent_row_list + (size_t)e.b * 2 * e.m.max_rows;
-  float gi = 0.f;
-  for (int d = w.lane; d < w.D; d += kWave) {
-    int ie = L.ent_of_dof[d];
-    float q = 0.f;
+  for (int d = w.lane; d < w.D; d += kWave) L.qfrc[d] = 0.f;
+  __syncthreads();
+  for (int ie = w.lane; ie < w.E; ie += kWave) {
+    const int s0 = L.es[ie], s1 = L.ee[ie];
+    if (s1 <= s0) continue;
     for (int t = ptr[ie]; t < ptr[ie + 1]; t++) {
       int c = lst[t];
       int nnz = e.EI(s.jac_nnz, c);
-      for (int k = 0; k < nnz; k++)
-        if (e.JI(c, k) == d) {
-          q = q + e.JV(c, k) * e.EF(s.efc_force, c);
-          break;
-        }
+      float f = e.EF(s.efc_force, c);
+      for (int k = 0; k < nnz; k++) {
+        int d = e.JI(c, k);
+        if (d >= s0 && d < s1) L.qfrc[d] = L.qfrc[d] + e.JV(c, k) * f;
+      }
     }
-    L.qfrc[d] = q;
+  }
+  float gi = 0.f;
+  for (int d = w.lane; d < w.D; d += kWave) {
     float v = 0.5f * (L.Ma[d] - L.force[d]) * (L.qacc[d] - L.acc_smooth[d]);
     gi += v;
   }
+  __syncthreads();
   gauss = wave_sum(gi);
   cost = wave_sum(ci) + gauss;
   __syncthreads();
@@ -327,19 +421,19 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, float& cost, flo
 
 // candidate rows j > i of column i inside the pattern, enumerated by index t: returns -1 past the end
 __device__ __forceinline__ int wave_col_candidate(const WaveCtx& w, int i, int ei, int t) {
-  const gsr_model& m = w.e.m;
-  int rest = m.ent_dof_end[ei] - i - 1;
+  const WaveLds& L = w.L;
+  int rest = L.ee[ei] - i - 1;
   if (t < rest) return i + 1 + t;
   t -= rest;
   for (int word = (ei + 1) >> 6; word < 2; word++) {
-    unsigned long long bits = w.L.pat[2 * ei + word];
+    unsigned long long bits = L.pat[2 * ei + word];
     int lo = ei + 1 - word * 64;
     if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
     while (bits) {
       int ej = word * 64 + __ffsll((long long)bits) - 1;
       bits &= bits - 1;
-      int n = m.ent_dof_end[ej] - m.ent_dof_start[ej];
-      if (t < n) return m.ent_dof_start[ej] + t;
+      int n = L.ee[ej] - L.es[ej];
+      if (t < n) return L.es[ej] + t;
       t -= n;
     }
   }
@@ -347,21 +441,23 @@ __device__ __forceinline__ int wave_col_candidate(const WaveCtx& w, int i, int e
 }
 
 // sum_{k < kmax, block(ent(a), ent(k)) and block(ent(b), ent(k)) in pattern} H[a][k] * H[b][k], k ascending
-__device__ __forceinline__ float wave_row_dot(const WaveCtx& w, int a, int bb, int kmax) {
-  const Env& e = w.e;
-  const gsr_model& m = e.m;
-  int ea = w.L.ent_of_dof[a], eb = w.L.ent_of_dof[bb];
-  unsigned long long m0 = w.L.pat[2 * ea] & w.L.pat[2 * eb], m1 = w.L.pat[2 * ea + 1] & w.L.pat[2 * eb + 1];
+template <class HS>
+__device__ __forceinline__ float wave_row_dot(const WaveCtx& w, const HS& h, int a, int bb, int kmax) {
+  const WaveLds& L = w.L;
+  int ea = L.ent_of_dof[a], eb = L.ent_of_dof[bb];
+  unsigned long long m0 = L.pat[2 * ea] & L.pat[2 * eb], m1 = L.pat[2 * ea + 1] & L.pat[2 * eb + 1];
   float acc = 0.f;
   for (int word = 0; word < 2; word++) {
     unsigned long long t = word ? m1 : m0;
     while (t) {
       int ek = word * 64 + __ffsll((long long)t) - 1;
       t &= t - 1;
-      int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+      int k0 = L.es[ek], k1 = L.ee[ek];
       if (k0 >= kmax) return acc;
       if (k1 > kmax) k1 = kmax;
-      for (int k = k0; k < k1; k++) acc += e.H(a, k) * e.H(bb, k);
+      const float* pa = h.blk(a, ea, ek);
+      const float* pb = h.blk(bb, eb, ek);
+      for (int k = 0; k < k1 - k0; k++) acc += pa[k] * pb[k];
     }
   }
   return acc;
@@ -369,43 +465,45 @@ __device__ __forceinline__ float wave_row_dot(const WaveCtx& w, int a, int bb, i
 
 // Left-looking Cholesky over the pattern with the entries and summation orders of func_nt_chol_factor. Entities of
 // one forward level have no predecessor in common level, so one lane factorises a whole entity block column.
-__device__ inline void wave_chol_factor(const WaveCtx& w) {
-  const Env& e = w.e;
-  const gsr_model& m = e.m;
-  const float EPS = m.eps;
+template <class HS>
+__device__ __forceinline__ void wave_chol_factor(const WaveCtx& w, const HS& h) {
+  const float EPS = w.e.m.eps;
   const WaveLds& L = w.L;
   int nl = L.n_lvl[0];
   for (int lv = 0; lv < nl; lv++) {
     for (int idx = L.lvl_start[lv] + w.lane; idx < L.lvl_start[lv + 1]; idx += kWave) {
       int ei = L.lvl_order[idx];
-      int s0 = m.ent_dof_start[ei], s1 = m.ent_dof_end[ei];
+      int s0 = L.es[ei], s1 = L.ee[ei];
       for (int i = s0; i < s1; i++) {
         // diagonal: tmp = H_ii, then tmp -= H_ik^2 for k < i in ascending order, like the reference
-        float tmp = e.H(i, i);
+        float* di = h.blk(i, ei, ei);
+        float tmp = di[i - s0];
         for (int word = 0; word < 2; word++) {
           unsigned long long t = L.pat[2 * ei + word];
           bool stop = false;
           while (t) {
             int ek = word * 64 + __ffsll((long long)t) - 1;
             t &= t - 1;
-            int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+            int k0 = L.es[ek], k1 = L.ee[ek];
             if (k0 >= i) { stop = true; break; }
             if (k1 > i) k1 = i;
-            for (int k = k0; k < k1; k++) {
-              float h = e.H(i, k);
-              tmp -= h * h;
+            const float* pk = h.blk(i, ei, ek);
+            for (int k = 0; k < k1 - k0; k++) {
+              float v = pk[k];
+              tmp -= v * v;
             }
           }
           if (stop) break;
         }
         float hii = sqrtf(fmaxf(tmp, EPS));
-        e.H(i, i) = hii;
+        di[i - s0] = hii;
         float inv = 1.f / hii;
         for (int t = 0;; t++) {
           int j = wave_col_candidate(w, i, ei, t);
           if (j < 0) break;
-          float dt = wave_row_dot(w, j, i, i);
-          e.H(j, i) = (e.H(j, i) - dt) * inv;
+          float dt = wave_row_dot(w, h, j, i, i);
+          float* pj = h.blk(j, L.ent_of_dof[j], ei) + (i - s0);
+          *pj = (*pj - dt) * inv;
         }
       }
     }
@@ -413,51 +511,63 @@ __device__ inline void wave_chol_factor(const WaveCtx& w) {
   }
 }
 
-// H = M + sum_c act D J^T J on the pattern, then factor
-__device__ inline void wave_hessian_direct(const WaveCtx& w) {
+// H = M + sum_c act D J^T J on the pattern (func_nt_hessian_direct), one lane per entity: the lane walks its
+// entity's rows in ascending order once, so every entry still receives its terms in row order; then factor.
+template <class HS>
+__device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& h) {
   const Env& e = w.e;
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const WaveLds& L = w.L;
   const int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
   const int* lst = s.ent_row_list + (size_t)e.b * 2 * m.max_rows;
-  for (int i = w.lane; i < w.D; i += kWave) {
-    int ei = L.ent_of_dof[i];
-    // zero row i on the pattern
-    for (int word = 0; word < 2; word++) {
-      unsigned long long t = L.pat[2 * ei + word];
-      while (t) {
-        int ek = word * 64 + __ffsll((long long)t) - 1;
-        t &= t - 1;
-        int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
-        if (k0 > i) break;
-        if (k1 > i + 1) k1 = i + 1;
-        for (int k = k0; k < k1; k++) e.H(i, k) = 0.f;
+  for (int ei = w.lane; ei < w.E; ei += kWave) {
+    const int s0 = L.es[ei], s1 = L.ee[ei];
+    if (s1 <= s0) continue;
+    // zero the entity's rows on the pattern
+    for (int i = s0; i < s1; i++) {
+      for (int word = 0; word < 2; word++) {
+        unsigned long long t = L.pat[2 * ei + word];
+        while (t) {
+          int ek = word * 64 + __ffsll((long long)t) - 1;
+          t &= t - 1;
+          int k0 = L.es[ek], k1 = L.ee[ek];
+          if (k0 > i) break;
+          if (k1 > i + 1) k1 = i + 1;
+          float* pk = h.blk(i, ei, ek);
+          for (int k = 0; k < k1 - k0; k++) pk[k] = 0.f;
+        }
       }
     }
     for (int t = ptr[ei]; t < ptr[ei + 1]; t++) {
       int c = lst[t];
       int nnz = e.EI(s.jac_nnz, c);
-      int k1 = -1;
-      for (int k = 0; k < nnz; k++)
-        if (e.JI(c, k) == i) { k1 = k; break; }
-      if (k1 < 0) continue;
-      float v1 = e.JV(c, k1);
-      if (!(fabsf(v1) > m.eps)) continue;
       float Dc = e.EF(s.efc_D, c), act = (float)(e.EI(s.efc_active, c) & 1);
-      for (int k2 = 0; k2 <= k1; k2++) {
-        int i2 = e.JI(c, k2);
-        e.H(i, i2) = e.H(i, i2) + e.JV(c, k2) * v1 * Dc * act;
+      for (int k1 = 0; k1 < nnz; k1++) {
+        int i = e.JI(c, k1);
+        if (i < s0 || i >= s1) continue;
+        float v1 = e.JV(c, k1);
+        if (!(fabsf(v1) > m.eps)) continue;
+        for (int k2 = 0; k2 <= k1; k2++) {
+          int i2 = e.JI(c, k2);
+          int e2 = L.ent_of_dof[i2];
+          float* p2 = h.blk(i, ei, e2) + (i2 - L.es[e2]);
+          *p2 = *p2 + e.JV(c, k2) * v1 * Dc * act;
+        }
       }
     }
-    for (int j = m.ent_dof_start[ei]; j <= i; j++) e.H(i, j) = e.H(i, j) + e.Mm(ei, i, j);
+    for (int i = s0; i < s1; i++) {
+      float* pi = h.blk(i, ei, ei);
+      for (int j = s0; j <= i; j++) pi[j - s0] = pi[j - s0] + w.mm(ei, i, j);
+    }
   }
   __syncthreads();
-  wave_chol_factor(w);
+  wave_chol_factor(w, h);
 }
 
 // rank-1 update / downdate of the factor with row c (func_nt_hessian_incremental); returns false on rank loss
-__device__ inline bool wave_rank1(const WaveCtx& w, int c, float sgn) {
+template <class HS>
+__device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c, float sgn) {
   const Env& e = w.e;
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
@@ -474,10 +584,11 @@ __device__ inline bool wave_rank1(const WaveCtx& w, int c, float sgn) {
   bool rank_ok = true;
   int e_lo = ea < eb ? ea : eb;
   for (int ek = e_lo; ek < w.E; ek++) {
-    for (int k = m.ent_dof_start[ek]; k < m.ent_dof_end[ek]; k++) {
+    for (int k = L.es[ek]; k < L.ee[ek]; k++) {
       float vk = L.vec[k];
       if (!(fabsf(vk) > m.eps)) continue;
-      float Lkk = e.H(k, k);
+      float* dk = h.blk(k, ek, ek) + (k - L.es[ek]);
+      float Lkk = *dk;
       float tmp = Lkk * Lkk + vk * vk * sgn;
       if (tmp < m.eps) {
         tmp = m.eps;
@@ -485,13 +596,14 @@ __device__ inline bool wave_rank1(const WaveCtx& w, int c, float sgn) {
       }
       float r = sqrtf(tmp), cc = r / Lkk, cinv = 1.f / cc, sv = vk / Lkk;
       __syncthreads();
-      if (w.lane == 0) e.H(k, k) = r;
+      if (w.lane == 0) *dk = r;
       for (int t = w.lane;; t += kWave) {
         int i = wave_col_candidate(w, k, ek, t);
         if (i < 0) break;
-        float h = (e.H(i, k) + sv * L.vec[i] * sgn) * cinv;
-        e.H(i, k) = h;
-        L.vec[i] = L.vec[i] * cc - sv * h;
+        float* pi = h.blk(i, L.ent_of_dof[i], ek) + (k - L.es[ek]);
+        float hv = (*pi + sv * L.vec[i] * sgn) * cinv;
+        *pi = hv;
+        L.vec[i] = L.vec[i] * cc - sv * hv;
       }
       __syncthreads();
     }
@@ -501,16 +613,15 @@ __device__ inline bool wave_rank1(const WaveCtx& w, int c, float sgn) {
 
 // Mgrad = H^-1 grad through the factor (func_nt_chol_solve): forward over forward levels, backward over backward
 // levels; one lane per entity, each entry with the reference's summation order.
-__device__ inline void wave_chol_solve(const WaveCtx& w) {
-  const Env& e = w.e;
-  const gsr_model& m = e.m;
+template <class HS>
+__device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h) {
   const WaveLds& L = w.L;
   __syncthreads();
   int nl = L.n_lvl[0];
   for (int lv = 0; lv < nl; lv++) {
     for (int idx = L.lvl_start[lv] + w.lane; idx < L.lvl_start[lv + 1]; idx += kWave) {
       int ei = L.lvl_order[idx];
-      for (int i = m.ent_dof_start[ei]; i < m.ent_dof_end[ei]; i++) {
+      for (int i = L.es[ei]; i < L.ee[ei]; i++) {
         float g = L.grad[i];
         for (int word = 0; word < 2; word++) {
           unsigned long long t = L.pat[2 * ei + word];
@@ -518,14 +629,15 @@ __device__ inline void wave_chol_solve(const WaveCtx& w) {
           while (t) {
             int ek = word * 64 + __ffsll((long long)t) - 1;
             t &= t - 1;
-            int k0 = m.ent_dof_start[ek], k1 = m.ent_dof_end[ek];
+            int k0 = L.es[ek], k1 = L.ee[ek];
             if (k0 >= i) { stop = true; break; }
             if (k1 > i) k1 = i;
-            for (int j = k0; j < k1; j++) g = g - e.H(i, j) * L.Mgrad[j];
+            const float* pk = h.blk(i, ei, ek);
+            for (int j = k0; j < k1; j++) g = g - pk[j - k0] * L.Mgrad[j];
           }
           if (stop) break;
         }
-        L.Mgrad[i] = g / e.H(i, i);
+        L.Mgrad[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
       }
     }
     __syncthreads();
@@ -534,24 +646,25 @@ __device__ inline void wave_chol_solve(const WaveCtx& w) {
   for (int lv = 0; lv < nl; lv++) {
     for (int idx = L.rlvl_start[lv] + w.lane; idx < L.rlvl_start[lv + 1]; idx += kWave) {
       int ei = L.rlvl_order[idx];
-      for (int i = m.ent_dof_end[ei] - 1; i >= m.ent_dof_start[ei]; i--) {
+      for (int i = L.ee[ei] - 1; i >= L.es[ei]; i--) {
         float g = L.Mgrad[i];
         for (int t = 0;; t++) {
           int j = wave_col_candidate(w, i, ei, t);
           if (j < 0) break;
-          g = g - e.H(j, i) * L.Mgrad[j];
+          g = g - h.blk(j, L.ent_of_dof[j], ei)[i - L.es[ei]] * L.Mgrad[j];
         }
-        L.Mgrad[i] = g / e.H(i, i);
+        L.Mgrad[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
       }
     }
     __syncthreads();
   }
 }
 
-__device__ inline void wave_update_gradient(const WaveCtx& w) {
+template <class HS>
+__device__ __forceinline__ void wave_update_gradient(const WaveCtx& w, const HS& h) {
   const WaveLds& L = w.L;
   for (int d = w.lane; d < w.D; d += kWave) L.grad[d] = L.Ma[d] - L.force[d] - L.qfrc[d];
-  wave_chol_solve(w);
+  wave_chol_solve(w, h);
 }
 
 struct WLS {
@@ -679,12 +792,86 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
 // Above this many activity changes in one Newton iteration the factor is rebuilt instead of updated row by row.
 constexpr int kMaxRank1PerIter = 8;
 
-__device__ inline void wave_solve(const Env& e, char* lds) {
+// Newton iterations (func_solve / func_solve_body) from the initial factorisation on; returns the iteration count
+template <class HS>
+__device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, float cost, float gauss) {
+  const Env& e = w.e;
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const WaveLds& L = w.L;
+  const int lane = w.lane, D = w.D, n_con = w.n_con;
+  wave_hessian_direct(w, h);
+  wave_update_gradient(w, h);
+  for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
+  __syncthreads();
+  int iters = 0;
+  if (n_con <= 0) return 0;
+  for (int it = 0; it < m.iterations; it++) {
+    iters++;
+    float alpha = wave_linesearch(w, gauss);
+    bool improved;
+    if (fabsf(alpha) < m.eps) {
+      improved = false;
+    } else {
+      for (int d = lane; d < D; d += kWave) {
+        L.qacc[d] = L.qacc[d] + L.search[d] * alpha;
+        L.Ma[d] = L.Ma[d] + L.mv[d] * alpha;
+      }
+      for (int c = lane; c < n_con; c += kWave) e.EF(s.efc_Jaref, c) = e.EF(s.efc_Jaref, c) + e.EF(s.efc_jv, c) * alpha;
+      __syncthreads();
+      float prev_cost = cost;
+      wave_update_constraint(w, cost, gauss);
+      // activity changes in row order; rank-1 updates or a rebuild of the factor
+      int n_changed = 0;
+      for (int c = lane; c < n_con; c += kWave) {
+        int a = e.EI(s.efc_active, c);
+        n_changed += ((a & 1) != ((a >> 1) & 1));
+      }
+      for (int o = kWave / 2; o > 0; o >>= 1) n_changed += __shfl_xor(n_changed, o, kWave);
+      if (n_changed > kMaxRank1PerIter) {
+        wave_hessian_direct(w, h);
+      } else if (n_changed > 0) {
+        bool rebuilt = false;
+        for (int c0 = 0; c0 < n_con && !rebuilt; c0 += kWave) {
+          int c = c0 + lane;
+          int a = c < n_con ? e.EI(s.efc_active, c) : 0;
+          unsigned long long chg = __ballot((a & 1) != ((a >> 1) & 1));
+          while (chg && !rebuilt) {
+            int j = __ffsll((long long)chg) - 1;
+            chg &= chg - 1;
+            int aj = __shfl(a, j, kWave);
+            if (!wave_rank1(w, h, c0 + j, (aj & 1) ? 1.f : -1.f)) {
+              wave_hessian_direct(w, h);
+              rebuilt = true;
+            }
+          }
+        }
+      }
+      wave_update_gradient(w, h);
+      float tol_scaled = (m.meaninertia * (float)(D > 1 ? D : 1)) * m.tolerance;
+      float improvement = prev_cost - cost;
+      float g2 = 0.f;
+      for (int d = lane; d < D; d += kWave) g2 += L.grad[d] * L.grad[d];
+      float gn = sqrtf(wave_sum(g2));
+      if (gn < tol_scaled || improvement < tol_scaled) {
+        improved = false;
+      } else {
+        improved = true;
+        for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
+      }
+      __syncthreads();
+    }
+    if (!improved) break;
+  }
+  return iters;
+}
+
+__device__ __forceinline__ void wave_solve(const Env& e, char* lds) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const int lane = threadIdx.x;
   const int D = m.n_dofs, E = m.n_entities;
-  WaveLds L = wave_lds_carve(lds, D, E);
+  WaveLds L = wave_lds_carve((lds_c*)lds, D, E);
   int n_con = e.I(s.n_con, 0), ne = e.I(s.n_con, 1), nef = ne + e.I(s.n_con, 2);
   WaveCtx w{e, L, lane, D, E, n_con, ne, nef};
   int ws = e.I(s.is_warmstart, 0);
@@ -705,64 +892,7 @@ __device__ inline void wave_solve(const Env& e, char* lds) {
   __syncthreads();
   float cost, gauss;
   wave_update_constraint(w, cost, gauss);
-  wave_hessian_direct(w);
-  wave_update_gradient(w);
-  for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
-  __syncthreads();
-  int iters = 0;
-  if (n_con > 0) {
-    for (int it = 0; it < m.iterations; it++) {
-      iters++;
-      float alpha = wave_linesearch(w, gauss);
-      bool improved;
-      if (fabsf(alpha) < m.eps) {
-        improved = false;
-      } else {
-        for (int d = lane; d < D; d += kWave) {
-          L.qacc[d] = L.qacc[d] + L.search[d] * alpha;
-          L.Ma[d] = L.Ma[d] + L.mv[d] * alpha;
-        }
-        for (int c = lane; c < n_con; c += kWave) e.EF(s.efc_Jaref, c) = e.EF(s.efc_Jaref, c) + e.EF(s.efc_jv, c) * alpha;
-        __syncthreads();
-        float prev_cost = cost;
-        wave_update_constraint(w, cost, gauss);
-        // activity changes in row order; rank-1 updates or a rebuild of the factor
-        int n_changed = 0;
-        for (int c = lane; c < n_con; c += kWave) {
-          int a = e.EI(s.efc_active, c);
-          n_changed += ((a & 1) != ((a >> 1) & 1));
-        }
-        for (int o = kWave / 2; o > 0; o >>= 1) n_changed += __shfl_xor(n_changed, o, kWave);
-        if (n_changed > kMaxRank1PerIter) {
-          wave_hessian_direct(w);
-        } else if (n_changed > 0) {
-          for (int c = 0; c < n_con; c++) {
-            int a = e.EI(s.efc_active, c);
-            int act = a & 1, prev = (a >> 1) & 1;
-            if (act == prev) continue;
-            if (!wave_rank1(w, c, act ? 1.f : -1.f)) {
-              wave_hessian_direct(w);
-              break;
-            }
-          }
-        }
-        wave_update_gradient(w);
-        float tol_scaled = (m.meaninertia * (float)(D > 1 ? D : 1)) * m.tolerance;
-        float improvement = prev_cost - cost;
-        float g2 = 0.f;
-        for (int d = lane; d < D; d += kWave) g2 += L.grad[d] * L.grad[d];
-        float gn = sqrtf(wave_sum(g2));
-        if (gn < tol_scaled || improvement < tol_scaled) {
-          improved = false;
-        } else {
-          improved = true;
-          for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
-        }
-        __syncthreads();
-      }
-      if (!improved) break;
-    }
-  }
+  int iters = L.hmode[0] ? wave_newton(w, HLds{L}, cost, gauss) : wave_newton(w, HGlobal{e, L.es}, cost, gauss);
   // epilogue (func_update_qacc, func_update_contact_force)
   int bad = 0;
   for (int d = lane; d < D; d += kWave) {

@@ -99,7 +99,7 @@ def finalize_model(M: RigidModel):
 class DeviceModel:
     """Static tables of one scene type uploaded once (torch-owned device memory) + the gsr_model descriptor."""
 
-    def __init__(self, M: RigidModel, device, solver_variant=None, collide_variant=None):
+    def __init__(self, M: RigidModel, device, solver_variant=None, collide_variant=None, dyn_variant=None):
         finalize_model(M)
         self.M = M
         self.device = torch.device(device)
@@ -124,6 +124,7 @@ class DeviceModel:
         if d.solver_variant == 1 and M.n_entities > WAVE_MAX_ENTITIES:
             raise ValueError(f"wavefront solver supports at most {WAVE_MAX_ENTITIES} entities")
         d.collide_variant = d.solver_variant if collide_variant is None else int(collide_variant)
+        d.dyn_variant = d.solver_variant if dyn_variant is None else int(dyn_variant)
         d.dt, d.tolerance, d.ls_tolerance = o.dt, o.tolerance, o.ls_tolerance
         d.eps = float(np.finfo(np.float32).eps)
         d.meaninertia = float(M.meaninertia)
@@ -173,7 +174,8 @@ class DeviceState:
 class RigidSolver:
     """Batched rigid solver for one scene type on one device."""
 
-    def __init__(self, model: RigidModel, n_envs: int, device="cuda", solver_variant=None, collide_variant=None):
+    def __init__(self, model: RigidModel, n_envs: int, device="cuda", solver_variant=None, collide_variant=None,
+                 dyn_variant=None):
         self.model = model
         self.n_envs = n_envs
         self._B = max(1, n_envs)
@@ -181,7 +183,7 @@ class RigidSolver:
         if self.device.type != "cuda":
             raise RuntimeError("the rigid step runs on a HIP device; there is no CPU execution path")
         self._lib = _lib.load()
-        self.dm = DeviceModel(model, self.device, solver_variant, collide_variant)
+        self.dm = DeviceModel(model, self.device, solver_variant, collide_variant, dyn_variant)
         self.st = DeviceState(self.dm, self._B)
         self._errno_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._substep = 0

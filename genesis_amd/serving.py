@@ -144,8 +144,9 @@ class Server:
             groups += [Group(p.S, b, n, p.envs_per_tile) for b, n in runs]
         return groups
 
-    def run(self, requests: List[Request], max_ticks: Optional[int] = None, idle_sleep: float = 2e-4):
-        """Serve `requests` (arrival times relative to the start) to completion. Returns a stats dict."""
+    def run(self, requests: List[Request], max_ticks: Optional[int] = None, idle_sleep: float = 2e-4, log=None):
+        """Serve `requests` (arrival times relative to the start) to completion. Returns a stats dict.
+        `log`, if given, receives a progress line every 2 s."""
         pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         queue: collections.deque = collections.deque()
         active: List[Request] = []
@@ -153,8 +154,12 @@ class Server:
         fb, dirty = None, True
         ev = torch.cuda.Event()
         t_start = time.perf_counter()
+        t_log = 0.0
         while pending or queue or active:
             now = time.perf_counter() - t_start
+            if log is not None and now - t_log > 2.0:
+                t_log = now
+                log(f"  t={now:.1f}s ticks {ticks} active {len(active)} queued {len(queue)} pending {len(pending)}")
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
             while queue and self._try_admit(queue[0], now):

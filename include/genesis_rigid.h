@@ -55,6 +55,8 @@ typedef struct gsr_model {
   int32_t enable_collision, enable_joint_limit, enable_multi_contact;
   int32_t solver_variant;  /* 0: one lane per env (rows env-minor), 1: one wavefront per env (rows env-major) */
   int32_t collide_variant; /* 0: one lane per env, 1: one wavefront per env (pairs across lanes) */
+  int32_t dyn_variant;     /* step_1 / step_2: 0 one lane per env, 1 one wavefront per env (entities across lanes) */
+  int32_t pad_variant;
   float dt, tolerance, ls_tolerance, eps, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
   float gravity[4];
   /* links */
