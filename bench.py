@@ -3,7 +3,7 @@
 Prints ONE JSON line. A step is one scene.step() (one substep) of every env of the workload, inputs resident in HBM.
 Workloads (seeded synthetic scenes, random-init state; no datasets):
   pile       64 free boxes per env (edges U[0.05, 0.2] m, 1000 kg/m^3, friction 1.0) falling in a 1 m column onto a
-             plane, 4096 envs, dt = 0.01                                    [default: the headline configuration]
+             plane, 4096 envs, dt = 0.01, timed over its 500 steps           [default: the headline configuration]
   arm        9-DoF Panda arm + plane, 64 envs, dt = 0.01, PD targets U(limits) resampled every 100 steps
   quadruped  go2 quadruped + plane, 32768 envs, dt = 0.005, PD targets standing + U(-0.3, 0.3) every step
   mixed      batching form: 16384 envs split (seed 3) among arm / quadruped / 20-box pile, dt = 0.01; times the plain
@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 KERNELS = ["k_step1", "k_collide", "k_solve", "k_step2"]
+HORIZON = {"pile": 500, "arm": 1000, "quadruped": 1000}  # steps of each configuration
 
 
 def b_alg(M, n_ctrl):
@@ -46,11 +47,12 @@ def make(workload, n_envs):
     return w, compile_model(w.entities, w.options)
 
 
-def cpu_baseline(w, M, seconds=15.0):
-    """The CPU oracle (scalar restatement of the reference step, FP32) on the host cores, env-chunked threads."""
+def cpu_baseline(w, M, steps, seconds=20.0):
+    """The CPU oracle (scalar restatement of the reference step, FP32) on the host cores, one env per thread, over
+    the same trajectory as the timed GPU run: from the seeded initial state, `steps` steps or until `seconds` pass."""
     from oracle.oracle import Oracle
     threads = max(1, min(16, os.cpu_count() or 1))
-    n = max(threads, 16)
+    n = threads
     O = Oracle(M, n, 32)
     if w.init_qpos is not None:
         O.set("qpos", w.init_qpos(n, M.n_qs))
@@ -58,13 +60,13 @@ def cpu_baseline(w, M, seconds=15.0):
     if w.name == "quadruped":
         ctrl_mode[:, :6] = 0
     O.set("ctrl_mode", ctrl_mode)
-    steps, t0 = 0, time.time()
-    while time.time() - t0 < seconds:
+    done, t0 = 0, time.time()
+    while done < steps and time.time() - t0 < seconds:
         O.run(1, n_threads=threads)
-        steps += 1
+        done += 1
     dt = time.time() - t0
-    return {"value": n * steps / dt, "unit": "env-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} envs x {steps} steps of the same workload from step 0 on the CPU oracle, {dt:.1f} s"}
+    return {"value": n * done / dt, "unit": "env-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} envs x steps 0-{done - 1} of the same workload on the CPU oracle, {dt:.1f} s"}
 
 
 def bench_mixed(args):
@@ -223,8 +225,8 @@ def calibrate_copy(mb):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's horizon)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="pile", choices=["pile", "arm", "quadruped", "mixed", "serve"])
     ap.add_argument("--serve-seconds", type=float, default=6.0, help="arrival window per offered-load level")
     ap.add_argument("--serve-budget-gb", type=float, default=12.0, help="device memory budget for admission")
@@ -242,6 +244,7 @@ def main():
         sys.exit(2)
 
     if args.workload == "mixed":
+        args.steps = args.steps or 100
         return bench_mixed(args)
     if args.workload == "serve":
         return bench_serve(args)
@@ -251,14 +254,22 @@ def main():
     from genesis_amd.workloads import Control, init_state
 
     w, M = make(args.workload, args.n_envs)
+    # the timed steps are the workload's own trajectory from its seeded initial state (the pile: free fall and
+    # settling); warm-up steps run first on the same state, which is then put back
+    args.steps = args.steps or HORIZON[w.name]
     S = RigidSolver(M, w.n_envs if not args.n_envs else args.n_envs, "cuda", solver_variant=args.solver_variant)
     init_state(w, M, S)
+    snap = {k: v.clone() for k, v in S.st.t.items()}
     ctrl = Control(w, M, S, args.seed)
-    step = 0
-    for _ in range(args.warmup):
+    for step in range(args.warmup):
         ctrl(step)
         S.substep()
-        step += 1
+    torch.cuda.synchronize()
+    for k, v in snap.items():
+        S.st.t[k].copy_(v)
+    del snap
+    ctrl = Control(w, M, S, args.seed)
+    step = 0
     torch.cuda.synchronize()
 
     # every kernel launch of the timed region is bracketed by events on the launch stream (torch's current stream)
@@ -307,7 +318,7 @@ def main():
                      "step_frac": balg * value / 1e9 / HBM_PEAK_GBS, "b_alg_per_env_step": balg},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, M)
+        out["cpu_baseline"] = cpu_baseline(w, M, args.steps)
     print(json.dumps(out))
 
 

@@ -15,6 +15,7 @@ HEADER = os.path.join(ROOT, "include", "genesis_rigid.h")
 LIB_PATH = os.path.join(HERE, "_build", "libgenesis_rigid.so")
 
 EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_group_size", "gsr_batch_info_size",
+           "gsr_gjk_workspace_words",
            "gsr_step", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno", "gsr_run_stage", "gsr_launch_kernel", "gsr_batch_table_bytes",
            "gsr_batch_upload", "gsr_step_fused", "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy"]
 

@@ -27,6 +27,20 @@ constexpr int kBlock = 64;
 
 __device__ __forceinline__ int env_id() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// Workgroup -> env for the one-env-per-workgroup kernels. The dispatcher deals workgroups to the 8 XCDs round-robin
+// (workgroup i to XCD i % 8) and every XCD has its own L2, while the env-minor state arrays put 32 neighbouring envs
+// in one 128-byte line. Within each chunk of 256 workgroups, XCD x therefore gets a run of 32 consecutive envs, so
+// the waves it holds at once share their lines in its L2 instead of each XCD fetching every line. A bijection on
+// [0, n): a partial last chunk is split the same way over its m workgroups.
+__device__ __forceinline__ int xcd_env(int blk, int n) {
+  constexpr int kRun = 32, kChunk = 8 * kRun;
+  const int base = blk - blk % kChunk;
+  const int m = min(kChunk, n - base);
+  const int j = blk - base;
+  const int q = m >> 3, r = m & 7, x = j & 7, k = j >> 3;
+  return base + x * q + min(x, r) + k;
+}
+
 __global__ void __launch_bounds__(kBlock) k_kinematics(gsr_model m, gsr_state s, const int32_t* envs_idx, int n_idx) {
   int t = env_id();
   if (t >= n_idx) return;
@@ -63,15 +77,15 @@ __device__ __forceinline__ void wave_step2(const Env& e) {
 }
 
 __global__ void __launch_bounds__(kWave) k_step1_wave(gsr_model m, gsr_state s) {
-  int b = blockIdx.x;
-  if (b >= s.n_envs) return;  // uniform across the block
+  if ((int)blockIdx.x >= s.n_envs) return;  // uniform across the block
+  const int b = xcd_env(blockIdx.x, s.n_envs);
   Env e(m, s, b);
   wave_step1(e);
 }
 
 __global__ void __launch_bounds__(kWave) k_step2_wave(gsr_model m, gsr_state s) {
-  int b = blockIdx.x;
-  if (b >= s.n_envs) return;
+  if ((int)blockIdx.x >= s.n_envs) return;  // uniform across the block
+  const int b = xcd_env(blockIdx.x, s.n_envs);
   Env e(m, s, b);
   wave_step2(e);
 }
@@ -87,8 +101,8 @@ __global__ void __launch_bounds__(kBlock) k_collide(gsr_model m, gsr_state s) {
 
 __global__ void __launch_bounds__(kWave) k_collide_wave(gsr_model m, gsr_state s) {
   extern __shared__ char lds[];
-  int b = blockIdx.x;
-  if (b >= s.n_envs) return;  // uniform across the block
+  if ((int)blockIdx.x >= s.n_envs) return;  // uniform across the block
+  const int b = xcd_env(blockIdx.x, s.n_envs);
   Env e(m, s, b);
   wave_collide(e, lds);
 }
@@ -102,8 +116,8 @@ __global__ void __launch_bounds__(kBlock) k_solve(gsr_model m, gsr_state s) {
 
 __global__ void __launch_bounds__(kWave) k_solve_wave(gsr_model m, gsr_state s) {
   extern __shared__ char lds[];
-  int b = blockIdx.x;
-  if (b >= s.n_envs) return;  // uniform across the block
+  if ((int)blockIdx.x >= s.n_envs) return;  // uniform across the block
+  const int b = xcd_env(blockIdx.x, s.n_envs);
   Env e(m, s, b);
   wave_solve(e, lds);
 }
@@ -257,7 +271,7 @@ k_fused(const char* __restrict__ table, uint32_t* queue, int persistent) {
   const BatchRange* __restrict__ R = reinterpret_cast<const BatchRange*>(table + kRangesOffset);
   const int n_tiles = H->n_tiles, n_groups = H->n_groups;
   if (!persistent) {
-    if ((int)blockIdx.x < n_tiles) fused_tile(types, R, n_groups, blockIdx.x, lds);
+    if ((int)blockIdx.x < n_tiles) fused_tile(types, R, n_groups, xcd_env(blockIdx.x, n_tiles), lds);
     return;
   }
   for (;;) {
@@ -362,6 +376,7 @@ int gsr_model_size(void) { return (int)sizeof(gsr_model); }
 int gsr_state_size(void) { return (int)sizeof(gsr_state); }
 int gsr_group_size(void) { return (int)sizeof(gsr_group); }
 int gsr_batch_info_size(void) { return (int)sizeof(gsr_batch_info); }
+int gsr_gjk_workspace_words(void) { return kGjkWsWords; }
 
 int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream) {
   int rc = check(model, state);

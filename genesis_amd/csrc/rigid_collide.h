@@ -5,6 +5,8 @@
 // convex pair in broadphase order first, plane-box pairs second.
 #pragma once
 #include "rigid_env.h"
+#include "rigid_support.h"
+#include "rigid_gjk.h"
 
 namespace gsr {
 
@@ -134,25 +136,37 @@ struct GlobalSink {
   // position of the k-th most recent contact (k = 0: the last one)
   __device__ f3 recent_pos(int k) const { return e.L3(e.s.contact_pos, e.I(e.s.n_contacts, 0) - 1 - k); }
   __device__ void flag(uint32_t bits) { e.I((int32_t*)e.s.errno_env, 0) |= (int)bits; }
+  __device__ static constexpr bool gjk_allowed() { return true; }
+  __device__ void defer() {}
 };
 
 constexpr int kMaxPairContacts = 8;
 
-struct LocalSink {
+struct PairContacts {
   f3 n[kMaxPairContacts], p[kMaxPairContacts];
   float pen[kMaxPairContacts];
   int cnt;
   uint32_t err;
+  bool deferred;  // the pair needs GJK/EPA: redo it when the env's GJK workspace is free
+};
+
+// kGjk: this lane owns the env's GJK workspace; without it a pair that needs GJK/EPA is only marked deferred (and the
+// GJK code is not compiled into that instance at all)
+template <bool kGjk>
+struct LocalSink {
+  PairContacts& c;
   __device__ void add(int, int, f3 nn, f3 pp, float pe) {
-    if (cnt < kMaxPairContacts) {
-      n[cnt] = nn;
-      p[cnt] = pp;
-      pen[cnt] = pe;
-      cnt++;
+    if (c.cnt < kMaxPairContacts) {
+      c.n[c.cnt] = nn;
+      c.p[c.cnt] = pp;
+      c.pen[c.cnt] = pe;
+      c.cnt++;
     }
   }
-  __device__ f3 recent_pos(int k) const { return p[cnt - 1 - k]; }
-  __device__ void flag(uint32_t bits) { err |= bits; }
+  __device__ f3 recent_pos(int k) const { return c.p[c.cnt - 1 - k]; }
+  __device__ void flag(uint32_t bits) { c.err |= bits; }
+  __device__ static constexpr bool gjk_allowed() { return kGjk; }
+  __device__ void defer() { c.deferred = true; }
 };
 
 __device__ inline float dev_tolerance(const gsr_model& m, int ga, int gb, float tol) {
@@ -160,32 +174,6 @@ __device__ inline float dev_tolerance(const gsr_model& m, int ga, int gb, float 
   float sz = sb;
   if (m.geom_type[ga] != GEOM_PLANE) sz = fminf(norm(T3(m.geom_init_aabb, 8 * ga + 7) - T3(m.geom_init_aabb, 8 * ga)), sb);
   return 0.5f * tol * sz;
-}
-
-// Geometry pose used by the narrowphase (may be perturbed for multi-contact); kept in registers.
-struct GPose {
-  f3 p;
-  q4 q;
-};
-
-__device__ inline f3 dev_support(const gsr_model& m, int g, const GPose& P, f3 d) {
-  int t = m.geom_type[g];
-  const float* data = m.geom_data + 7 * g;
-  if (t == GEOM_SPHERE) return P.p + d * data[0];
-  if (t == GEOM_CAPSULE) {
-    float r = data[0], hl = 0.5f * data[1];
-    f3 axis = qrot(f3{0.f, 0.f, 1.f}, P.q);
-    float side = dot(d, axis) < 0.f ? -1.f : 1.f;
-    f3 ep = P.p + (hl * side) * axis;
-    return ep + d * r;
-  }
-  if (t == GEOM_BOX) {
-    f3 db = qrot_inv(d, P.q);
-    f3 v{(db.x < 0.f ? -1.f : 1.f) * data[0] * 0.5f, (db.y < 0.f ? -1.f : 1.f) * data[1] * 0.5f,
-         (db.z < 0.f ? -1.f : 1.f) * data[2] * 0.5f};
-    return xform(v, P.p, P.q);
-  }
-  return P.p;
 }
 
 struct MPRSimplex {
@@ -413,6 +401,8 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb, Sink& snk
   int n_con = 0;
   int gmin = ga < gb ? ga : gb, gmax = ga < gb ? gb : ga;
   int i_pair = m.pair_idx[(size_t)gmin * m.n_geoms + gmax];
+  // the pair's normal cache, read once and written back at the end (det 0 sets it, later detections read it)
+  f3 cache = e.L3(s.cache_normal, i_pair);
   for (int det = 0; det < 5; det++) {
     GPose A = A0, Bp = B0;
     if (multi && is_col0) {
@@ -431,7 +421,7 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb, Sink& snk
         is_col = pen > 0.f;
       } else {
         bool updated = false;
-        f3 nws = e.L3(s.cache_normal, i_pair);
+        f3 nws = cache;
         bool guess = fabsf(nws.x) > EPS || fabsf(nws.y) > EPS || fabsf(nws.z) > EPS;
         for (int im = 0; im < 2; im++) {
           if (im == 1 && det == 0 && !is_col && guess) {
@@ -446,7 +436,21 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb, Sink& snk
         }
         if (pen > tol) {
           bool prefer_gjk = !guess || (m.mc_tolerance * pen >= m.mpr_to_gjk * tol);
-          if (prefer_gjk) snk.flag(GSR_ERR_DEEP_PENETRATION);
+          if (prefer_gjk) {
+            if (!snk.gjk_allowed()) {
+              snk.defer();
+              return;
+            }
+            snk.flag(GSR_ERR_DEEP_PENETRATION);
+            float gpen;
+            f3 gn, gp;
+            is_col = dev_gjk_contact(m, gjk_ws(e), ga, gb, A, Bp, gpen, gn, gp);
+            pen = gpen;
+            if (is_col) {
+              cpos = gp;
+              normal = gn;
+            }
+          }
         }
       }
     }
@@ -458,9 +462,9 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb, Sink& snk
           dev_contact_orthogonals(e, ga, gb, normal, ax0, ax1);
           n_con = 1;
         }
-        e.S3(s.cache_normal, i_pair, normal);
+        cache = normal;
       } else {
-        e.S3(s.cache_normal, i_pair, zero3());
+        cache = zero3();
       }
     } else if (multi && is_col0 && is_col) {
       f3 cpa = qrot((cpos - 0.5f * pen * normal) - cp0, qinv(qr)) + cp0;
@@ -484,6 +488,7 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb, Sink& snk
       }
     }
   }
+  e.S3(s.cache_normal, i_pair, cache);
 }
 
 template <class Sink>

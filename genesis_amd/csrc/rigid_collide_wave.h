@@ -147,37 +147,53 @@ __device__ inline int wave_narrow_pass(const Env& e, int lane, int nb, bool plan
   const gsr_state& s = e.s;
   for (int p0 = 0; p0 < nb; p0 += 64) {
     const int ip = p0 + lane;
-    LocalSink snk;
-    snk.cnt = 0;
-    snk.err = 0;
+    PairContacts pc;
+    pc.cnt = 0;
+    pc.err = 0;
+    pc.deferred = false;
     int ga = 0, gb = 0;
+    bool pb = false;
     if (ip < nb) {
       int pr = e.I(s.broad_pairs, ip);
       ga = pr >> 16;
       gb = pr & 0xffff;
       if (m.geom_type[ga] > m.geom_type[gb]) { int t = ga; ga = gb; gb = t; }
-      bool pb = m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX;
+      pb = m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX;
       if (pb == plane_box_pass) {
+        LocalSink<false> snk{pc};
         if (pb) dev_plane_box(e, ga, gb, snk);
         else dev_convex_convex(e, ga, gb, snk);
       }
     }
+    // pairs that need GJK/EPA share the env's workspace: redo them one lane at a time, with the workspace
+    unsigned long long def = __ballot(pc.deferred);
+    while (def) {
+      int j = __ffsll((long long)def) - 1;
+      def &= def - 1;
+      if (lane == j) {
+        pc.cnt = 0;
+        pc.err = 0;
+        pc.deferred = false;
+        LocalSink<true> snk{pc};
+        dev_convex_convex(e, ga, gb, snk);
+      }
+    }
     int total;
-    int k = base + wave_excl_scan(snk.cnt, lane, total);
+    int k = base + wave_excl_scan(pc.cnt, lane, total);
     float fr = fmaxf(fmaxf(m.geom_friction[ga] * 1.f, m.geom_friction[gb] * 1.f), 1e-2f);
-    for (int c = 0; c < snk.cnt; c++, k++) {
+    for (int c = 0; c < pc.cnt; c++, k++) {
       if (k < m.max_contacts) {
         e.I(s.contact_geoms, 2 * k) = ga;
         e.I(s.contact_geoms, 2 * k + 1) = gb;
-        e.S3(s.contact_normal, k, snk.n[c]);
-        e.S3(s.contact_pos, k, snk.p[c]);
-        e.F(s.contact_pen, k) = snk.pen[c];
+        e.S3(s.contact_normal, k, pc.n[c]);
+        e.S3(s.contact_pos, k, pc.p[c]);
+        e.F(s.contact_pen, k) = pc.pen[c];
         e.F(s.contact_friction, k) = fr;
       } else {
-        snk.err |= GSR_ERR_CONTACT_OVERFLOW;
+        pc.err |= GSR_ERR_CONTACT_OVERFLOW;
       }
     }
-    err |= snk.err;
+    err |= pc.err;
     base += total;
   }
   return base;

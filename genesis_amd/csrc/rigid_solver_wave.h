@@ -31,6 +31,13 @@ __device__ __forceinline__ float wave_sum(float v) {
   for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
   return v;
 }
+// Sums over rows or dofs whose rounding steers discrete decisions (the line search's bracketing, the Newton stop
+// test) are accumulated in double: the reference adds them one by one in FP32, an order a wavefront cannot follow,
+// and a tree of FP32 partial sums has a different rounding error; the double sum is closer to both.
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
 __device__ __forceinline__ int wave_or(int v) {
   for (int o = kWave / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
   return v;
@@ -362,11 +369,11 @@ __device__ inline void wave_mass_mul(const WaveCtx& w, const lds_f* x, lds_f* ou
 }
 
 // func_update_constraint; returns (cost, gauss)
-__device__ inline void wave_update_constraint(const WaveCtx& w, float& cost, float& gauss) {
+__device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, float& gauss) {
   const Env& e = w.e;
   const gsr_state& s = e.s;
   const WaveLds& L = w.L;
-  float ci = 0.f;
+  double ci = 0.0;
   for (int c = w.lane; c < w.n_con; c += kWave) {
     int act_prev = e.EI(s.efc_active, c) & 1;
     int act = 1;
@@ -379,14 +386,14 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, float& cost, flo
       floss_force = (float)ln * f + (float)lp * -f;
       float fc = (float)ln * f * (-0.5f * rf - ja);
       fc += (float)lp * f * (-0.5f * rf + ja);
-      ci += fc;
+      ci += (double)fc;
     } else if (w.nef <= c) {
       act = ja < 0.f;
     }
     e.EI(s.efc_active, c) = act | (act_prev << 1);
     float Dc = e.EF(s.efc_D, c);
     e.EF(s.efc_force, c) = floss_force + (-ja * Dc * (float)act);
-    ci += 0.5f * (ja * ja * Dc * (float)act);
+    ci += (double)(0.5f * (ja * ja * Dc * (float)act));
   }
   __syncthreads();
   // J^T f per dof, rows in ascending order through the entity row list; one lane per entity reads each of its
@@ -408,14 +415,15 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, float& cost, flo
       }
     }
   }
-  float gi = 0.f;
+  double gi = 0.0;
   for (int d = w.lane; d < w.D; d += kWave) {
     float v = 0.5f * (L.Ma[d] - L.force[d]) * (L.qacc[d] - L.acc_smooth[d]);
-    gi += v;
+    gi += (double)v;
   }
   __syncthreads();
-  gauss = wave_sum(gi);
-  cost = wave_sum(ci) + gauss;
+  const double gd = wave_sum_d(gi);
+  gauss = (float)gd;
+  cost = wave_sum_d(ci) + gd;
   __syncthreads();
 }
 
@@ -438,6 +446,23 @@ __device__ __forceinline__ int wave_col_candidate(const WaveCtx& w, int i, int e
     }
   }
   return -1;
+}
+
+// f(j) for every entry j of column i (entity ei) below the diagonal, in the order of wave_col_candidate: the rest of
+// ei's own dofs, then the dofs of the pattern entities after ei
+template <class F>
+__device__ __forceinline__ void wave_for_col(const WaveLds& L, int i, int ei, F f) {
+  for (int j = i + 1; j < L.ee[ei]; j++) f(j);
+  for (int word = (ei + 1) >> 6; word < 2; word++) {
+    unsigned long long bits = L.pat[2 * ei + word];
+    int lo = ei + 1 - word * 64;
+    if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
+    while (bits) {
+      int ej = word * 64 + __ffsll((long long)bits) - 1;
+      bits &= bits - 1;
+      for (int j = L.es[ej]; j < L.ee[ej]; j++) f(j);
+    }
+  }
 }
 
 // sum_{k < kmax, block(ent(a), ent(k)) and block(ent(b), ent(k)) in pattern} H[a][k] * H[b][k], k ascending
@@ -498,13 +523,11 @@ __device__ __forceinline__ void wave_chol_factor(const WaveCtx& w, const HS& h) 
         float hii = sqrtf(fmaxf(tmp, EPS));
         di[i - s0] = hii;
         float inv = 1.f / hii;
-        for (int t = 0;; t++) {
-          int j = wave_col_candidate(w, i, ei, t);
-          if (j < 0) break;
+        wave_for_col(L, i, ei, [&](int j) {
           float dt = wave_row_dot(w, h, j, i, i);
           float* pj = h.blk(j, L.ent_of_dof[j], ei) + (i - s0);
           *pj = (*pj - dt) * inv;
-        }
+        });
       }
     }
     __syncthreads();
@@ -582,12 +605,39 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
   if (w.lane < nnz) L.vec[e.JI(c, w.lane)] = e.JV(c, w.lane) * sq;
   __syncthreads();
   bool rank_ok = true;
-  int e_lo = ea < eb ? ea : eb;
-  for (int ek = e_lo; ek < w.E; ek++) {
-    for (int k = L.es[ek]; k < L.ee[ek]; k++) {
+  // Entities whose vec entries can be nonzero: the row's two, then every pattern entity below one that had a nonzero
+  // entry. Popping the lowest one each time visits them in ascending order, like a sweep over all entities; the
+  // entities skipped hold exact zeros, which that sweep skips too.
+  unsigned long long act0 = 0ull, act1 = 0ull;
+  if (ea < 64) act0 |= 1ull << ea; else act1 |= 1ull << (ea - 64);
+  if (eb < 64) act0 |= 1ull << eb; else act1 |= 1ull << (eb - 64);
+  for (;;) {
+    int ek;
+    if (act0) { ek = __ffsll((long long)act0) - 1; act0 &= act0 - 1; }
+    else if (act1) { ek = 64 + __ffsll((long long)act1) - 1; act1 &= act1 - 1; }
+    else break;
+    const int k0 = L.es[ek], k1 = L.ee[ek];
+    // this lane's column entry among the pattern entities below ek: the lane-th dof of their concatenation
+    unsigned long long b0 = ek + 1 >= 64 ? 0ull : (L.pat[2 * ek] & (~0ull << (ek + 1)));
+    unsigned long long b1 = L.pat[2 * ek + 1] & (ek + 1 <= 64 ? ~0ull : (ek + 1 >= 128 ? 0ull : (~0ull << (ek + 1 - 64))));
+    int n_below = 0, j_below = -1;
+    for (int word = 0; word < 2; word++) {
+      unsigned long long t = word ? b1 : b0;
+      while (t) {
+        int ej = word * 64 + __ffsll((long long)t) - 1;
+        t &= t - 1;
+        int n = L.ee[ej] - L.es[ej];
+        if (w.lane >= n_below && w.lane < n_below + n) j_below = L.es[ej] + (w.lane - n_below);
+        n_below += n;
+      }
+    }
+    const bool fast = (k1 - k0) + n_below <= kWave;
+    bool any = false;
+    for (int k = k0; k < k1; k++) {
       float vk = L.vec[k];
       if (!(fabsf(vk) > m.eps)) continue;
-      float* dk = h.blk(k, ek, ek) + (k - L.es[ek]);
+      any = true;
+      float* dk = h.blk(k, ek, ek) + (k - k0);
       float Lkk = *dk;
       float tmp = Lkk * Lkk + vk * vk * sgn;
       if (tmp < m.eps) {
@@ -597,15 +647,31 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
       float r = sqrtf(tmp), cc = r / Lkk, cinv = 1.f / cc, sv = vk / Lkk;
       __syncthreads();
       if (w.lane == 0) *dk = r;
-      for (int t = w.lane;; t += kWave) {
-        int i = wave_col_candidate(w, k, ek, t);
-        if (i < 0) break;
-        float* pi = h.blk(i, L.ent_of_dof[i], ek) + (k - L.es[ek]);
-        float hv = (*pi + sv * L.vec[i] * sgn) * cinv;
-        *pi = hv;
-        L.vec[i] = L.vec[i] * cc - sv * hv;
+      if (fast) {
+        const int rest = k1 - k - 1;
+        const int jb = __shfl(j_below, max(w.lane - rest, 0), kWave);
+        const int i = w.lane < rest ? k + 1 + w.lane : (w.lane < rest + n_below ? jb : -1);
+        if (i >= 0) {
+          float* pi = h.blk(i, L.ent_of_dof[i], ek) + (k - k0);
+          float hv = (*pi + sv * L.vec[i] * sgn) * cinv;
+          *pi = hv;
+          L.vec[i] = L.vec[i] * cc - sv * hv;
+        }
+      } else {
+        for (int t = w.lane;; t += kWave) {
+          int i = wave_col_candidate(w, k, ek, t);
+          if (i < 0) break;
+          float* pi = h.blk(i, L.ent_of_dof[i], ek) + (k - k0);
+          float hv = (*pi + sv * L.vec[i] * sgn) * cinv;
+          *pi = hv;
+          L.vec[i] = L.vec[i] * cc - sv * hv;
+        }
       }
       __syncthreads();
+    }
+    if (any) {
+      act0 |= b0;
+      act1 |= b1;
     }
   }
   return rank_ok;
@@ -648,11 +714,8 @@ __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h) {
       int ei = L.rlvl_order[idx];
       for (int i = L.ee[ei] - 1; i >= L.es[ei]; i--) {
         float g = L.Mgrad[i];
-        for (int t = 0;; t++) {
-          int j = wave_col_candidate(w, i, ei, t);
-          if (j < 0) break;
-          g = g - h.blk(j, L.ent_of_dof[j], ei)[i - L.es[ei]] * L.Mgrad[j];
-        }
+        const int r = i - L.es[ei];
+        wave_for_col(L, i, ei, [&](int j) { g = g - h.blk(j, L.ent_of_dof[j], ei)[r] * L.Mgrad[j]; });
         L.Mgrad[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
       }
     }
@@ -674,7 +737,7 @@ struct WLS {
 __device__ inline WLS wave_ls_point(const WaveCtx& w, float qg0, float qg1, float qg2, float alpha, int& ls_it) {
   const Env& e = w.e;
   const gsr_state& s = e.s;
-  float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
   for (int c = w.lane; c < w.n_con; c += kWave) {
     float ja = e.EF(s.efc_Jaref, c), jv = e.EF(s.efc_jv, c);
     float x = ja + alpha * jv;
@@ -691,18 +754,18 @@ __device__ inline WLS wave_ls_point(const WaveCtx& w, float qg0, float qg1, floa
     } else if (w.nef <= c) {
       act = x < 0.f;
     }
-    t0 += q0 * (float)act;
-    t1 += q1 * (float)act;
-    t2 += q2 * (float)act;
+    t0 += (double)(q0 * (float)act);
+    t1 += (double)(q1 * (float)act);
+    t2 += (double)(q2 * (float)act);
   }
-  t0 = qg0 + wave_sum(t0);
-  t1 = qg1 + wave_sum(t1);
-  t2 = qg2 + wave_sum(t2);
+  const float T0 = (float)((double)qg0 + wave_sum_d(t0));
+  const float T1 = (float)((double)qg1 + wave_sum_d(t1));
+  const float T2 = (float)((double)qg2 + wave_sum_d(t2));
   WLS p;
   p.alpha = alpha;
-  p.cost = alpha * alpha * t2 + alpha * t1 + t0;
-  p.d0 = 2.f * alpha * t2 + t1;
-  p.d1 = 2.f * t2;
+  p.cost = alpha * alpha * T2 + alpha * T1 + T0;
+  p.d0 = 2.f * alpha * T2 + T1;
+  p.d1 = 2.f * T2;
   if (p.d1 <= 0.f) p.d1 = e.m.eps;
   ls_it++;
   return p;
@@ -713,9 +776,9 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const WaveLds& L = w.L;
-  float sn = 0.f;
-  for (int d = w.lane; d < w.D; d += kWave) sn += L.search[d] * L.search[d];
-  float snorm = sqrtf(wave_sum(sn));
+  double sn = 0.0;
+  for (int d = w.lane; d < w.D; d += kWave) sn += (double)(L.search[d] * L.search[d]);
+  float snorm = sqrtf((float)wave_sum_d(sn));
   float scale = m.meaninertia * (float)(w.D > 1 ? w.D : 1);
   float gtol = m.tolerance * m.ls_tolerance * snorm * scale;
   if (snorm < m.eps) return 0.f;
@@ -731,13 +794,13 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
     e.QD(c, 1) = Dc * (jv * ja);
     e.QD(c, 2) = Dc * (0.5f * jv * jv);
   }
-  float a1 = 0.f, a2 = 0.f;
+  double a1 = 0.0, a2 = 0.0;
   for (int d = w.lane; d < w.D; d += kWave) {
     float sd = L.search[d];
-    a1 += sd * L.Ma[d] - sd * L.force[d];
-    a2 += 0.5f * sd * L.mv[d];
+    a1 += (double)(sd * L.Ma[d] - sd * L.force[d]);
+    a2 += (double)(0.5f * sd * L.mv[d]);
   }
-  float qg0 = gauss, qg1 = wave_sum(a1), qg2 = wave_sum(a2);
+  float qg0 = gauss, qg1 = (float)wave_sum_d(a1), qg2 = (float)wave_sum_d(a2);
   __syncthreads();
   int ls_it = 0;
   WLS p0 = wave_ls_point(w, qg0, qg1, qg2, 0.f, ls_it);
@@ -794,7 +857,7 @@ constexpr int kMaxRank1PerIter = 8;
 
 // Newton iterations (func_solve / func_solve_body) from the initial factorisation on; returns the iteration count
 template <class HS>
-__device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, float cost, float gauss) {
+__device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, double cost, float gauss) {
   const Env& e = w.e;
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
@@ -819,7 +882,7 @@ __device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, float 
       }
       for (int c = lane; c < n_con; c += kWave) e.EF(s.efc_Jaref, c) = e.EF(s.efc_Jaref, c) + e.EF(s.efc_jv, c) * alpha;
       __syncthreads();
-      float prev_cost = cost;
+      double prev_cost = cost;
       wave_update_constraint(w, cost, gauss);
       // activity changes in row order; rank-1 updates or a rebuild of the factor
       int n_changed = 0;
@@ -849,10 +912,10 @@ __device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, float 
       }
       wave_update_gradient(w, h);
       float tol_scaled = (m.meaninertia * (float)(D > 1 ? D : 1)) * m.tolerance;
-      float improvement = prev_cost - cost;
-      float g2 = 0.f;
-      for (int d = lane; d < D; d += kWave) g2 += L.grad[d] * L.grad[d];
-      float gn = sqrtf(wave_sum(g2));
+      double improvement = prev_cost - cost;
+      double g2 = 0.0;
+      for (int d = lane; d < D; d += kWave) g2 += (double)(L.grad[d] * L.grad[d]);
+      float gn = sqrtf((float)wave_sum_d(g2));
       if (gn < tol_scaled || improvement < tol_scaled) {
         improved = false;
       } else {
@@ -890,7 +953,8 @@ __device__ __forceinline__ void wave_solve(const Env& e, char* lds) {
     e.EF(s.efc_Jaref, c) = ja;
   }
   __syncthreads();
-  float cost, gauss;
+  double cost;
+  float gauss;
   wave_update_constraint(w, cost, gauss);
   int iters = L.hmode[0] ? wave_newton(w, HLds{L}, cost, gauss) : wave_newton(w, HGlobal{e, L.es}, cost, gauss);
   // epilogue (func_update_qacc, func_update_contact_force)

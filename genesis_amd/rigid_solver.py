@@ -82,6 +82,7 @@ def state_sizes(M: RigidModel) -> dict:
         efc_force=R, efc_Jaref=R, efc_jv=R, efc_quad=3 * R, efc_active=R,
         qacc=D, Ma=D, grad=D, Mgrad=D, search=D, mv=D, qfrc_constraint=D, nt_vec=D, nt_H=D * (D + 1) // 2,
         solver_iters=1, qpos_next=Q, vel_next=D, ent_row_ptr=M.n_entities + 1, ent_row_list=2 * R, efc_ent=R,
+        gjk_ws=_lib.load().gsr_gjk_workspace_words(),
     )
     missing = set(_lib.STATE_POINTERS) - set(sz)
     assert not missing, f"state arrays without a size rule: {missing}"

@@ -98,19 +98,93 @@ def pile_entities(n_boxes: int, rng: np.random.Generator, rho=1000.0, friction=1
     return ents
 
 
+def _quat_to_mat(q):
+    w, x, y, z = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def boxes_overlap(c1, R1, h1, c2, R2, h2):
+    """Separating-axis test for oriented boxes (broadcast over leading dims): centres c (..., 3), rotations R
+    (..., 3, 3) whose columns are the box axes, half extents h (..., 3). True where no axis among the 6 face normals
+    and the 9 edge cross products separates them."""
+    A = np.swapaxes(R1, -1, -2)  # (..., 3 axes, 3)
+    B = np.swapaxes(R2, -1, -2)
+    A, B = np.broadcast_arrays(A, B)
+    cr = np.cross(A[..., :, None, :], B[..., None, :, :]).reshape(*A.shape[:-2], 9, 3)
+    n = np.linalg.norm(cr, axis=-1, keepdims=True)
+    cr = np.where(n > 1e-9, cr / np.maximum(n, 1e-30), 0.0)  # parallel edges: a zero axis never separates
+    L = np.concatenate([A, B, cr], -2)  # (..., 15, 3)
+    T = (c2 - c1)[..., None, :]
+    r1 = (np.abs(L @ np.swapaxes(A, -1, -2)) * h1[..., None, :]).sum(-1)
+    r2 = (np.abs(L @ np.swapaxes(B, -1, -2)) * h2[..., None, :]).sum(-1)
+    sep = np.abs((L * T).sum(-1)) > r1 + r2
+    return ~sep.any(-1)
+
+
+PILE_BLOCK = 256  # envs per independently seeded block of initial poses
+
+
+def pile_poses(n_envs, sizes, seed, lo=(-0.5, -0.5, 0.1), hi=(0.5, 0.5, 1.1), gap=0.005):
+    """Seeded random poses of boxes `sizes` (n_boxes, 3) inside the box [lo, hi] (1 m^3 by default) for n_envs envs,
+    drawn by rejection: uniform position and uniform orientation per box, kept when the box (grown by `gap`) overlaps
+    none of the boxes placed before it and stays above the plane z = 0. Env i depends only on (seed, i): poses come
+    in blocks of PILE_BLOCK envs, block k drawn from default_rng((seed, k)). Returns pos (n, b, 3), quat (n, b, 4)."""
+    sizes = np.asarray(sizes, np.float64)
+    nb = len(sizes)
+    h = sizes / 2
+    rad = np.linalg.norm(h, axis=-1)
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    n_blocks = -(-n_envs // PILE_BLOCK)
+    N = n_blocks * PILE_BLOCK
+    rngs = [np.random.default_rng((seed, k)) for k in range(n_blocks)]
+    pos = np.zeros((N, nb, 3))
+    quat = np.zeros((N, nb, 4))
+    R = np.zeros((N, nb, 3, 3))
+    for i in range(nb):
+        todo = np.arange(N)
+        for _ in range(10000):
+            # each block draws its own candidates, in env order, from its own generator
+            blk = todo // PILE_BLOCK
+            cnt = np.bincount(blk, minlength=n_blocks)
+            p = np.concatenate([rngs[k].uniform(lo, hi, (cnt[k], 3)) for k in range(n_blocks) if cnt[k]])
+            q = np.concatenate([_uniform_quats(rngs[k], cnt[k]) for k in range(n_blocks) if cnt[k]])
+            Rm = _quat_to_mat(q)
+            ok = p[:, 2] - (np.abs(Rm[:, 2, :]) * h[i]).sum(-1) > gap
+            if i > 0:
+                # separating-axis test only where the bounding spheres meet
+                d = np.linalg.norm(pos[todo, :i] - p[:, None], axis=-1)
+                ii, jj = np.nonzero(d < rad[i] + rad[:i] + 2 * gap)
+                if ii.size:
+                    ov = boxes_overlap(p[ii], Rm[ii], h[i] + gap, pos[todo[ii], jj], R[todo[ii], jj], h[jj] + gap)
+                    bad = np.zeros(todo.size, bool)
+                    bad[ii[ov]] = True
+                    ok &= ~bad
+            acc = todo[ok]
+            pos[acc, i], quat[acc, i], R[acc, i] = p[ok], q[ok], Rm[ok]
+            todo = todo[~ok]
+            if todo.size == 0:
+                break
+        if todo.size:
+            raise RuntimeError(f"no room for box {i} in {todo.size} envs")
+    return pos[:n_envs], quat[:n_envs]
+
+
 def pile_workload(n_boxes=64, n_envs=4096, n_steps=500, seed=1, max_collision_pairs=1000) -> Workload:
-    """Box sizes are shared by every env of one scene type; poses differ per env (seeded)."""
-    rng = np.random.default_rng(seed)
+    """Box sizes are shared by every env of one scene type; poses differ per env (seeded, non-overlapping, in a
+    1 m^3 column above the plane)."""
     opts = CompileOptions(dt=0.01, max_collision_pairs=max_collision_pairs)
-    ents = pile_entities(n_boxes, rng)
+    ents = pile_entities(n_boxes, np.random.default_rng(seed))
+    sizes = np.array([e.links[0].geoms[0].data[:3] for e in ents[1:]], np.float64)
+    cache = {}
 
     def init_qpos(n, n_qs):
-        r = np.random.default_rng(seed + 1)
-        q = np.zeros((n, n_qs))
-        pos = np.stack([r.uniform(-0.5, 0.5, (n, n_boxes)), r.uniform(-0.5, 0.5, (n, n_boxes)),
-                        r.uniform(0.1, 1.1, (n, n_boxes))], -1)
-        quat = _uniform_quats(r, n * n_boxes).reshape(n, n_boxes, 4)
-        q[:, :] = np.concatenate([pos, quat], -1).reshape(n, n_boxes * 7)
+        if cache.get("n", 0) < n:
+            cache["pos"], cache["quat"] = pile_poses(n, sizes, seed + 1)
+            cache["n"] = n
+        q = np.concatenate([cache["pos"][:n], cache["quat"][:n]], -1).reshape(n, n_boxes * 7)
+        assert q.shape[1] == n_qs
         return q
 
     return Workload("pile", ents, opts, n_envs, n_steps, init_qpos=init_qpos)

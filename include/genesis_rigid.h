@@ -41,7 +41,7 @@ extern "C" {
 #define GSR_ERR_CONTACT_OVERFLOW 2u
 #define GSR_ERR_NAN_QACC 4u
 #define GSR_ERR_NAN_STATE 8u
-#define GSR_ERR_DEEP_PENETRATION 0x100u /* MPR result kept where the reference would fall back to GJK/EPA */
+#define GSR_ERR_DEEP_PENETRATION 0x100u /* informational: the GJK/EPA fallback ran for a deep penetration */
 
 /* reset flags for gsr_reset_envs */
 #define GSR_RESET_COLLIDER 1u  /* contact-normal cache + first_time (sort warm start) */
@@ -118,6 +118,7 @@ typedef struct gsr_state {
   int32_t *solver_iters;
   int32_t *ent_row_ptr, *ent_row_list; /* per-entity ordered constraint-row lists (wavefront solver) */
   int32_t *efc_ent;                    /* entity pair of each row (wavefront solver) */
+  float *gjk_ws;                       /* GJK/EPA polytope workspace, gsr_gjk_workspace_words() per env, env-major */
   /* integration scratch */
   float *qpos_next, *vel_next;
 } gsr_state;
@@ -128,6 +129,8 @@ int gsr_model_size(void);
 int gsr_state_size(void);
 int gsr_group_size(void);
 int gsr_batch_info_size(void);
+/* 32-bit words of GJK/EPA workspace one env needs (size of gsr_state.gjk_ws per env) */
+int gsr_gjk_workspace_words(void);
 
 int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream);
 int gsr_forward_kinematics(const gsr_model* model, const gsr_state* state, const int32_t* envs_idx, int32_t n_idx,

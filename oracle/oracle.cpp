@@ -932,6 +932,530 @@ void rotate_frame(Env<T>& e, int g, V3<T> cpos, Q4<T> qrot) {
 }
 
 // func_convex_convex_contact (collider_decomp.py:2328-2725) -- MPR path; the GJK/EPA fallback is not restated
+// ------------------------------------------------------------------------------------------------------------
+// GJK + EPA, the "safe" (non-MuJoCo-compatible) path taken when MPR reports a deep penetration
+// (gjk_decomp.py: func_gjk_contact :165-417, func_safe_gjk :3656-3856, func_safe_gjk_support :4167-4280,
+//  func_search_valid_simplex_vertex :3982-4076, func_safe_epa :4338-4540, func_safe_epa_witness :4543-4600,
+//  func_safe_epa_init :4604-4653, func_safe_attach_face_to_polytope :4657-4739, func_plane_normal :4742-4780,
+//  func_epa_horizon :1317-1383, func_triangle_affine_coords :2037-2094, func_project_origin_to_plane :3414-3464)
+// ------------------------------------------------------------------------------------------------------------
+constexpr int kGjkIters = 50, kEpaIters = 50, kPolyFaces = 6 * kEpaIters, kPolyVerts = 5 + kEpaIters;
+constexpr int kHorizon = 6 + kEpaIters, kHStack = 3 * kPolyFaces;
+
+template <class T>
+struct GjkConst {  // gjk_decomp.py:87-113
+  T FLOAT_MIN = T(1e-15), FLOAT_MAX = T(1e15), FLOAT_MIN_SQ = T(1e-30), FLOAT_MAX_SQ = T(1e30);
+  T tolerance = T(1e-6), degen_sq = T(1e-10), max_reproj = T(1e-5);
+};
+
+template <class T>
+struct GVert {
+  V3<T> o1, o2, mk;
+  int id1 = -1, id2 = -1;
+};
+
+template <class T>
+struct GFace {
+  int v[3] = {0, 0, 0}, adj[3] = {0, 0, 0};
+  V3<T> n;
+  T dist2 = 0;
+  int map_idx = -1;
+};
+
+template <class T>
+struct GjkWork {
+  GVert<T> s[4];
+  int ns = 0, last_searched = 0;
+  GVert<T> pv[kPolyVerts];
+  int npv = 0;
+  GFace<T> f[kPolyFaces];
+  int nf = 0;
+  int fmap[kPolyFaces];
+  int nfmap = 0;
+  int hf[kHorizon], he[kHorizon];
+  int nh = 0;
+  int sf[kHStack], se[kHStack];
+  V3<T> w1, w2;
+  int n_witness = 0;
+  T distance = 0;
+};
+
+inline bool gjk_discrete(int t) { return t == GEOM_BOX || t == GEOM_MESH; }
+
+// support_driver (gjk_decomp.py:3605-3652) with the box vertex id of support_field_decomp.py:289-311
+template <class T>
+void gjk_support_one(const Model<T>& m, const Env<T>& e, V3<T> d, int g, V3<T>& v, int& vid) {
+  int t = m.geom_type[g];
+  vid = -1;
+  if (t == GEOM_BOX) {
+    V3<T> db = inv_transform_by_quat(d, e.g_quat[g]);
+    const T* data = &m.geom_data[7 * g];
+    V3<T> vl((db.x < 0 ? T(-1) : T(1)) * data[0] * T(0.5), (db.y < 0 ? T(-1) : T(1)) * data[1] * T(0.5),
+             (db.z < 0 ? T(-1) : T(1)) * data[2] * T(0.5));
+    vid = (vl.x > 0 ? 1 : 0) + (vl.y > 0 ? 2 : 0) + (vl.z > 0 ? 4 : 0) + m.geom_vert_start[g];
+    v = transform_by_trans_quat(vl, e.g_pos[g], e.g_quat[g]);
+  } else {
+    v = support_driver(m, e, d, g);
+  }
+}
+
+template <class T>
+GVert<T> gjk_support(const Model<T>& m, const Env<T>& e, int ga, int gb, V3<T> d) {  // func_support
+  GVert<T> r;
+  gjk_support_one(m, e, d, ga, r.o1, r.id1);
+  gjk_support_one(m, e, -d, gb, r.o2, r.id2);
+  r.mk = r.o1 - r.o2;
+  return r;
+}
+
+template <class T>
+int gjk_count_supports(const Model<T>& m, const Env<T>& e, V3<T> d, int g) {  // count_support_driver
+  if (m.geom_type[g] != GEOM_BOX) return 1;
+  V3<T> db = inv_transform_by_quat(d, e.g_quat[g]);
+  int z = (db.x == 0) + (db.y == 0) + (db.z == 0);
+  return 1 << z;
+}
+
+template <class T>
+bool gjk_duplicate(const GjkWork<T>& w, int id1, int id2) {
+  for (int i = 0; i < w.ns; i++) {
+    if (id1 == -1 || w.s[i].id1 != id1) continue;
+    if (id2 == -1 || w.s[i].id2 != id2) continue;
+    return true;
+  }
+  return false;
+}
+
+template <class T>
+bool gjk_degenerate(const GjkWork<T>& w, const GjkConst<T>& c, V3<T> mk) {
+  for (int i = 0; i < w.ns; i++) {
+    V3<T> d = w.s[i].mk - mk;
+    if (dot(d, d) < c.degen_sq) return true;
+  }
+  if (w.ns == 2) {  // func_is_colinear
+    V3<T> e1 = w.s[1].mk - w.s[0].mk, e2 = mk - w.s[0].mk;
+    V3<T> n = cross(e1, e2);
+    return dot(n, n) < c.degen_sq * dot(e1, e1) * dot(e2, e2);
+  }
+  if (w.ns == 3) {  // func_is_coplanar
+    V3<T> e1 = normalized(w.s[1].mk - w.s[0].mk), e2 = normalized(w.s[2].mk - w.s[0].mk);
+    V3<T> n = cross(e1, e2);
+    V3<T> diff = mk - w.s[0].mk;
+    T nd = dot(n, diff);
+    return nd * nd < c.degen_sq * dot(n, n) * dot(diff, diff);
+  }
+  return false;
+}
+
+template <class T>
+bool gjk_valid(const GjkWork<T>& w, const GjkConst<T>& c, const GVert<T>& v) {
+  return !gjk_duplicate(w, v.id1, v.id2) && !gjk_degenerate(w, c, v.mk);
+}
+
+template <class T>
+GVert<T> gjk_safe_support(const Model<T>& m, const Env<T>& e, const GjkWork<T>& w, const GjkConst<T>& c, int ga,
+                          int gb, V3<T> dir) {
+  const T EPS = m.EPS;
+  GVert<T> r;
+  for (int i = 0; i < 9; i++) {
+    V3<T> nd = dir;
+    if (i > 0) {
+      int j = i - 1;
+      // the reference scales by 1 - 2 * (j & bit) with the bit's value, not a 0/1 flag
+      nd.x += -T(1 - 2 * (j & 1)) * EPS;
+      nd.y += -T(1 - 2 * (j & 2)) * EPS;
+      nd.z += -T(1 - 2 * (j & 4)) * EPS;
+    }
+    nd = nd * (T(2) - dot(nd, dir));
+    int num = gjk_count_supports(m, e, nd, ga) * gjk_count_supports(m, e, -nd, gb);
+    if (i > 0 && num > 1) continue;
+    r = gjk_support(m, e, ga, gb, nd);
+    if (i == 0) {
+      if (num > 1) continue;
+      break;
+    }
+    if (i == 8) break;
+    if (gjk_valid(w, c, r)) break;
+  }
+  return r;
+}
+
+template <class T>
+V3<T> gjk_discrete_vertex(const Model<T>& m, const Env<T>& e, int g, int iv) {
+  V3<T> vl;
+  if (m.geom_type[g] == GEOM_BOX) {
+    const T* data = &m.geom_data[7 * g];
+    vl = V3<T>(((iv & 1) ? T(1) : T(-1)) * data[0] * T(0.5), ((iv & 2) ? T(1) : T(-1)) * data[1] * T(0.5),
+               ((iv & 4) ? T(1) : T(-1)) * data[2] * T(0.5));
+  } else {
+    vl = m.verts[m.geom_vert_start[g] + iv];
+  }
+  return transform_by_trans_quat(vl, e.g_pos[g], e.g_quat[g]);
+}
+
+template <class T>
+bool gjk_search_valid(const Model<T>& m, const Env<T>& e, GjkWork<T>& w, const GjkConst<T>& c, int ga, int gb,
+                      GVert<T>& out) {
+  if (gjk_discrete(m.geom_type[ga]) && gjk_discrete(m.geom_type[gb])) {
+    int n0 = m.geom_vert_end[ga] - m.geom_vert_start[ga], n1 = m.geom_vert_end[gb] - m.geom_vert_start[gb];
+    int cases = n0 * n1;
+    for (int k = 0; k < cases; k++) {
+      int mm = (k + w.last_searched) % cases;
+      int i = mm / n1, j = mm % n1;
+      out.id1 = m.geom_vert_start[ga] + i;
+      out.id2 = m.geom_vert_start[gb] + j;
+      out.o1 = gjk_discrete_vertex(m, e, ga, i);
+      out.o2 = gjk_discrete_vertex(m, e, gb, j);
+      out.mk = out.o1 - out.o2;
+      if (gjk_valid(w, c, out)) {
+        w.last_searched = (mm + 1) % cases;
+        return true;
+      }
+    }
+    return false;
+  }
+  if (w.ns == 3) {
+    V3<T> dir = normalized(cross(w.s[2].mk - w.s[0].mk, w.s[1].mk - w.s[0].mk));
+    for (int i = 0; i < 2; i++) {
+      out = gjk_safe_support(m, e, w, c, ga, gb, i == 0 ? dir : -dir);
+      if (gjk_valid(w, c, out)) return true;
+    }
+  }
+  return false;
+}
+
+// returns 1 when the origin is inside the simplex (intersection), 0 otherwise
+template <class T>
+int gjk_safe(const Model<T>& m, const Env<T>& e, GjkWork<T>& w, const GjkConst<T>& c, int ga, int gb) {
+  bool init_ok = true;
+  w.ns = 0;
+  for (int i = 0; i < 4; i++) {
+    V3<T> dir;
+    T val = T(1 - 2 * (i % 2));
+    if (2 - i / 2 == 2) dir.z = val;
+    else dir.y = val;
+    GVert<T> v = gjk_safe_support(m, e, w, c, ga, gb, dir);
+    if (!gjk_valid(w, c, v)) {
+      if (!gjk_search_valid(m, e, w, c, ga, gb, v)) {
+        init_ok = false;
+        break;
+      }
+    }
+    w.s[i] = v;
+    w.ns++;
+  }
+  int flag = 0;
+  if (init_ok) {
+    for (int it = 0; it < kGjkIters; it++) {
+      V3<T> nrm[4];
+      T sd[4];
+      for (int j = 0; j < 4; j++) {
+        int s0 = 2, s1 = 1, s2 = 3, ap = 0;
+        if (j == 1) { s0 = 0; s1 = 2; s2 = 3; ap = 1; }
+        else if (j == 2) { s0 = 1; s1 = 0; s2 = 3; ap = 2; }
+        else if (j == 3) { s0 = 0; s1 = 1; s2 = 2; ap = 3; }
+        V3<T> v1 = w.s[s0].mk, v2 = w.s[s1].mk, v3 = w.s[s2].mk, va = w.s[ap].mk;
+        V3<T> n = normalized(cross(v3 - v1, v2 - v1));
+        if (dot(n, va - v1) > 0) n = -n;
+        nrm[j] = n;
+        sd[j] = dot(n, v1);
+      }
+      int mi = 0;
+      for (int j = 1; j < 4; j++)
+        if (sd[j] < sd[mi]) mi = j;
+      if (sd[mi] >= 0) { flag = 1; break; }
+      w.ns = 3;
+      if (mi != 3) w.s[mi] = w.s[3];
+      GVert<T> v = gjk_safe_support(m, e, w, c, ga, gb, nrm[mi]);
+      if (gjk_duplicate(w, v.id1, v.id2)) break;
+      if (gjk_degenerate(w, c, v.mk)) break;
+      if (dot(v.mk, nrm[mi]) < 0) break;
+      w.s[3] = v;
+      w.ns = 4;
+    }
+  }
+  w.distance = flag ? T(0) : c.FLOAT_MAX;
+  return flag;
+}
+
+template <class T>
+bool plane_normal(const GjkConst<T>& c, V3<T> v1, V3<T> v2, V3<T> v3, V3<T>& normal) {
+  V3<T> d21 = v2 - v1, d31 = v3 - v1, d32 = v3 - v2;
+  for (int i = 0; i < 3; i++) {
+    V3<T> n = i == 0 ? cross(d32, d21) : (i == 1 ? cross(d21, d31) : cross(d31, d32));
+    T nn = norm(n);
+    if (nn == 0) return false;
+    if (nn > c.FLOAT_MIN) {
+      normal = normalized(n);
+      return true;
+    }
+  }
+  return false;
+}
+
+template <class T>
+bool epa_attach(GjkWork<T>& w, const GjkConst<T>& c, int v1, int v2, int v3, int a1, int a2, int a3) {
+  int n = w.nf;
+  GFace<T>& F = w.f[n];
+  F.v[0] = v1; F.v[1] = v2; F.v[2] = v3;
+  F.adj[0] = a1; F.adj[1] = a2; F.adj[2] = a3;
+  w.nf++;
+  V3<T> normal;
+  bool ok = plane_normal(c, w.pv[v3].mk, w.pv[v2].mk, w.pv[v1].mk, normal);
+  if (ok) {
+    V3<T> center = (w.pv[v1].mk + w.pv[v2].mk + w.pv[v3].mk) / T(3);
+    T max_orient = -dot(normal, center), max_abs = std::fabs(max_orient);
+    for (int iv = 0; iv < w.npv; iv++) {
+      if (iv == v1 || iv == v2 || iv == v3) continue;
+      T orient = dot(normal, w.pv[iv].mk - center);
+      if (std::fabs(orient) > max_abs) {
+        max_abs = std::fabs(orient);
+        max_orient = orient;
+      }
+    }
+    if (max_orient > 0) normal = -normal;
+    F.n = normal;
+    T md = c.FLOAT_MAX;
+    for (int i = 0; i < 3; i++) {
+      T d = dot(normal, w.pv[F.v[i]].mk);
+      T d2 = d * d;
+      if (d2 < md) md = d2;
+    }
+    F.dist2 = md;
+    F.map_idx = -1;
+  } else {
+    // degenerate face: never the nearest (the reference leaves the slot's previous contents)
+    F.n = V3<T>();
+    F.dist2 = c.FLOAT_MAX_SQ;
+  }
+  return ok;
+}
+
+template <class T>
+void epa_delete_face(GjkWork<T>& w, int i_f) {
+  int idx = w.f[i_f].map_idx;
+  if (idx >= 0) {
+    int last = w.fmap[w.nfmap - 1];
+    w.fmap[idx] = last;
+    w.f[last].map_idx = idx;
+    w.nfmap--;
+  }
+  w.f[i_f].map_idx = -2;
+}
+
+// returns true on error (horizon not a closed loop, or a buffer full)
+template <class T>
+bool epa_horizon(GjkWork<T>& w, const GjkConst<T>& c, int nearest, V3<T> wv) {
+  w.sf[0] = nearest;
+  w.se[0] = 0;
+  int top = 1;
+  bool first = true;
+  while (top > 0) {
+    int i_f = w.sf[top - 1], i_e = w.se[top - 1];
+    V3<T> v = w.pv[w.f[i_f].v[0]].mk;
+    top--;
+    bool deleted = w.f[i_f].map_idx == -2;
+    if (!first && deleted) continue;
+    bool visible = dot(w.f[i_f].n, wv - v) > c.FLOAT_MIN;
+    if (visible || first) {
+      epa_delete_face(w, i_f);
+      for (int k = first ? 0 : 1; k < 3; k++) {
+        int e2 = (i_e + k) % 3;
+        int adj = w.f[i_f].adj[e2];
+        if (w.f[adj].map_idx != -2) {
+          int sv = w.f[i_f].v[(e2 + 1) % 3];
+          const int* vv = w.f[adj].v;
+          int ae = vv[0] == sv ? 0 : (vv[1] == sv ? 1 : 2);
+          if (top >= kHStack) return true;
+          w.sf[top] = adj;
+          w.se[top] = ae;
+          top++;
+        }
+      }
+    } else {
+      if (w.nh >= kHorizon) return true;
+      w.hf[w.nh] = i_f;
+      w.he[w.nh] = i_e;
+      w.nh++;
+    }
+    first = false;
+  }
+  return false;
+}
+
+template <class T>
+void project_origin_to_plane(const GjkConst<T>& c, V3<T> v1, V3<T> v2, V3<T> v3, V3<T>& point) {
+  V3<T> d21 = v2 - v1, d31 = v3 - v1, d32 = v3 - v2;
+  for (int i = 0; i < 3; i++) {
+    V3<T> n = i == 0 ? cross(d32, d21) : (i == 1 ? cross(d21, d31) : cross(d31, d32));
+    V3<T> v = i == 0 ? v2 : (i == 1 ? v1 : v3);
+    T nv = dot(n, v), nn = dot(n, n);
+    if (nn == 0) return;
+    if (nn > c.FLOAT_MIN) {
+      point = n * (nv / nn);
+      return;
+    }
+    if (i == 2) {
+      if (!(nn < c.FLOAT_MIN)) point = n * (nv / nn);
+    }
+  }
+}
+
+template <class T>
+V3<T> triangle_affine_coords(V3<T> p, V3<T> t1, V3<T> t2, V3<T> t3) {
+  T ms[3];
+  for (int i = 0; i < 3; i++) {
+    int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+    if (i == 1) std::swap(i1, i2);
+    ms[i] = t2[i1] * t3[i2] - t2[i2] * t3[i1] - t1[i1] * t3[i2] + t1[i2] * t3[i1] + t1[i1] * t2[i2] - t1[i2] * t2[i1];
+  }
+  T m_max = 0;
+  int ix = 0, iy = 0;
+  for (int i = 0; i < 3; i++) {
+    T a = std::fabs(ms[i]);
+    if (a >= std::fabs(ms[(i + 1) % 3]) && a >= std::fabs(ms[(i + 2) % 3])) {
+      m_max = ms[i];
+      ix = (i + 1) % 3;
+      iy = (i + 2) % 3;
+      if (i == 1) std::swap(ix, iy);
+      break;
+    }
+  }
+  T cs[3];
+  for (int i = 0; i < 3; i++) {
+    V3<T> tv1 = i == 0 ? t2 : (i == 1 ? t3 : t1), tv2 = i == 0 ? t3 : (i == 1 ? t1 : t2);
+    cs[i] = p[ix] * tv1[iy] + p[iy] * tv2[ix] + tv1[ix] * tv2[iy] - p[ix] * tv2[iy] - p[iy] * tv1[ix] - tv2[ix] * tv1[iy];
+  }
+  return V3<T>(cs[0] / m_max, cs[1] / m_max, cs[2] / m_max);
+}
+
+template <class T>
+bool epa_witness(GjkWork<T>& w, const GjkConst<T>& c, int i_f) {
+  const int* iv = w.f[i_f].v;
+  V3<T> v1 = w.pv[iv[0]].mk, v2 = w.pv[iv[1]].mk, v3 = w.pv[iv[2]].mk;
+  V3<T> proj;
+  project_origin_to_plane(c, v1, v2, v3, proj);
+  V3<T> l = triangle_affine_coords(proj, v1, v2, v3);
+  V3<T> pl = v1 * l.x + v2 * l.y + v3 * l.z;
+  T err = norm(proj - pl);
+  T me = std::max(std::max(dot(v1 - v2, v1 - v2), dot(v2 - v3, v2 - v3)), std::max(dot(v3 - v1, v3 - v1), c.FLOAT_MIN_SQ));
+  T inv = T(1) / std::sqrt(me);
+  if (err * inv > c.max_reproj) return false;
+  w.w1 = w.pv[iv[0]].o1 * l.x + w.pv[iv[1]].o1 * l.y + w.pv[iv[2]].o1 * l.z;
+  w.w2 = w.pv[iv[0]].o2 * l.x + w.pv[iv[1]].o2 * l.y + w.pv[iv[2]].o2 * l.z;
+  return true;
+}
+
+template <class T>
+void epa_safe(const Model<T>& m, const Env<T>& e, GjkWork<T>& w, const GjkConst<T>& c, int ga, int gb) {
+  const T EPS = m.EPS;
+  // func_safe_epa_init: the tetrahedron's 4 faces
+  w.npv = 0; w.nf = 0; w.nfmap = 0; w.nh = 0;
+  int vi[4];
+  for (int i = 0; i < 4; i++) { w.pv[w.npv] = w.s[i]; vi[i] = w.npv++; }
+  const int fv[4][3] = {{vi[0], vi[1], vi[2]}, {vi[0], vi[3], vi[1]}, {vi[0], vi[2], vi[3]}, {vi[3], vi[2], vi[1]}};
+  const int fa[4][3] = {{1, 3, 2}, {2, 3, 0}, {0, 3, 1}, {2, 0, 1}};
+  for (int i = 0; i < 4; i++) epa_attach(w, c, fv[i][0], fv[i][1], fv[i][2], fa[i][0], fa[i][1], fa[i][2]);
+  for (int i = 0; i < 4; i++) { w.fmap[i] = i; w.f[i].map_idx = i; }
+  w.nfmap = 4;
+  // func_safe_epa
+  T upper = c.FLOAT_MAX, upper2 = c.FLOAT_MAX_SQ, lower = 0, tol = c.tolerance;
+  int nearest = -1, prev = -1;
+  bool discrete = gjk_discrete(m.geom_type[ga]) && gjk_discrete(m.geom_type[gb]);
+  if (discrete) tol = EPS;
+  for (int k = 0; k < kEpaIters; k++) {
+    prev = nearest;
+    T lower2 = c.FLOAT_MAX_SQ;
+    for (int i = 0; i < w.nfmap; i++) {
+      int i_f = w.fmap[i];
+      if (w.f[i_f].dist2 < lower2) { lower2 = w.f[i_f].dist2; nearest = i_f; }
+    }
+    if (lower2 > upper2 || nearest == -1) { nearest = prev; break; }
+    lower = std::sqrt(lower2);
+    V3<T> dir = w.f[nearest].n;
+    // func_epa_support with dir_norm = 1 (dir / 1)
+    int wi = w.npv;
+    w.pv[w.npv++] = gjk_support(m, e, ga, gb, dir / T(1));
+    V3<T> wv = w.pv[wi].mk;
+    T upper_k = dot(wv, dir);
+    if (upper_k < upper) { upper = upper_k; upper2 = upper * upper; }
+    if (upper - lower < tol) break;
+    if (discrete) {
+      bool repeated = false;
+      for (int i = 0; i < w.npv; i++) {
+        if (i == wi) continue;
+        if (w.pv[i].id1 == w.pv[wi].id1 && w.pv[i].id2 == w.pv[wi].id2) { repeated = true; break; }
+      }
+      if (repeated) break;
+    }
+    if (epa_horizon(w, c, nearest, wv)) { nearest = -1; break; }
+    if (w.nh < 3) { nearest = -1; break; }
+    int nfaces = w.nf, nedges = w.nh;
+    if (nfaces + nedges >= kPolyFaces) break;
+    bool ok = true;
+    for (int i = 0; i < nedges; i++) {
+      int f0 = nfaces + i, f1 = nfaces + (i + 1) % nedges;
+      int hf = w.hf[i], he = w.he[i];
+      int hv1 = w.f[hf].v[he], hv2 = w.f[hf].v[(he + 1) % 3];
+      w.f[hf].adj[he] = f0;
+      int adj0 = i > 0 ? f0 - 1 : nfaces + nedges - 1;
+      ok = epa_attach(w, c, wi, hv2, hv1, f1, hf, adj0);
+      if (!ok) break;
+      T d2 = w.f[w.nf - 1].dist2;
+      if (d2 >= lower2 - EPS && d2 <= upper2 + EPS) {
+        w.fmap[w.nfmap] = f0;
+        w.f[f0].map_idx = w.nfmap;
+        w.nfmap++;
+      }
+    }
+    if (!ok) break;
+    w.nh = 0;
+    if (w.nfmap == 0 || nearest == -1) break;
+  }
+  if (nearest != -1) {
+    T d2 = w.f[nearest].dist2;
+    if (epa_witness(w, c, nearest)) {
+      w.n_witness = 1;
+      w.distance = -std::sqrt(d2);
+    } else {
+      w.n_witness = 0;
+      w.distance = 0;
+    }
+  } else {
+    w.n_witness = 0;
+    w.distance = 0;
+  }
+}
+
+// func_gjk_contact, safe path: returns is_col with penetration, normal and contact position
+template <class T>
+bool gjk_contact(const Model<T>& m, const Env<T>& e, int ga, int gb, T& pen, V3<T>& normal, V3<T>& pos) {
+  static thread_local GjkWork<T> w;
+  GjkConst<T> c;
+  w.last_searched = 0;
+  w.n_witness = 0;
+  if (gjk_safe(m, e, w, c, ga, gb)) epa_safe(m, e, w, c, ga, gb);
+  bool is_col = w.distance < 0;
+  pen = is_col ? -w.distance : T(0);
+  int n = 0;
+  if (is_col) {
+    for (int i = 0; i < w.n_witness; i++) {
+      V3<T> p = T(0.5) * (w.w1 + w.w2);
+      V3<T> nn = w.w2 - w.w1;
+      T len = norm(nn);
+      if (len < c.FLOAT_MIN) continue;
+      normal = nn / len;
+      pos = p;
+      n++;
+    }
+  }
+  if (n == 0) {
+    is_col = false;
+    pen = 0;
+  }
+  return is_col;
+}
+
 template <class T>
 void convex_convex_contact(const Model<T>& m, Env<T>& e, int ga, int gb) {
   const T EPS = m.EPS;
@@ -972,11 +1496,18 @@ void convex_convex_contact(const Model<T>& m, Env<T>& e, int ga, int gb) {
           }
           if (!updated) { mpr_contact(m, e, ga, gb, nws, is_col, normal, pen, cpos); updated = true; }
         }
-        // GJK/EPA fallback (collider_decomp.py:2498-2503) would be taken when pen > tol and
-        // (no cached normal or mc_tolerance*pen >= mpr_to_gjk*tol); flagged, MPR result kept.
+        // GJK/EPA fallback (collider_decomp.py:2498-2503, 2513-2605) when MPR reports a deep penetration and
+        // either no cached normal was available or the overlap is large
         if (pen > tol) {
           bool prefer_gjk = !guess || (m.mc_tolerance * pen >= m.mpr_to_gjk * tol);
-          if (prefer_gjk) e.errno_ |= 0x100u;
+          if (prefer_gjk) {
+            e.errno_ |= 0x100u;  // informational: the GJK/EPA fallback ran
+            T gpen;
+            V3<T> gn, gp;
+            is_col = gjk_contact(m, e, ga, gb, gpen, gn, gp);
+            pen = gpen;
+            if (is_col) { cpos = gp; normal = gn; }
+          }
         }
       }
     }

@@ -127,6 +127,61 @@ def test_broadphase_bit_exact_and_contacts(case):
         assert total > 0
 
 
+def _overlapping_pile_qpos(M, B, rng):
+    """Random box poses in a 0.6 x 0.6 x 0.7 m column, overlaps allowed (deep initial penetrations)."""
+    q = np.tile(M.qpos0.astype(np.float64), (B, 1))
+    n = M.n_qs // 7
+    q = q.reshape(B, n, 7)
+    q[:, :, 0:2] = rng.uniform(-0.3, 0.3, (B, n, 2))
+    q[:, :, 2] = rng.uniform(0.1, 0.8, (B, n))
+    qq = rng.normal(size=(B, n, 4))
+    q[:, :, 3:7] = qq / np.linalg.norm(qq, axis=-1, keepdims=True)
+    return q.reshape(B, -1)
+
+
+def test_deep_penetration_contacts_match_oracle():
+    """Overlapping boxes: MPR hands deep pairs to GJK/EPA (errno bit 0x100 on both sides); from the same state the
+    HIP collision stage gives the oracle's contacts in the same order, with positions, normals and depths within
+    1e-4, over several resynced steps."""
+    _need_gpu()
+    from tests.sync import gpu_to_oracle as sync
+    ents, opt = CASES["pile64"]()
+    M = compile_model(ents, opt)
+    B = 16
+    S = _solver(M, B, 1)
+    S.set_qpos(torch.as_tensor(_overlapping_pile_qpos(M, B, np.random.default_rng(7)), dtype=torch.float32))
+    O = _oracle(M, B)
+    n_total = 0
+    for step in range(6):
+        torch.cuda.synchronize()
+        keep = {k: v.clone() for k, v in S.st.t.items()}
+        sync(S, O)
+        S.run_stage(3)
+        O.run(stage=3)
+        torch.cuda.synchronize()
+        t = S.st.t
+        ncg = t["n_contacts"][0].cpu().numpy()
+        err = t["errno_env"][0].cpu().numpy()
+        geoms = t["contact_geoms"].T.cpu().numpy().reshape(B, -1, 2)
+        pos = t["contact_pos"].T.cpu().numpy().reshape(B, -1, 3)
+        nrm = t["contact_normal"].T.cpu().numpy().reshape(B, -1, 3)
+        pen = t["contact_pen"].T.cpu().numpy()
+        oc = O.get("contacts")
+        assert ((err & 0x100) != 0).all() and ((O.get("errno") & 0x100) != 0).all()
+        for b in range(B):
+            n = int(ncg[b])
+            assert n == len(oc[b]), f"step {step} env {b}"
+            np.testing.assert_array_equal(geoms[b, :n], oc[b][:, 0:2].astype(np.int32))
+            np.testing.assert_allclose(pos[b, :n], oc[b][:, 4:7], atol=1e-4)
+            np.testing.assert_allclose(nrm[b, :n], oc[b][:, 7:10], atol=1e-4)
+            np.testing.assert_allclose(pen[b, :n], oc[b][:, 10], atol=1e-4)
+            n_total += n
+        for k, v in keep.items():
+            t[k].copy_(v)
+        S.substep()
+    assert n_total > 0
+
+
 def _control(case, M, B, step, rng, S):
     if case == "arm" and step % 100 == 0:
         lo = np.where(np.isfinite(M.dof_limit[:, 0]), M.dof_limit[:, 0], -1)
@@ -137,28 +192,28 @@ def _control(case, M, B, step, rng, S):
         S.control_dofs_position(torch.as_tensor(tgt, dtype=torch.float32), dofs_idx=np.arange(6, 18))
 
 
-@pytest.mark.parametrize("case,B,steps,variant", [
-    ("arm", 64, 150, 0), ("quadruped", 128, 150, 0), ("pile8", 64, 150, 0),
-    ("arm", 64, 150, 1), ("quadruped", 128, 150, 1), ("pile8", 64, 150, 1), ("pile64", 16, 60, 1),
+@pytest.mark.parametrize("case,B,steps,variant,pre", [
+    ("arm", 64, 150, 0, 0), ("quadruped", 128, 150, 0, 0), ("pile8", 64, 150, 0, 0),
+    ("arm", 64, 150, 1, 0), ("quadruped", 128, 150, 1, 0), ("pile8", 64, 150, 1, 0),
+    ("pile64", 16, 40, 1, 110), ("pile64-deep", 16, 30, 0, 0),
 ])
-def test_step_parity_resynced(case, B, steps, variant):
+def test_step_parity_resynced(case, B, steps, variant, pre):
     """Each step starts the oracle from the HIP state (full persistent state incl. caches), then compares.
 
-    variant 0 = one lane per env, 1 = one wavefront per env with entity-block sparse Newton."""
+    variant 0 = one lane per env, 1 = one wavefront per env with entity-block sparse Newton. pile64 starts from the
+    benchmark's non-overlapping poses and is checked after `pre` unchecked steps, in the contact-rich landing phase;
+    pile64-deep starts from random overlapping poses (deep penetrations: the GJK/EPA fallback on most pairs)."""
     _need_gpu()
-    ents, opt = CASES[case]()
+    ents, opt = CASES["pile64" if case == "pile64-deep" else case]()
     M = compile_model(ents, opt)
     rng = np.random.default_rng(7)
     S = _solver(M, B, variant)
-    if case.startswith("pile"):
-        q = np.tile(M.qpos0.astype(np.float64), (B, 1))
-        n = M.n_qs // 7
-        q = q.reshape(B, n, 7)
-        q[:, :, 0:2] = rng.uniform(-0.3, 0.3, (B, n, 2))
-        q[:, :, 2] = rng.uniform(0.1, 0.8, (B, n))
-        qq = rng.normal(size=(B, n, 4))
-        q[:, :, 3:7] = qq / np.linalg.norm(qq, axis=-1, keepdims=True)
-        S.set_qpos(torch.as_tensor(q.reshape(B, -1), dtype=torch.float32))
+    if case == "pile64":
+        S.set_qpos(torch.as_tensor(W.pile_workload(n_boxes=64).init_qpos(B, M.n_qs), dtype=torch.float32))
+        for _ in range(pre):
+            S.substep()
+    elif case.startswith("pile"):
+        S.set_qpos(torch.as_tensor(_overlapping_pile_qpos(M, B, rng), dtype=torch.float32))
     if case == "quadruped":
         S.set_qpos(torch.as_tensor(np.tile(np.concatenate([W.GO2_INIT_BASE, W.GO2_STANDING]), (B, 1)), dtype=torch.float32))
     O = _oracle(M, B)

@@ -235,9 +235,17 @@ def test_mpr_sphere_sphere_and_capsule(Oracle):
     o.run(stage=3)
     c = o.get("contacts")[0]
     assert len(c) == 1
+    # no cached normal and pen > tolerance: the reference hands the pair to GJK/EPA (collider_decomp.py:2498-2503).
+    # EPA stops once upper - lower < 1e-6 with lower = the returned depth <= the true depth <= upper
+    # (gjk_decomp.py func_safe_epa); on the Minkowski sum, a sphere of radius R = 0.25, a face within 1e-6 of the
+    # surface tilts its normal by at most sqrt(2e-6 / R).
+    assert o.get("errno")[0] & 0x100
     d = np.linalg.norm([0.2, 0.05, 0])
-    np.testing.assert_allclose(c[0, 10], 0.25 - d, rtol=1e-6)
-    np.testing.assert_allclose(np.abs(c[0, 7:10]), np.abs(np.array([0.2, 0.05, 0]) / d), atol=1e-6)
+    gap = (0.25 - d) - c[0, 10]
+    assert -1e-12 <= gap < 1e-6
+    n_true = np.array([0.2, 0.05, 0]) / d
+    ang = np.arccos(min(1.0, abs(float(np.dot(c[0, 7:10], n_true)))))
+    assert ang <= np.sqrt(2e-6 / 0.25)
     M = compile_model([body(GEOM_CAPSULE, (0.05, 0.4), (0, 0, 1)), body(GEOM_SPHERE, (0.1,), (0.12, 0, 1.1))], opt)
     o = Oracle(M, 1, 64)
     o.run(stage=3)
@@ -274,3 +282,47 @@ def test_fp64_vs_fp32_drift_small(Oracle):
         o.set("ctrl_pos", tgt)
         o.run(100)
     np.testing.assert_allclose(o32.get("qpos"), o64.get("qpos"), atol=2e-3)
+
+
+def _sat_depth(pa, Ra, ha, pb, Rb, hb):
+    """Minimum translation distance of two overlapping boxes by the separating-axis theorem (15 axes)."""
+    axes = [Ra[:, i] for i in range(3)] + [Rb[:, i] for i in range(3)]
+    for i in range(3):
+        for j in range(3):
+            c = np.cross(Ra[:, i], Rb[:, j])
+            if np.linalg.norm(c) > 1e-9:
+                axes.append(c / np.linalg.norm(c))
+    best, best_ax = np.inf, None
+    for L in axes:
+        ra = sum(abs(Ra[:, i] @ L) * ha[i] for i in range(3))
+        rb = sum(abs(Rb[:, i] @ L) * hb[i] for i in range(3))
+        ov = ra + rb - abs((pb - pa) @ L)
+        if ov < best:
+            best, best_ax = ov, L
+    return best, best_ax
+
+
+def test_gjk_epa_deep_box_overlap_matches_separating_axis_depth(Oracle):
+    """gjk_decomp.py safe GJK + EPA (taken after MPR on a deep overlap with no cached normal): for two boxes the
+    penetration is the exact minimum translation distance, which the separating-axis theorem gives in closed form."""
+    rng = np.random.default_rng(4)
+    checked = 0
+    for trial in range(12):
+        sa, sb = rng.uniform(0.1, 0.4, 3), rng.uniform(0.1, 0.4, 3)
+        qa, qb = rng.normal(size=4), rng.normal(size=4)
+        qa, qb = qa / np.linalg.norm(qa), qb / np.linalg.norm(qb)
+        pa = np.array([0.0, 0.0, 2.0])
+        pb = pa + rng.normal(size=3) * 0.05  # deep overlap
+        ents = [W.box_entity(sa, pa, quat=qa), W.box_entity(sb, pb, quat=qb)]
+        M = compile_model(ents, CompileOptions(dt=0.01, gravity=(0.0, 0.0, 0.0)))
+        o = Oracle(M, 1, 64)
+        o.run(1, stage=3)
+        c = o.get("contacts")[0]
+        assert len(c) >= 1
+        depth, ax = _sat_depth(pa, quat_to_R(qa), sa / 2, pb, quat_to_R(qb), sb / 2)
+        assert depth > 0.02
+        np.testing.assert_allclose(c[0, 10], depth, rtol=1e-6, atol=1e-9)
+        assert abs(abs(c[0, 7:10] @ ax) - 1.0) < 1e-6
+        assert int(o.get("errno")[0]) & 0x100  # the fallback ran
+        checked += 1
+    assert checked == 12
