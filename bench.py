@@ -173,10 +173,10 @@ def bench_serve(args):
                 p.S.dm.desc.solver_variant = g.solver_variant
     budget = int(args.serve_budget_gb * 2**30)
     server = Server(pools, budget, launch=launch)
-    sample = poisson_requests(1000.0, 100.0, seed=99)
+    sample = poisson_requests(1000.0, 100.0, seed=99, max_count=20000)
     mean_work = float(np.mean([r.n_envs * r.horizon for r in sample]))
     # capacity: a saturated queue (every request waiting at t = 0)
-    sat = poisson_requests(1e9, 1.0, seed=5)[:args.serve_saturate]
+    sat = poisson_requests(1e9, 1.0, seed=5, max_count=args.serve_saturate)
     for r in sat:
         r.arrival = 0.0
     log(f"serve: capacity run with {len(sat)} queued requests")
@@ -189,7 +189,7 @@ def bench_serve(args):
         reqs = poisson_requests(rate, args.serve_seconds, seed=4, start_id=0)
         st = server.run(reqs, log=log)
         st.update(load=load, offered_requests_per_s=rate, n_requests=len(reqs))
-        log(f"serve: load {load:.1f}: {len(reqs)} requests, p50 {st['step_ms_p50']:.3f} ms, p99 {st['step_ms_p99']:.3f} ms")
+        log(f"serve: load {load:.1f}: {len(reqs)} requests, p50 {st['step_ms_p50']} ms, p99 {st['step_ms_p99']} ms")
         sweep.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()})
     at80 = next(x for x in sweep if x["load"] == 0.8)
     out = {
@@ -228,7 +228,7 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's horizon)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="pile", choices=["pile", "arm", "quadruped", "mixed", "serve"])
-    ap.add_argument("--serve-seconds", type=float, default=6.0, help="arrival window per offered-load level")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window per offered-load level")
     ap.add_argument("--serve-budget-gb", type=float, default=12.0, help="device memory budget for admission")
     ap.add_argument("--serve-saturate", type=int, default=24, help="requests queued at t=0 to measure capacity")
     ap.add_argument("--n-envs", type=int, default=0)
@@ -286,6 +286,8 @@ def main():
         if S._substep % 10 == 0:
             S.check_errno()
         step += 1
+        if step % 100 == 0:
+            print(f"bench: {step} / {args.steps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0

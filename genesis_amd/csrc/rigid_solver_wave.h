@@ -55,8 +55,18 @@ struct WaveLds {
   lds_f* vec;
   lds_f* force;
   lds_f* acc_smooth;
-  lds_u64* pat;  // [E][2] entity pattern bits (row e: entities e' with block (e, e') nonzero)
-  lds_i* ent_of_dof;          // [D]
+  // The factorisation works in elimination order: entity e is eliminated at position vof[e] (a minimum-degree order
+  // of the env's entity graph), and "factor space" numbers entities by position and dofs position by position.
+  // pat, es, ee, ent_of_dof, the levels and the LDS Hessian layout are in factor space; nent, nes, nee, moff are in
+  // the model's own numbering.
+  lds_u64* pat;  // [E][2] pattern bits (row v: positions v' with block (v, v') nonzero)
+  lds_i* ent_of_dof;          // [D] factor space: position of the entity of each factor dof
+  lds_i* nent;                // [D] entity of each model dof
+  lds_i* pdof;                // [D] factor dof of each model dof
+  lds_i* vof;                 // [E] position of each entity
+  lds_i* ord;                 // [E] entity at each position
+  lds_i* nes;                 // [E] first model dof of each entity
+  lds_i* nee;                 // [E] end model dof of each entity
   lds_i* lvl_order;           // [E] entities sorted by forward level (elimination dependencies)
   lds_i* lvl_start;           // [E + 2]
   lds_i* rlvl_order;          // [E] entities sorted by backward level
@@ -67,14 +77,14 @@ struct WaveLds {
   // (r = i - es[e]) and holds the dofs of the pattern entities below e (entity k at pre[pstart[e] + rank of k in
   // the row]) followed by e's own dofs up to i.
   lds_f* hl;                // [kWaveHBudget]
-  lds_i* es;                  // [E] first dof of each entity
-  lds_i* ee;                  // [E] end dof of each entity
+  lds_i* es;                  // [E] first factor dof of the entity at each position
+  lds_i* ee;                  // [E] end factor dof of the entity at each position
   lds_i* hoff;                // [E]
   lds_i* below;               // [E]
   lds_i* pstart;              // [E]
   lds_i* pre;                 // [kWavePreSlots]
   lds_i* hmode;               // [1] 1 = H in LDS, 0 = H in global memory
-  lds_i* moff;                // [E] offset of each entity's mass block (copy of ent_mass_offset)
+  lds_i* moff;                // [E] offset of each entity's mass block (copy of ent_mass_offset, model numbering)
 };
 
 constexpr int kWaveHBudget = 5120;   // floats of LDS for the Hessian of one env
@@ -82,7 +92,7 @@ constexpr int kWavePreSlots = 512;   // off-diagonal pattern blocks addressable 
 
 __host__ __device__ inline size_t wave_lds_bytes(int D, int E) {
   return sizeof(float) * (10 * (size_t)D + kWaveHBudget) + sizeof(unsigned long long) * 2 * (size_t)E +
-         sizeof(int) * (size_t)D + sizeof(int) * (5 * (size_t)E + 6) + sizeof(int) * (6 * (size_t)E + kWavePreSlots + 1) +
+         sizeof(int) * 3 * (size_t)D + sizeof(int) * (9 * (size_t)E + 6) + sizeof(int) * (6 * (size_t)E + kWavePreSlots + 1) +
          64;
 }
 
@@ -104,7 +114,13 @@ __device__ inline WaveLds wave_lds_carve(lds_c* base, int D, int E) {
   lds_c* pc = reinterpret_cast<lds_c*>(f) + (((p + 7u) & ~7u) - p);
   w.pat = reinterpret_cast<lds_u64*>(pc);
   w.ent_of_dof = reinterpret_cast<lds_i*>(w.pat + 2 * E);
-  w.lvl_order = w.ent_of_dof + D;
+  w.nent = w.ent_of_dof + D;
+  w.pdof = w.nent + D;
+  w.vof = w.pdof + D;
+  w.ord = w.vof + E;
+  w.nes = w.ord + E;
+  w.nee = w.nes + E;
+  w.lvl_order = w.nee + E;
   w.lvl_start = w.lvl_order + E;
   w.rlvl_order = w.lvl_start + E + 2;
   w.rlvl_start = w.rlvl_order + E;
@@ -129,39 +145,80 @@ struct WaveCtx {
   const Env& e;
   const WaveLds& L;
   int lane, D, E, n_con, ne, nef;
-  // entity-block mass matrix entry (i, j) of entity ie, with the block geometry read from LDS
+  // entity-block mass matrix entry (i, j) of entity ie (model numbering: entity and dofs), geometry read from LDS
   __device__ __forceinline__ float mm(int ie, int i, int j) const {
-    int st = L.es[ie], n = L.ee[ie] - st;
+    int st = L.nes[ie], n = L.nee[ie] - st;
     return e.s.mass_blk[e.ix(L.moff[ie] + (i - st) * n + (j - st))];
   }
 };
 
-// entities touched by constraint row c (first and last stored dof; rows touch at most two entities)
+// entities touched by constraint row c (first and last stored dof; rows touch at most two entities), model numbering
 __device__ __forceinline__ void row_entities(const WaveCtx& w, int c, int& ea, int& eb) {
   int nnz = w.e.EI(w.e.s.jac_nnz, c);
-  ea = w.L.ent_of_dof[w.e.JI(c, 0)];
-  eb = w.L.ent_of_dof[w.e.JI(c, nnz - 1)];
+  ea = w.L.nent[w.e.JI(c, 0)];
+  eb = w.L.nent[w.e.JI(c, nnz - 1)];
 }
 
-// Build the entity pattern (with symbolic fill) and the per-entity ordered row lists.
+// Minimum-degree elimination order of the entity graph (entities as nodes, an edge per pair of entities sharing a
+// constraint row), with its fill: eliminating a node joins its remaining neighbours into a clique. Ties go to the
+// lower entity index; entities without dofs come last. Lanes own entities lane and lane + 64; adj0/adj1 hold their
+// adjacency (two words each) on entry and the filled adjacency on exit. Writes L.ord and L.vof.
+__device__ inline void wave_min_degree(const WaveCtx& w, unsigned long long (&adj0)[2], unsigned long long (&adj1)[2]) {
+  const WaveLds& L = w.L;
+  const int E = w.E, lane = w.lane;
+  const int e0 = lane, e1 = lane + 64;
+  const bool has0 = e0 < E && L.nee[e0] > L.nes[e0];
+  const bool has1 = e1 < E && L.nee[e1] > L.nes[e1];
+  unsigned long long al0 = __ballot(has0), al1 = __ballot(has1);
+  int n_alive = __popcll(al0) + __popcll(al1);
+  for (int step = 0; step < n_alive; step++) {
+    // (degree, entity) minimum over the alive entities
+    int k0 = has0 && ((al0 >> lane) & 1ull) ? ((__popcll(adj0[0] & al0) + __popcll(adj0[1] & al1)) << 8) | e0 : 0x7fffffff;
+    int k1 = has1 && ((al1 >> lane) & 1ull) ? ((__popcll(adj1[0] & al0) + __popcll(adj1[1] & al1)) << 8) | e1 : 0x7fffffff;
+    int k = min(k0, k1);
+    for (int o = kWave / 2; o > 0; o >>= 1) k = min(k, __shfl_xor(k, o, kWave));
+    const int v = k & 0xff;
+    // v's live neighbours become a clique
+    const int src = v & 63;
+    unsigned long long n0 = __shfl(v < 64 ? adj0[0] : adj1[0], src, kWave);
+    unsigned long long n1 = __shfl(v < 64 ? adj0[1] : adj1[1], src, kWave);
+    if (v < 64) al0 &= ~(1ull << v); else al1 &= ~(1ull << (v - 64));
+    n0 &= al0;
+    n1 &= al1;
+    if ((n0 >> lane) & 1ull) { adj0[0] |= n0 & ~(1ull << lane); adj0[1] |= n1; }
+    if ((n1 >> lane) & 1ull) { adj1[0] |= n0; adj1[1] |= n1 & ~(1ull << lane); }
+    if (lane == 0) {
+      L.ord[step] = v;
+      L.vof[v] = step;
+    }
+  }
+  if (lane == 0) {
+    int pos = n_alive;
+    for (int ie = 0; ie < E; ie++)
+      if (L.nee[ie] <= L.nes[ie]) {
+        L.ord[pos] = ie;
+        L.vof[ie] = pos++;
+      }
+  }
+  __syncthreads();
+}
+
+// Build the factor-space structure: entity graph, minimum-degree order with its fill, pattern, dof maps, levels and
+// LDS Hessian layout, plus the per-entity ordered row lists (model numbering).
 __device__ inline void wave_build_structure(const WaveCtx& w) {
   const Env& e = w.e;
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const WaveLds& L = w.L;
-  for (int d = w.lane; d < w.D; d += kWave) L.ent_of_dof[d] = m.dof_entity[d];
+  for (int d = w.lane; d < w.D; d += kWave) L.nent[d] = m.dof_entity[d];
   for (int i = w.lane; i < w.E; i += kWave) {
-    L.es[i] = m.ent_dof_start[i];
-    L.ee[i] = m.ent_dof_end[i];
+    L.nes[i] = m.ent_dof_start[i];
+    L.nee[i] = m.ent_dof_end[i];
     L.moff[i] = m.ent_mass_offset[i];
   }
   for (int i = w.lane; i < 2 * w.E; i += kWave) L.pat[i] = 0ull;
   __syncthreads();
-  if (w.lane == 0) {
-    for (int ie = 0; ie < w.E; ie++)
-      if (L.ee[ie] > L.es[ie]) L.pat[2 * ie + (ie >> 6)] |= 1ull << (ie & 63);
-  }
-  __syncthreads();
+  // entity graph in model numbering (pat as scratch)
   for (int c = w.lane; c < w.n_con; c += kWave) {
     int ea, eb;
     row_entities(w, c, ea, eb);
@@ -171,32 +228,50 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
     }
   }
   __syncthreads();
-  if (w.lane == 0) {
-    // symbolic elimination in entity order: neighbours above k of each eliminated k become mutually adjacent
-    for (int k = 0; k < w.E; k++) {
-      unsigned long long s0 = L.pat[2 * k], s1 = L.pat[2 * k + 1];
-      // keep entities > k
-      if (k < 64) {
-        s0 &= (k == 63) ? 0ull : (~0ull << (k + 1));
-      } else {
-        s0 = 0ull;
-        s1 &= (k == 127) ? 0ull : (~0ull << (k - 63));
-      }
-      unsigned long long t0 = s0, t1 = s1;
-      while (t0) {
-        int i = __ffsll((long long)t0) - 1;
-        t0 &= t0 - 1;
-        L.pat[2 * i] |= s0;
-        L.pat[2 * i + 1] |= s1;
-      }
-      while (t1) {
-        int i = 64 + __ffsll((long long)t1) - 1;
-        t1 &= t1 - 1;
-        L.pat[2 * i] |= s0;
-        L.pat[2 * i + 1] |= s1;
+  unsigned long long adj0[2] = {0ull, 0ull}, adj1[2] = {0ull, 0ull};
+  if (w.lane < w.E) { adj0[0] = L.pat[2 * w.lane]; adj0[1] = L.pat[2 * w.lane + 1]; }
+  if (w.lane + 64 < w.E) { adj1[0] = L.pat[2 * (w.lane + 64)]; adj1[1] = L.pat[2 * (w.lane + 64) + 1]; }
+  wave_min_degree(w, adj0, adj1);
+  // factor-space pattern: row vof[e] gets the positions of e's filled neighbours and its own
+  for (int i = w.lane; i < 2 * w.E; i += kWave) L.pat[i] = 0ull;
+  __syncthreads();
+  for (int h = 0; h < 2; h++) {
+    const int ie = w.lane + 64 * h;
+    if (ie >= w.E || L.nee[ie] <= L.nes[ie]) continue;
+    const unsigned long long* a = h ? adj1 : adj0;
+    const int vi = L.vof[ie];
+    unsigned long long r0 = 0ull, r1 = 0ull;
+    if (vi < 64) r0 |= 1ull << vi; else r1 |= 1ull << (vi - 64);
+    for (int word = 0; word < 2; word++) {
+      unsigned long long t = a[word];
+      while (t) {
+        int x = word * 64 + __ffsll((long long)t) - 1;
+        t &= t - 1;
+        int vx = L.vof[x];
+        if (vx < 64) r0 |= 1ull << vx; else r1 |= 1ull << (vx - 64);
       }
     }
+    L.pat[2 * vi] = r0;
+    L.pat[2 * vi + 1] = r1;
   }
+  // factor dof ranges by position, then the dof maps
+  if (w.lane == 0) {
+    int acc = 0;
+    for (int v = 0; v < w.E; v++) {
+      int ie = L.ord[v];
+      L.es[v] = acc;
+      acc += L.nee[ie] - L.nes[ie];
+      L.ee[v] = acc;
+    }
+  }
+  __syncthreads();
+  for (int d = w.lane; d < w.D; d += kWave) {
+    int ie = L.nent[d], v = L.vof[ie];
+    int p = L.es[v] + (d - L.nes[ie]);
+    L.pdof[d] = p;
+    L.ent_of_dof[p] = v;
+  }
+  __syncthreads();
   // entity pair of every row, once (env-major scratch)
   int* rent = s.efc_ent + (size_t)e.b * m.max_rows;
   for (int c = w.lane; c < w.n_con; c += kWave) {
@@ -361,9 +436,9 @@ struct HLds {
 __device__ inline void wave_mass_mul(const WaveCtx& w, const lds_f* x, lds_f* out) {
   const Env& e = w.e;
   for (int i = w.lane; i < w.D; i += kWave) {
-    int ie = w.L.ent_of_dof[i];
+    int ie = w.L.nent[i];
     float v = 0.f;
-    for (int j = w.L.es[ie]; j < w.L.ee[ie]; j++) v += w.mm(ie, i, j) * x[j];
+    for (int j = w.L.nes[ie]; j < w.L.nee[ie]; j++) v += w.mm(ie, i, j) * x[j];
     out[i] = v;
   }
 }
@@ -403,7 +478,7 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, fl
   for (int d = w.lane; d < w.D; d += kWave) L.qfrc[d] = 0.f;
   __syncthreads();
   for (int ie = w.lane; ie < w.E; ie += kWave) {
-    const int s0 = L.es[ie], s1 = L.ee[ie];
+    const int s0 = L.nes[ie], s1 = L.nee[ie];
     if (s1 <= s0) continue;
     for (int t = ptr[ie]; t < ptr[ie + 1]; t++) {
       int c = lst[t];
@@ -562,26 +637,31 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
         }
       }
     }
-    for (int t = ptr[ei]; t < ptr[ei + 1]; t++) {
+    // the entity's rows in ascending order; each unordered dof pair of a row once (k2 <= k1 in the row's model
+    // order, as the reference walks it), stored at (max, min) of its factor dofs by the lane owning the max
+    const int ne = L.ord[ei];
+    for (int t = ptr[ne]; t < ptr[ne + 1]; t++) {
       int c = lst[t];
       int nnz = e.EI(s.jac_nnz, c);
       float Dc = e.EF(s.efc_D, c), act = (float)(e.EI(s.efc_active, c) & 1);
       for (int k1 = 0; k1 < nnz; k1++) {
-        int i = e.JI(c, k1);
-        if (i < s0 || i >= s1) continue;
         float v1 = e.JV(c, k1);
         if (!(fabsf(v1) > m.eps)) continue;
+        const int p1 = L.pdof[e.JI(c, k1)];
         for (int k2 = 0; k2 <= k1; k2++) {
-          int i2 = e.JI(c, k2);
-          int e2 = L.ent_of_dof[i2];
-          float* p2 = h.blk(i, ei, e2) + (i2 - L.es[e2]);
-          *p2 = *p2 + e.JV(c, k2) * v1 * Dc * act;
+          const int p2 = L.pdof[e.JI(c, k2)];
+          const int pi = p1 >= p2 ? p1 : p2, pj = p1 >= p2 ? p2 : p1;
+          if (pi < s0 || pi >= s1) continue;
+          const int ej = L.ent_of_dof[pj];
+          float* q = h.blk(pi, ei, ej) + (pj - L.es[ej]);
+          *q = *q + e.JV(c, k2) * v1 * Dc * act;
         }
       }
     }
+    const int n0 = L.nes[ne];
     for (int i = s0; i < s1; i++) {
       float* pi = h.blk(i, ei, ei);
-      for (int j = s0; j <= i; j++) pi[j - s0] = pi[j - s0] + w.mm(ei, i, j);
+      for (int j = s0; j <= i; j++) pi[j - s0] = pi[j - s0] + w.mm(ne, n0 + (i - s0), n0 + (j - s0));
     }
   }
   __syncthreads();
@@ -598,11 +678,13 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
   float sq = sqrtf(e.EF(s.efc_D, c));
   int ea, eb;
   row_entities(w, c, ea, eb);
-  // vec is zero outside the entities reachable from ea / eb through the pattern; clear those entities
+  ea = L.vof[ea];
+  eb = L.vof[eb];
+  // vec (factor space) is zero outside the entities reachable from ea / eb through the pattern
   for (int d = w.lane; d < w.D; d += kWave) L.vec[d] = 0.f;
   __syncthreads();
   int nnz = e.EI(s.jac_nnz, c);
-  if (w.lane < nnz) L.vec[e.JI(c, w.lane)] = e.JV(c, w.lane) * sq;
+  if (w.lane < nnz) L.vec[L.pdof[e.JI(c, w.lane)]] = e.JV(c, w.lane) * sq;
   __syncthreads();
   bool rank_ok = true;
   // Entities whose vec entries can be nonzero: the row's two, then every pattern entity below one that had a nonzero
@@ -677,10 +759,11 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
   return rank_ok;
 }
 
-// Mgrad = H^-1 grad through the factor (func_nt_chol_solve): forward over forward levels, backward over backward
-// levels; one lane per entity, each entry with the reference's summation order.
+// x = H^-1 x in place, x in factor space, through the factor (func_nt_chol_solve): forward over forward levels,
+// backward over backward levels; one lane per entity. x[i] still holds its right-hand side when lane reads it, and
+// every x[j] it subtracts is final (an earlier level, or an earlier dof of the same entity).
 template <class HS>
-__device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h) {
+__device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h, lds_f* x) {
   const WaveLds& L = w.L;
   __syncthreads();
   int nl = L.n_lvl[0];
@@ -688,7 +771,7 @@ __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h) {
     for (int idx = L.lvl_start[lv] + w.lane; idx < L.lvl_start[lv + 1]; idx += kWave) {
       int ei = L.lvl_order[idx];
       for (int i = L.es[ei]; i < L.ee[ei]; i++) {
-        float g = L.grad[i];
+        float g = x[i];
         for (int word = 0; word < 2; word++) {
           unsigned long long t = L.pat[2 * ei + word];
           bool stop = false;
@@ -699,11 +782,11 @@ __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h) {
             if (k0 >= i) { stop = true; break; }
             if (k1 > i) k1 = i;
             const float* pk = h.blk(i, ei, ek);
-            for (int j = k0; j < k1; j++) g = g - pk[j - k0] * L.Mgrad[j];
+            for (int j = k0; j < k1; j++) g = g - pk[j - k0] * x[j];
           }
           if (stop) break;
         }
-        L.Mgrad[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
+        x[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
       }
     }
     __syncthreads();
@@ -713,21 +796,28 @@ __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h) {
     for (int idx = L.rlvl_start[lv] + w.lane; idx < L.rlvl_start[lv + 1]; idx += kWave) {
       int ei = L.rlvl_order[idx];
       for (int i = L.ee[ei] - 1; i >= L.es[ei]; i--) {
-        float g = L.Mgrad[i];
+        float g = x[i];
         const int r = i - L.es[ei];
-        wave_for_col(L, i, ei, [&](int j) { g = g - h.blk(j, L.ent_of_dof[j], ei)[r] * L.Mgrad[j]; });
-        L.Mgrad[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
+        wave_for_col(L, i, ei, [&](int j) { g = g - h.blk(j, L.ent_of_dof[j], ei)[r] * x[j]; });
+        x[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
       }
     }
     __syncthreads();
   }
 }
 
+// grad = M qacc - force - J^T f, Mgrad = H^-1 grad (through vec, the factor-space work vector)
 template <class HS>
 __device__ __forceinline__ void wave_update_gradient(const WaveCtx& w, const HS& h) {
   const WaveLds& L = w.L;
-  for (int d = w.lane; d < w.D; d += kWave) L.grad[d] = L.Ma[d] - L.force[d] - L.qfrc[d];
-  wave_chol_solve(w, h);
+  for (int d = w.lane; d < w.D; d += kWave) {
+    float g = L.Ma[d] - L.force[d] - L.qfrc[d];
+    L.grad[d] = g;
+    L.vec[L.pdof[d]] = g;
+  }
+  wave_chol_solve(w, h, L.vec);
+  for (int d = w.lane; d < w.D; d += kWave) L.Mgrad[d] = L.vec[L.pdof[d]];
+  __syncthreads();
 }
 
 struct WLS {

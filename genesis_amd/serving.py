@@ -220,13 +220,15 @@ class Server:
 
 
 def poisson_requests(rate: float, duration: float, seed: int = 4, scenes=("arm", "quadruped", "pile20"),
-                     max_envs: int = 4096, horizon=(50, 500), start_id: int = 0) -> List[Request]:
-    """Poisson arrivals over [0, duration): scene uniform, n_envs log-uniform in [1, max_envs], horizon uniform."""
+                     max_envs: int = 4096, horizon=(50, 500), start_id: int = 0,
+                     max_count: Optional[int] = None) -> List[Request]:
+    """Poisson arrivals over [0, duration), at most max_count of them: scene uniform, n_envs log-uniform in
+    [1, max_envs], horizon uniform."""
     rng = np.random.default_rng(seed)
     out, t, i = [], 0.0, start_id
     while True:
         t += rng.exponential(1.0 / rate)
-        if t >= duration:
+        if t >= duration or (max_count is not None and len(out) >= max_count):
             return out
         n = int(round(np.exp(rng.uniform(0.0, np.log(max_envs)))))
         out.append(Request(i, scenes[rng.integers(len(scenes))], max(1, n), int(rng.integers(horizon[0], horizon[1] + 1)),

@@ -217,9 +217,10 @@ def test_step_parity_resynced(case, B, steps, variant, pre):
     if case == "quadruped":
         S.set_qpos(torch.as_tensor(np.tile(np.concatenate([W.GO2_INIT_BASE, W.GO2_STANDING]), (B, 1)), dtype=torch.float32))
     O = _oracle(M, B)
-    # The wavefront solver reorders floating-point reductions, so it is held to the FP32 oracle within TOL except
-    # where the FP32 oracle itself is ill-conditioned: there it must stay within 4x the oracle's own FP32-vs-FP64
-    # spread on the same step (contact-rich Newton solves near an active-set switch; see the DESIGN notes).
+    # The wavefront solver factorises in a fill-reducing entity order and sums rows in wave trees, so its rounding is
+    # not the oracle's: it is held to the FP32 oracle within TOL, except in an env whose FP32 solve is itself
+    # ill-conditioned on that step: there every entry must stay within 4x the largest FP32-vs-FP64 oracle spread of
+    # that env (the FP32 precision of that env's coupled solve, measured by an FP64 shadow oracle).
     O64 = _oracle(M, B, 64) if variant == 1 else None
     worst = 0.0
     mismatched_contacts = 0
@@ -247,7 +248,7 @@ def test_step_parity_resynced(case, B, steps, variant, pre):
             bound = TOL + TOL * np.abs(o_[ok])
             if O64 is not None:
                 o64 = (O64.get("qpos") if name == "qpos" else O64.get("vel"))[ok]
-                spread = np.abs(o_[ok] - o64)
+                spread = np.abs(o_[ok] - o64).max(axis=1, keepdims=True)
                 n_conditioned += int(((err > bound) & (err <= 4 * spread + TOL)).sum())
                 bound = np.maximum(bound, 4 * spread + TOL)
             worst = max(worst, float((err - bound).max(initial=-1.0)))
