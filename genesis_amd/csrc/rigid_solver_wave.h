@@ -415,12 +415,16 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
 // Both stores hand out a pointer p with H(i, k) = p[k - es[ek]] for the dofs k of entity ek <= ent(i) in the pattern
 // row of ent(i).
 struct HGlobal {
+  // entries written by one lane and read by another must pass through a fence (global memory, unlike LDS, gives a
+  // wavefront no ordering between its lanes' accesses)
+  static constexpr bool kFence = true;
   const Env& e;
   const lds_i* es;
   __device__ __forceinline__ float* blk(int i, int ei, int ek) const { return &e.H(i, es[ek]); }
 };
 
 struct HLds {
+  static constexpr bool kFence = false;
   const WaveLds& L;
   __device__ __forceinline__ float* blk(int i, int ei, int ek) const {
     int r = i - L.es[ei];
@@ -563,46 +567,97 @@ __device__ __forceinline__ float wave_row_dot(const WaveCtx& w, const HS& h, int
   return acc;
 }
 
-// Left-looking Cholesky over the pattern with the entries and summation orders of func_nt_chol_factor. Entities of
-// one forward level have no predecessor in common level, so one lane factorises a whole entity block column.
+// The n entities of one level are independent; the wavefront is cut into groups of G lanes (the largest power of two
+// with G * n <= 64, at least 1), group q taking entities q, q + 64 / G, ... of the level. Inside a group, the terms
+// of every dot product are dealt round-robin to its lanes and summed by a butterfly, so all lanes of the group hold
+// the same value; each entry is written by the group's lane 0. A wavefront's LDS accesses complete in order, so an
+// entry written by one lane is seen by the reads any lane issues after it.
+struct LaneGroup {
+  int G, q, r, ng;
+};
+__device__ __forceinline__ LaneGroup lane_group(int n, int lane) {
+  int G = kWave;
+  while (G > 1 && G * n > kWave) G >>= 1;
+  return LaneGroup{G, lane / G, lane & (G - 1), kWave / G};
+}
+__device__ __forceinline__ float group_sum(float v, int G) {
+  for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// this lane's share (terms t with t = r mod G) of sum_{k < i in row i's pattern} a(k) * b(k): pattern entities in
+// ascending order, dofs ascending; a and b are given the entity and the dof
+template <class F>
+__device__ __forceinline__ float row_terms(const WaveLds& L, int i, int ei, int G, int r, F term) {
+  float acc = 0.f;
+  int t0 = 0;
+  for (int word = 0; word < 2; word++) {
+    unsigned long long t = L.pat[2 * ei + word];
+    while (t) {
+      int ek = word * 64 + __ffsll((long long)t) - 1;
+      t &= t - 1;
+      int k0 = L.es[ek], k1 = L.ee[ek];
+      if (k0 >= i) return acc;
+      if (k1 > i) k1 = i;
+      for (int j = k0 + ((r - t0) & (G - 1)); j < k1; j += G) acc += term(ek, k0, j);
+      t0 += k1 - k0;
+    }
+  }
+  return acc;
+}
+
+// f(j) for this lane's share (positions t = r mod G) of the entries of column i below the diagonal, in the order of
+// wave_col_candidate
+template <class F>
+__device__ __forceinline__ void col_terms(const WaveLds& L, int i, int ei, int G, int r, F f) {
+  int t0 = 0;
+  {
+    const int k1 = L.ee[ei];
+    for (int j = i + 1 + (r & (G - 1)); j < k1; j += G) f(j);
+    t0 = k1 - i - 1;
+  }
+  for (int word = (ei + 1) >> 6; word < 2; word++) {
+    unsigned long long bits = L.pat[2 * ei + word];
+    int lo = ei + 1 - word * 64;
+    if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
+    while (bits) {
+      int ej = word * 64 + __ffsll((long long)bits) - 1;
+      bits &= bits - 1;
+      const int k0 = L.es[ej], k1 = L.ee[ej];
+      for (int j = k0 + ((r - t0) & (G - 1)); j < k1; j += G) f(j);
+      t0 += k1 - k0;
+    }
+  }
+}
+
+// Left-looking Cholesky over the pattern (func_nt_chol_factor's entries, level by level, lane groups per entity)
 template <class HS>
 __device__ __forceinline__ void wave_chol_factor(const WaveCtx& w, const HS& h) {
   const float EPS = w.e.m.eps;
   const WaveLds& L = w.L;
   int nl = L.n_lvl[0];
   for (int lv = 0; lv < nl; lv++) {
-    for (int idx = L.lvl_start[lv] + w.lane; idx < L.lvl_start[lv + 1]; idx += kWave) {
+    const int i0 = L.lvl_start[lv], i1 = L.lvl_start[lv + 1];
+    const LaneGroup g = lane_group(i1 - i0, w.lane);
+    for (int idx = i0 + g.q; idx < i1; idx += g.ng) {
       int ei = L.lvl_order[idx];
       int s0 = L.es[ei], s1 = L.ee[ei];
       for (int i = s0; i < s1; i++) {
-        // diagonal: tmp = H_ii, then tmp -= H_ik^2 for k < i in ascending order, like the reference
+        float sq = row_terms(L, i, ei, g.G, g.r, [&](int ek, int k0, int j) {
+          float v = h.blk(i, ei, ek)[j - k0];
+          return v * v;
+        });
+        sq = group_sum(sq, g.G);
         float* di = h.blk(i, ei, ei);
-        float tmp = di[i - s0];
-        for (int word = 0; word < 2; word++) {
-          unsigned long long t = L.pat[2 * ei + word];
-          bool stop = false;
-          while (t) {
-            int ek = word * 64 + __ffsll((long long)t) - 1;
-            t &= t - 1;
-            int k0 = L.es[ek], k1 = L.ee[ek];
-            if (k0 >= i) { stop = true; break; }
-            if (k1 > i) k1 = i;
-            const float* pk = h.blk(i, ei, ek);
-            for (int k = 0; k < k1 - k0; k++) {
-              float v = pk[k];
-              tmp -= v * v;
-            }
-          }
-          if (stop) break;
-        }
-        float hii = sqrtf(fmaxf(tmp, EPS));
-        di[i - s0] = hii;
+        float hii = sqrtf(fmaxf(di[i - s0] - sq, EPS));
         float inv = 1.f / hii;
-        wave_for_col(L, i, ei, [&](int j) {
+        col_terms(L, i, ei, g.G, g.r, [&](int j) {
           float dt = wave_row_dot(w, h, j, i, i);
           float* pj = h.blk(j, L.ent_of_dof[j], ei) + (i - s0);
           *pj = (*pj - dt) * inv;
         });
+        if (g.r == 0) di[i - s0] = hii;
+        if (HS::kFence) __threadfence_block();
       }
     }
     __syncthreads();
@@ -760,46 +815,38 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
 }
 
 // x = H^-1 x in place, x in factor space, through the factor (func_nt_chol_solve): forward over forward levels,
-// backward over backward levels; one lane per entity. x[i] still holds its right-hand side when lane reads it, and
-// every x[j] it subtracts is final (an earlier level, or an earlier dof of the same entity).
+// backward over backward levels, lane groups per entity. x[i] still holds its right-hand side when it is read, and
+// every x[j] subtracted from it is final (an earlier level, or an earlier dof of the same entity).
 template <class HS>
 __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h, lds_f* x) {
   const WaveLds& L = w.L;
   __syncthreads();
   int nl = L.n_lvl[0];
   for (int lv = 0; lv < nl; lv++) {
-    for (int idx = L.lvl_start[lv] + w.lane; idx < L.lvl_start[lv + 1]; idx += kWave) {
+    const int i0 = L.lvl_start[lv], i1 = L.lvl_start[lv + 1];
+    const LaneGroup g = lane_group(i1 - i0, w.lane);
+    for (int idx = i0 + g.q; idx < i1; idx += g.ng) {
       int ei = L.lvl_order[idx];
       for (int i = L.es[ei]; i < L.ee[ei]; i++) {
-        float g = x[i];
-        for (int word = 0; word < 2; word++) {
-          unsigned long long t = L.pat[2 * ei + word];
-          bool stop = false;
-          while (t) {
-            int ek = word * 64 + __ffsll((long long)t) - 1;
-            t &= t - 1;
-            int k0 = L.es[ek], k1 = L.ee[ek];
-            if (k0 >= i) { stop = true; break; }
-            if (k1 > i) k1 = i;
-            const float* pk = h.blk(i, ei, ek);
-            for (int j = k0; j < k1; j++) g = g - pk[j - k0] * x[j];
-          }
-          if (stop) break;
-        }
-        x[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
+        float acc = row_terms(L, i, ei, g.G, g.r, [&](int ek, int k0, int j) { return h.blk(i, ei, ek)[j - k0] * x[j]; });
+        float v = (x[i] - group_sum(acc, g.G)) / h.blk(i, ei, ei)[i - L.es[ei]];
+        if (g.r == 0) x[i] = v;
       }
     }
     __syncthreads();
   }
   nl = L.n_lvl[1];
   for (int lv = 0; lv < nl; lv++) {
-    for (int idx = L.rlvl_start[lv] + w.lane; idx < L.rlvl_start[lv + 1]; idx += kWave) {
+    const int i0 = L.rlvl_start[lv], i1 = L.rlvl_start[lv + 1];
+    const LaneGroup g = lane_group(i1 - i0, w.lane);
+    for (int idx = i0 + g.q; idx < i1; idx += g.ng) {
       int ei = L.rlvl_order[idx];
       for (int i = L.ee[ei] - 1; i >= L.es[ei]; i--) {
-        float g = x[i];
         const int r = i - L.es[ei];
-        wave_for_col(L, i, ei, [&](int j) { g = g - h.blk(j, L.ent_of_dof[j], ei)[r] * x[j]; });
-        x[i] = g / h.blk(i, ei, ei)[i - L.es[ei]];
+        float acc = 0.f;
+        col_terms(L, i, ei, g.G, g.r, [&](int j) { acc += h.blk(j, L.ent_of_dof[j], ei)[r] * x[j]; });
+        float v = (x[i] - group_sum(acc, g.G)) / h.blk(i, ei, ei)[r];
+        if (g.r == 0) x[i] = v;
       }
     }
     __syncthreads();
