@@ -28,6 +28,19 @@ KERNELS = ["k_step1", "k_collide", "k_solve", "k_step2"]
 HORIZON = {"pile": 500, "arm": 1000, "quadruped": 1000}  # steps of each configuration
 
 
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by what the same counter reads for a device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(workload)
+    if not rec or rec.get("kernel") != kernel:
+        return None
+    return rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]
+
+
 def b_alg(M, n_ctrl):
     """Algorithmic HBM bytes per env-step: read+write qpos and qvel, read+write the warm start, read PD targets."""
     return 4 * (2 * M.n_qs + 4 * M.n_dofs + n_ctrl)
@@ -316,7 +329,8 @@ def main():
                    "solver": ["lane-per-env", "wavefront-per-env"][S.dm.desc.solver_variant]},
         "kernel_ms": {k: round(float(v), 4) for k, v in zip(KERNELS, k_avg)},
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(w.name, S.kernel_name(dom)),
                      "step_frac": balg * value / 1e9 / HBM_PEAK_GBS, "b_alg_per_env_step": balg},
     }
     if not args.no_cpu_baseline:

@@ -43,18 +43,45 @@ __device__ __forceinline__ int wave_or(int v) {
   return v;
 }
 
+// A constraint row of up to kStage entries (every row of the arm, quadruped and box scenes) is loaded into
+// registers with independent loads before it is used, so a lane does not wait for one global load per entry; wider
+// rows take the generic loops.
+constexpr int kStage = 12;
+struct RowStage {
+  float v[kStage];
+  int d[kStage];
+};
+__device__ __forceinline__ void stage_row(const Env& e, int c, int nnz, RowStage& r) {
+  const float* pv = &e.JV(c, 0);
+  const int* pi = &e.JI(c, 0);
+#pragma unroll
+  for (int k = 0; k < kStage; k++) {
+    r.v[k] = k < nnz ? pv[k] : 0.f;
+    r.d[k] = k < nnz ? pi[k] : 0;
+  }
+}
+// sum_k J[c, k] x[dof k] in entry order
+__device__ __forceinline__ float row_dot_vec(const Env& e, int c, int nnz, float acc, const lds_f* x) {
+  if (nnz <= kStage) {
+    RowStage r;
+    stage_row(e, c, nnz, r);
+#pragma unroll
+    for (int k = 0; k < kStage; k++)
+      if (k < nnz) acc += r.v[k] * x[r.d[k]];
+    return acc;
+  }
+  for (int k = 0; k < nnz; k++) acc += e.JV(c, k) * x[e.JI(c, k)];
+  return acc;
+}
+
 // LDS layout of one env (floats / ints), sized on the host by wave_lds_bytes()
 struct WaveLds {
   lds_f* qacc;
   lds_f* Ma;
-  lds_f* grad;
-  lds_f* Mgrad;
   lds_f* search;
   lds_f* mv;
   lds_f* qfrc;
-  lds_f* vec;
-  lds_f* force;
-  lds_f* acc_smooth;
+  lds_f* vec;  // factor-space work vector: rank-1 updates, then the gradient and the Newton direction
   // The factorisation works in elimination order: entity e is eliminated at position vof[e] (a minimum-degree order
   // of the env's entity graph), and "factor space" numbers entities by position and dofs position by position.
   // pat, es, ee, ent_of_dof, the levels and the LDS Hessian layout are in factor space; nent, nes, nee, moff are in
@@ -88,10 +115,10 @@ struct WaveLds {
 };
 
 constexpr int kWaveHBudget = 5120;   // floats of LDS for the Hessian of one env
-constexpr int kWavePreSlots = 512;   // off-diagonal pattern blocks addressable in LDS mode
+constexpr int kWavePreSlots = 256;   // off-diagonal pattern blocks addressable in LDS mode
 
 __host__ __device__ inline size_t wave_lds_bytes(int D, int E) {
-  return sizeof(float) * (10 * (size_t)D + kWaveHBudget) + sizeof(unsigned long long) * 2 * (size_t)E +
+  return sizeof(float) * (6 * (size_t)D + kWaveHBudget) + sizeof(unsigned long long) * 2 * (size_t)E +
          sizeof(int) * 3 * (size_t)D + sizeof(int) * (9 * (size_t)E + 6) + sizeof(int) * (6 * (size_t)E + kWavePreSlots + 1) +
          64;
 }
@@ -101,14 +128,10 @@ __device__ inline WaveLds wave_lds_carve(lds_c* base, int D, int E) {
   lds_f* f = reinterpret_cast<lds_f*>(base);
   w.qacc = f; f += D;
   w.Ma = f; f += D;
-  w.grad = f; f += D;
-  w.Mgrad = f; f += D;
   w.search = f; f += D;
   w.mv = f; f += D;
   w.qfrc = f; f += D;
   w.vec = f; f += D;
-  w.force = f; f += D;
-  w.acc_smooth = f; f += D;
   w.hl = f; f += kWaveHBudget;
   uint32_t p = (uint32_t)reinterpret_cast<uintptr_t>(f);
   lds_c* pc = reinterpret_cast<lds_c*>(f) + (((p + 7u) & ~7u) - p);
@@ -414,26 +437,41 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
 // ---- Hessian storage ----------------------------------------------------------------------------------------------
 // Both stores hand out a pointer p with H(i, k) = p[k - es[ek]] for the dofs k of entity ek <= ent(i) in the pattern
 // row of ent(i).
+// Entry (i, j) of the factor, with j a dof of entity ek in the pattern row of ei = ent(i), sits at
+// row(i, ei) + off(ei, ek) + j: the row part depends on the dof, the offset only on the entity pair;
+// blk(i, ei, ek) = row(i, ei) + off(ei, ek) + es[ek] is the start of block (ei, ek) in row i.
 struct HGlobal {
   // entries written by one lane and read by another must pass through a fence (global memory, unlike LDS, gives a
   // wavefront no ordering between its lanes' accesses)
   static constexpr bool kFence = true;
+  typedef float* ptr;
   const Env& e;
   const lds_i* es;
   __device__ __forceinline__ float* blk(int i, int ei, int ek) const { return &e.H(i, es[ek]); }
+  __device__ __forceinline__ float* row(int i, int ei) const { return &e.H(i, 0); }
+  __device__ __forceinline__ int off(int ei, int ek) const { return 0; }
+  // the wavefront solver's rows are env-major, so an env's packed triangle is contiguous from &H(0, 0)
+  __device__ __forceinline__ float* base() const { return &e.H(0, 0); }
+  __device__ __forceinline__ int row_off(int i, int ei) const { return i * (i + 1) / 2; }
 };
 
 struct HLds {
   static constexpr bool kFence = false;
+  typedef lds_f* ptr;
   const WaveLds& L;
-  __device__ __forceinline__ float* blk(int i, int ei, int ek) const {
-    int r = i - L.es[ei];
-    lds_f* row = L.hl + L.hoff[ei] + r * L.below[ei] + r * (r + 1) / 2;
-    if (ek == ei) return (float*)(row + L.below[ei]);
+  __device__ __forceinline__ int off(int ei, int ek) const {
+    if (ek == ei) return L.below[ei] - L.es[ei];
     unsigned long long w0 = L.pat[2 * ei], w1 = L.pat[2 * ei + 1];
     int pc = ek < 64 ? __popcll(w0 & ((1ull << ek) - 1ull)) : __popcll(w0) + __popcll(w1 & ((1ull << (ek - 64)) - 1ull));
-    return (float*)(row + L.pre[L.pstart[ei] + pc]);
+    return L.pre[L.pstart[ei] + pc] - L.es[ek];
   }
+  __device__ __forceinline__ int row_off(int i, int ei) const {
+    int r = i - L.es[ei];
+    return L.hoff[ei] + r * L.below[ei] + r * (r + 1) / 2;
+  }
+  __device__ __forceinline__ lds_f* base() const { return L.hl; }
+  __device__ __forceinline__ lds_f* row(int i, int ei) const { return L.hl + row_off(i, ei); }
+  __device__ __forceinline__ lds_f* blk(int i, int ei, int ek) const { return row(i, ei) + off(ei, ek) + L.es[ek]; }
 };
 
 // M x (entity blocks), one dof per lane, serial inner sum in the reference order
@@ -484,19 +522,44 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, fl
   for (int ie = w.lane; ie < w.E; ie += kWave) {
     const int s0 = L.nes[ie], s1 = L.nee[ie];
     if (s1 <= s0) continue;
-    for (int t = ptr[ie]; t < ptr[ie + 1]; t++) {
-      int c = lst[t];
-      int nnz = e.EI(s.jac_nnz, c);
-      float f = e.EF(s.efc_force, c);
-      for (int k = 0; k < nnz; k++) {
-        int d = e.JI(c, k);
-        if (d >= s0 && d < s1) L.qfrc[d] = L.qfrc[d] + e.JV(c, k) * f;
+    // two rows per iteration: both rows' loads are issued before either is accumulated (in row order)
+    const int t_end = ptr[ie + 1];
+    for (int t = ptr[ie]; t < t_end; t += 2) {
+      const bool two = t + 1 < t_end;
+      const int ca = lst[t], cb = two ? lst[t + 1] : ca;
+      // a zero force adds +-0 to sums that start at +0 and so never hold -0: skipping a row changes no bit
+      const float fa = e.EF(s.efc_force, ca), fb = two ? e.EF(s.efc_force, cb) : 0.f;
+      const int na = e.EI(s.jac_nnz, ca), nb = e.EI(s.jac_nnz, cb);
+      if (na <= kStage && nb <= kStage) {
+        RowStage ra, rb;
+        stage_row(e, ca, fa != 0.f ? na : 0, ra);
+        stage_row(e, cb, fb != 0.f ? nb : 0, rb);
+#pragma unroll
+        for (int k = 0; k < kStage; k++) {
+          int d = ra.d[k];
+          if (k < na && fa != 0.f && d >= s0 && d < s1) L.qfrc[d] = L.qfrc[d] + ra.v[k] * fa;
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; k++) {
+          int d = rb.d[k];
+          if (k < nb && fb != 0.f && d >= s0 && d < s1) L.qfrc[d] = L.qfrc[d] + rb.v[k] * fb;
+        }
+      } else {
+        for (int q = 0; q < (two ? 2 : 1); q++) {
+          const int c = q ? cb : ca, nnz = q ? nb : na;
+          const float f = q ? fb : fa;
+          if (f == 0.f) continue;
+          for (int k = 0; k < nnz; k++) {
+            int d = e.JI(c, k);
+            if (d >= s0 && d < s1) L.qfrc[d] = L.qfrc[d] + e.JV(c, k) * f;
+          }
+        }
       }
     }
   }
   double gi = 0.0;
   for (int d = w.lane; d < w.D; d += kWave) {
-    float v = 0.5f * (L.Ma[d] - L.force[d]) * (L.qacc[d] - L.acc_smooth[d]);
+    float v = 0.5f * (L.Ma[d] - e.F(s.force, d)) * (L.qacc[d] - e.F(s.acc_smooth, d));
     gi += (double)v;
   }
   __syncthreads();
@@ -559,8 +622,8 @@ __device__ __forceinline__ float wave_row_dot(const WaveCtx& w, const HS& h, int
       int k0 = L.es[ek], k1 = L.ee[ek];
       if (k0 >= kmax) return acc;
       if (k1 > kmax) k1 = kmax;
-      const float* pa = h.blk(a, ea, ek);
-      const float* pb = h.blk(bb, eb, ek);
+      const auto pa = h.blk(a, ea, ek);
+      const auto pb = h.blk(bb, eb, ek);
       for (int k = 0; k < k1 - k0; k++) acc += pa[k] * pb[k];
     }
   }
@@ -648,12 +711,12 @@ __device__ __forceinline__ void wave_chol_factor(const WaveCtx& w, const HS& h) 
           return v * v;
         });
         sq = group_sum(sq, g.G);
-        float* di = h.blk(i, ei, ei);
+        auto di = h.blk(i, ei, ei);
         float hii = sqrtf(fmaxf(di[i - s0] - sq, EPS));
         float inv = 1.f / hii;
         col_terms(L, i, ei, g.G, g.r, [&](int j) {
           float dt = wave_row_dot(w, h, j, i, i);
-          float* pj = h.blk(j, L.ent_of_dof[j], ei) + (i - s0);
+          auto pj = h.blk(j, L.ent_of_dof[j], ei) + (i - s0);
           *pj = (*pj - dt) * inv;
         });
         if (g.r == 0) di[i - s0] = hii;
@@ -687,7 +750,7 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
           int k0 = L.es[ek], k1 = L.ee[ek];
           if (k0 > i) break;
           if (k1 > i + 1) k1 = i + 1;
-          float* pk = h.blk(i, ei, ek);
+          auto pk = h.blk(i, ei, ek);
           for (int k = 0; k < k1 - k0; k++) pk[k] = 0.f;
         }
       }
@@ -697,8 +760,48 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
     const int ne = L.ord[ei];
     for (int t = ptr[ne]; t < ptr[ne + 1]; t++) {
       int c = lst[t];
+      // an inactive row adds +-0 to entries that start at +0 and so never hold -0: skipping it changes no bit
+      if (!(e.EI(s.efc_active, c) & 1)) continue;
       int nnz = e.EI(s.jac_nnz, c);
-      float Dc = e.EF(s.efc_D, c), act = (float)(e.EI(s.efc_active, c) & 1);
+      float Dc = e.EF(s.efc_D, c), act = 1.f;
+      if (nnz <= kStage) {
+        RowStage r;
+        stage_row(e, c, nnz, r);
+        // the row's other entity (by position); its dofs are columns of this entity's rows only when it comes first
+        int na, nb;
+        row_entities(w, c, na, nb);
+        const int va = L.vof[na], vb = L.vof[nb];
+        const int eo = va == ei ? vb : va;
+        const int off_own = h.off(ei, ei), off_oth = eo < ei ? h.off(ei, eo) : 0;
+        // per entry: factor dof, whether it is this entity's (bit k of own), its column position, its row offset
+        int pd[kStage], co[kStage], ro[kStage];
+        unsigned own = 0u;
+#pragma unroll
+        for (int k = 0; k < kStage; k++) {
+          pd[k] = k < nnz ? L.pdof[r.d[k]] : s0;
+          const bool o = pd[k] >= s0 && pd[k] < s1;
+          own |= (unsigned)o << k;
+          co[k] = pd[k] + (o ? off_own : off_oth);
+          ro[k] = h.row_off(o ? pd[k] : s0, ei);
+        }
+        const auto hb = h.base();
+#pragma unroll
+        for (int k1 = 0; k1 < kStage; k1++) {
+          const float v1 = r.v[k1];
+          if (k1 < nnz && fabsf(v1) > m.eps) {
+#pragma unroll
+            for (int k2 = 0; k2 <= k1; k2++) {
+              // entry (max, min) of the two factor dofs, kept by the lane whose entity holds the max
+              const bool first = pd[k1] >= pd[k2];
+              if ((own >> (first ? k1 : k2)) & 1u) {
+                auto q = hb + (first ? ro[k1] + co[k2] : ro[k2] + co[k1]);
+                *q = *q + r.v[k2] * v1 * Dc * act;
+              }
+            }
+          }
+        }
+        continue;
+      }
       for (int k1 = 0; k1 < nnz; k1++) {
         float v1 = e.JV(c, k1);
         if (!(fabsf(v1) > m.eps)) continue;
@@ -708,14 +811,14 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
           const int pi = p1 >= p2 ? p1 : p2, pj = p1 >= p2 ? p2 : p1;
           if (pi < s0 || pi >= s1) continue;
           const int ej = L.ent_of_dof[pj];
-          float* q = h.blk(pi, ei, ej) + (pj - L.es[ej]);
+          auto q = h.blk(pi, ei, ej) + (pj - L.es[ej]);
           *q = *q + e.JV(c, k2) * v1 * Dc * act;
         }
       }
     }
     const int n0 = L.nes[ne];
     for (int i = s0; i < s1; i++) {
-      float* pi = h.blk(i, ei, ei);
+      auto pi = h.blk(i, ei, ei);
       for (int j = s0; j <= i; j++) pi[j - s0] = pi[j - s0] + w.mm(ne, n0 + (i - s0), n0 + (j - s0));
     }
   }
@@ -774,7 +877,7 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
       float vk = L.vec[k];
       if (!(fabsf(vk) > m.eps)) continue;
       any = true;
-      float* dk = h.blk(k, ek, ek) + (k - k0);
+      auto dk = h.blk(k, ek, ek) + (k - k0);
       float Lkk = *dk;
       float tmp = Lkk * Lkk + vk * vk * sgn;
       if (tmp < m.eps) {
@@ -789,7 +892,7 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
         const int jb = __shfl(j_below, max(w.lane - rest, 0), kWave);
         const int i = w.lane < rest ? k + 1 + w.lane : (w.lane < rest + n_below ? jb : -1);
         if (i >= 0) {
-          float* pi = h.blk(i, L.ent_of_dof[i], ek) + (k - k0);
+          auto pi = h.blk(i, L.ent_of_dof[i], ek) + (k - k0);
           float hv = (*pi + sv * L.vec[i] * sgn) * cinv;
           *pi = hv;
           L.vec[i] = L.vec[i] * cc - sv * hv;
@@ -798,7 +901,7 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
         for (int t = w.lane;; t += kWave) {
           int i = wave_col_candidate(w, k, ek, t);
           if (i < 0) break;
-          float* pi = h.blk(i, L.ent_of_dof[i], ek) + (k - k0);
+          auto pi = h.blk(i, L.ent_of_dof[i], ek) + (k - k0);
           float hv = (*pi + sv * L.vec[i] * sgn) * cinv;
           *pi = hv;
           L.vec[i] = L.vec[i] * cc - sv * hv;
@@ -853,18 +956,21 @@ __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h, l
   }
 }
 
-// grad = M qacc - force - J^T f, Mgrad = H^-1 grad (through vec, the factor-space work vector)
+// grad = M qacc - force - J^T f into vec (factor space), then vec = H^-1 grad in place; returns this lane's share
+// of |grad|^2 (dofs lane, lane + 64, ... in that order). The Newton direction is -vec read back through pdof.
 template <class HS>
-__device__ __forceinline__ void wave_update_gradient(const WaveCtx& w, const HS& h) {
+__device__ __forceinline__ double wave_update_gradient(const WaveCtx& w, const HS& h) {
   const WaveLds& L = w.L;
+  const Env& e = w.e;
+  double g2 = 0.0;
   for (int d = w.lane; d < w.D; d += kWave) {
-    float g = L.Ma[d] - L.force[d] - L.qfrc[d];
-    L.grad[d] = g;
+    float g = L.Ma[d] - e.F(e.s.force, d) - L.qfrc[d];
+    g2 += (double)(g * g);
     L.vec[L.pdof[d]] = g;
   }
   wave_chol_solve(w, h, L.vec);
-  for (int d = w.lane; d < w.D; d += kWave) L.Mgrad[d] = L.vec[L.pdof[d]];
   __syncthreads();
+  return g2;
 }
 
 struct WLS {
@@ -924,7 +1030,7 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
   for (int c = w.lane; c < w.n_con; c += kWave) {
     float jv = 0.f;
     int nnz = e.EI(s.jac_nnz, c);
-    for (int k = 0; k < nnz; k++) jv += e.JV(c, k) * L.search[e.JI(c, k)];
+    jv = row_dot_vec(e, c, nnz, jv, L.search);
     e.EF(s.efc_jv, c) = jv;
     float Dc = e.EF(s.efc_D, c), ja = e.EF(s.efc_Jaref, c);
     e.QD(c, 0) = Dc * (0.5f * ja * ja);
@@ -934,7 +1040,7 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
   double a1 = 0.0, a2 = 0.0;
   for (int d = w.lane; d < w.D; d += kWave) {
     float sd = L.search[d];
-    a1 += (double)(sd * L.Ma[d] - sd * L.force[d]);
+    a1 += (double)(sd * L.Ma[d] - sd * e.F(s.force, d));
     a2 += (double)(0.5f * sd * L.mv[d]);
   }
   float qg0 = gauss, qg1 = (float)wave_sum_d(a1), qg2 = (float)wave_sum_d(a2);
@@ -1002,7 +1108,7 @@ __device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, double
   const int lane = w.lane, D = w.D, n_con = w.n_con;
   wave_hessian_direct(w, h);
   wave_update_gradient(w, h);
-  for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
+  for (int d = lane; d < D; d += kWave) L.search[d] = -L.vec[L.pdof[d]];
   __syncthreads();
   int iters = 0;
   if (n_con <= 0) return 0;
@@ -1047,17 +1153,15 @@ __device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, double
           }
         }
       }
-      wave_update_gradient(w, h);
+      double g2 = wave_update_gradient(w, h);
       float tol_scaled = (m.meaninertia * (float)(D > 1 ? D : 1)) * m.tolerance;
       double improvement = prev_cost - cost;
-      double g2 = 0.0;
-      for (int d = lane; d < D; d += kWave) g2 += (double)(L.grad[d] * L.grad[d]);
       float gn = sqrtf((float)wave_sum_d(g2));
       if (gn < tol_scaled || improvement < tol_scaled) {
         improved = false;
       } else {
         improved = true;
-        for (int d = lane; d < D; d += kWave) L.search[d] = -L.Mgrad[d];
+        for (int d = lane; d < D; d += kWave) L.search[d] = -L.vec[L.pdof[d]];
       }
       __syncthreads();
     }
@@ -1075,18 +1179,15 @@ __device__ __forceinline__ void wave_solve(const Env& e, char* lds) {
   int n_con = e.I(s.n_con, 0), ne = e.I(s.n_con, 1), nef = ne + e.I(s.n_con, 2);
   WaveCtx w{e, L, lane, D, E, n_con, ne, nef};
   int ws = e.I(s.is_warmstart, 0);
-  for (int d = lane; d < D; d += kWave) {
-    L.force[d] = e.F(s.force, d);
-    L.acc_smooth[d] = e.F(s.acc_smooth, d);
-    L.qacc[d] = (n_con > 0 && ws) ? e.F(s.qacc_ws, d) : L.acc_smooth[d];
-  }
+  // force (the smooth force until the epilogue) and acc_smooth are read from the state: nothing writes them here
+  for (int d = lane; d < D; d += kWave) L.qacc[d] = (n_con > 0 && ws) ? e.F(s.qacc_ws, d) : e.F(s.acc_smooth, d);
   __syncthreads();
   wave_build_structure(w);
   wave_mass_mul(w, L.qacc, L.Ma);
   for (int c = lane; c < n_con; c += kWave) {
     float ja = -e.EF(s.efc_aref, c);
     int nnz = e.EI(s.jac_nnz, c);
-    for (int k = 0; k < nnz; k++) ja += e.JV(c, k) * L.qacc[e.JI(c, k)];
+    ja = row_dot_vec(e, c, nnz, ja, L.qacc);
     e.EF(s.efc_Jaref, c) = ja;
   }
   __syncthreads();

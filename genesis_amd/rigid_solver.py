@@ -221,6 +221,12 @@ class RigidSolver:
     def launch_kernel(self, k: int):
         _lib.check(self._lib.gsr_launch_kernel(self.dm.desc, self.st.desc, k, self._stream()), "gsr_launch_kernel")
 
+    def kernel_name(self, k: int) -> str:
+        """Device kernel that launch_kernel(k) runs for this model's variant switches."""
+        d = self.dm.desc
+        wave = [d.dyn_variant, d.collide_variant, d.solver_variant, d.dyn_variant][k]
+        return ["k_step1", "k_collide", "k_solve", "k_step2"][k] + ("_wave" if wave else "")
+
     def run_stage(self, stage: int):
         _lib.check(self._lib.gsr_run_stage(self.dm.desc, self.st.desc, stage, self._stream()), "gsr_run_stage")
 
