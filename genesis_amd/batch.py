@@ -43,7 +43,8 @@ class Group:
 class FusedBatch:
     """One device table over several groups; `step()` runs fused substeps, `capture()`/`replay()` a hipGraph."""
 
-    def __init__(self, groups: Sequence[Group], persistent: bool = True, waves_per_simd: int = 4):
+    def __init__(self, groups: Sequence[Group], persistent: bool = True, waves_per_simd: int = 4,
+                 gjk_inline: bool = False):
         if not groups:
             raise ValueError("empty batch")
         self.lib = _lib.load()
@@ -66,6 +67,7 @@ class FusedBatch:
         self.info = _lib.GsrBatchInfo()
         self.info.persistent = int(bool(persistent))
         self.info.waves_per_simd = int(waves_per_simd)
+        self.info.gjk_inline = int(bool(gjk_inline))
         torch.cuda.synchronize(self.device)  # the table's zero fill precedes the synchronous upload
         _lib.check(self.lib.gsr_batch_upload(arr, n, ctypes.c_void_p(self.table.data_ptr()), nbytes,
                                              ctypes.byref(self.info)), "gsr_batch_upload")
@@ -154,8 +156,10 @@ def time_group(group: Group, steps: int = 3, warmup: int = 1, **launch) -> float
     return time_batch([group], steps, warmup, **launch)
 
 
-LAUNCH_CANDIDATES = [dict(persistent=True, waves_per_simd=4), dict(persistent=False, waves_per_simd=4),
-                     dict(persistent=True, waves_per_simd=2), dict(persistent=False, waves_per_simd=2)]
+# launch forms: persistent queue or one workgroup per tile; 128 or 256 VGPRs; GJK/EPA inside the fused kernel (one
+# launch per substep, lower occupancy) or deferred to a per-type resume launch
+LAUNCH_CANDIDATES = [dict(persistent=p, waves_per_simd=w, gjk_inline=g)
+                     for g in (False, True) for w in (4, 2) for p in (True, False)]
 
 
 def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: bool = False, log=print):

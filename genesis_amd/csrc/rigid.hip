@@ -90,13 +90,25 @@ __global__ void __launch_bounds__(kWave) k_step2_wave(gsr_model m, gsr_state s) 
   wave_step2(e);
 }
 
+// an env whose narrowphase deferred pairs to the GJK/EPA pass: its contact list and rows are completed there
+__device__ __forceinline__ bool gjk_pending(const Env& e) { return e.I(e.s.gjk_defer_n, 0) > 0; }
+
 __global__ void __launch_bounds__(kBlock) k_collide(gsr_model m, gsr_state s) {
   int b = env_id();
   if (b >= s.n_envs) return;
   Env e(m, s, b);
   dev_equality_rows(e);
   dev_collision(e);
-  dev_inequality_rows(e);
+  if (!gjk_pending(e)) dev_inequality_rows(e);
+}
+
+// the GJK/EPA pass of the plain step, one wavefront per env: deferred pairs, contact insertion, then the env's
+// inequality rows; envs without deferred pairs return at once
+__global__ void __launch_bounds__(kWave) k_gjk_fixup(gsr_model m, gsr_state s) {
+  if ((int)blockIdx.x >= s.n_envs) return;
+  const int b = xcd_env(blockIdx.x, s.n_envs);
+  Env e(m, s, b);
+  if (wave_gjk_fixup(e)) wave_rows(e);
 }
 
 __global__ void __launch_bounds__(kWave) k_collide_wave(gsr_model m, gsr_state s) {
@@ -170,38 +182,27 @@ constexpr size_t kRangesOffset = (kTypesOffset + sizeof(FusedTypes) + 255) & ~(s
 // scalar, invariant load, like a kernel argument, without copying the descriptors anywhere.
 typedef const FusedTypes __attribute__((address_space(4))) ConstTypes;
 
-// Stages are separate (not inlined) functions so each gets its own register allocation.
-__device__ __noinline__ void tile_lane_stages_1(const Env& e) {
-  dev_smooth_dynamics(e);
-  dev_equality_rows(e);
-  dev_collision(e);
-  dev_inequality_rows(e);
-}
-__device__ __forceinline__ void tile_lane_dynamics(const Env& e) { dev_smooth_dynamics(e); }
-__device__ __forceinline__ void tile_lane_solve(const Env& e) { dev_solve(e); }
-__device__ __forceinline__ void tile_wave_collide(const Env& e, char* lds) { wave_collide(e, lds); }
-__device__ __forceinline__ void tile_wave_solve(const Env& e, char* lds) { wave_solve(e, lds); }
-__device__ __forceinline__ void tile_lane_stages_2(const Env& e) {
-  dev_integrate(e);
-  dev_kinematics(e);
-}
-
 // One substep of envs [first, first + n_here) of a group; the workgroup is one wavefront. Per-env stages run one
 // lane per env; the wavefront collision and solver variants take the tile's envs one after the other.
+// kGjk: GJK/EPA runs inside the collision stage (no env stops early, no resume launch) instead of being deferred
+template <bool kGjk>
 __device__ __forceinline__ void run_tile(const gsr_model& m, const gsr_state& s, int env0, int n_here, char* lds) {
   const int lane = threadIdx.x;
   const bool mine = lane < n_here;
   const bool cw = m.collide_variant == 1, sw = m.solver_variant == 1, dv = m.dyn_variant == 1;
+  // an env with pairs deferred to GJK/EPA stops after its collision; k_gjk_resume finishes its substep
   if (!cw && !sw && !dv) {
     if (mine) {
       Env e(m, s, env0 + lane);
       dev_smooth_dynamics(e);
       dev_equality_rows(e);
-      dev_collision(e);
-      dev_inequality_rows(e);
-      dev_solve(e);
-      dev_integrate(e);
-      dev_kinematics(e);
+      dev_collision<kGjk>(e);
+      if (!gjk_pending(e)) {
+        dev_inequality_rows(e);
+        dev_solve(e);
+        dev_integrate(e);
+        dev_kinematics(e);
+      }
     }
     return;
   }
@@ -211,36 +212,61 @@ __device__ __forceinline__ void run_tile(const gsr_model& m, const gsr_state& s,
     if (!dv) dev_smooth_dynamics(e);
     if (!cw) {
       dev_equality_rows(e);
-      dev_collision(e);
-      dev_inequality_rows(e);
+      dev_collision<kGjk>(e);
+      if (!gjk_pending(e)) dev_inequality_rows(e);
     }
   }
   __syncthreads();
   for (int j = 0; j < n_here; j++) {
     Env e(m, s, env0 + j);
     if (dv) wave_step1(e);
-    if (cw) wave_collide(e, lds);
+    if (cw) wave_collide<kGjk>(e, lds);
     __syncthreads();
-    if (sw) wave_solve(e, lds);
+    if (sw && !gjk_pending(e)) wave_solve(e, lds);
     __syncthreads();
   }
   if (mine && (!sw || !dv)) {
     Env e(m, s, env0 + lane);
-    if (!sw) dev_solve(e);
-    if (!dv) {
-      dev_integrate(e);
-      dev_kinematics(e);
+    if (!gjk_pending(e)) {
+      if (!sw) dev_solve(e);
+      if (!dv) {
+        dev_integrate(e);
+        dev_kinematics(e);
+      }
     }
   }
   if (dv) {
     __syncthreads();
     for (int j = 0; j < n_here; j++) {
       Env e(m, s, env0 + j);
-      wave_step2(e);
+      if (!gjk_pending(e)) wave_step2(e);
     }
   }
 }
 
+// The rest of the substep for the envs of one scene type that the fused launch left at their collision: the GJK/EPA
+// pass and inequality rows on lane 0, then the model's solver and integration variants. One wavefront per env; envs
+// without deferred pairs return at once.
+__global__ void __launch_bounds__(kWave) k_gjk_resume(gsr_model m, gsr_state s) {
+  extern __shared__ char lds[];
+  if ((int)blockIdx.x >= s.n_envs) return;
+  const int b = xcd_env(blockIdx.x, s.n_envs);
+  Env e(m, s, b);
+  if (!gjk_pending(e)) return;  // uniform across the block
+  wave_gjk_fixup(e);
+  wave_rows(e);
+  if (m.solver_variant == 1) wave_solve(e, lds);
+  else if (threadIdx.x == 0) dev_solve(e);
+  __syncthreads();
+  if (m.dyn_variant == 1) {
+    wave_step2(e);
+  } else if (threadIdx.x == 0) {
+    dev_integrate(e);
+    dev_kinematics(e);
+  }
+}
+
+template <bool kGjk>
 __device__ __forceinline__ void fused_tile(ConstTypes* types, const BatchRange* __restrict__ R, int n_groups,
                                            int tile, char* lds) {
   // the range holding the tile: binary search over the cumulative tile ends
@@ -258,11 +284,12 @@ __device__ __forceinline__ void fused_tile(ConstTypes* types, const BatchRange* 
   // the type index is uniform, so the descriptor fields are scalar loads from the kernel-argument segment
   const int k = __builtin_amdgcn_readfirstlane(g.type);
   const FusedTypes* ty = (const FusedTypes*)types;
-  run_tile(ty->m[k], ty->s[k], env0, n_here, lds);
+  run_tile<kGjk>(ty->m[k], ty->s[k], env0, n_here, lds);
 }
 
-// W = resident wavefronts per SIMD the register allocation must allow (2: 256 VGPRs, 4: 128 VGPRs)
-template <int W>
+// W = resident wavefronts per SIMD the register allocation must allow (2: 256 VGPRs, 4: 128 VGPRs);
+// G = GJK/EPA inside the launch (1) or deferred to k_gjk_resume (0)
+template <int W, int G>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(W, 8)))
 k_fused(const char* __restrict__ table, uint32_t* queue, int persistent) {
   extern __shared__ char lds[];
@@ -271,7 +298,7 @@ k_fused(const char* __restrict__ table, uint32_t* queue, int persistent) {
   const BatchRange* __restrict__ R = reinterpret_cast<const BatchRange*>(table + kRangesOffset);
   const int n_tiles = H->n_tiles, n_groups = H->n_groups;
   if (!persistent) {
-    if ((int)blockIdx.x < n_tiles) fused_tile(types, R, n_groups, xcd_env(blockIdx.x, n_tiles), lds);
+    if ((int)blockIdx.x < n_tiles) fused_tile<G != 0>(types, R, n_groups, xcd_env(blockIdx.x, n_tiles), lds);
     return;
   }
   for (;;) {
@@ -279,7 +306,7 @@ k_fused(const char* __restrict__ table, uint32_t* queue, int persistent) {
     if (threadIdx.x == 0) t = atomicAdd(&queue[0], 1u);
     const int tile = __builtin_amdgcn_readfirstlane((int)__shfl((int)t, 0, kWave));
     if (tile >= n_tiles) break;
-    fused_tile(types, R, n_groups, tile, lds);
+    fused_tile<G != 0>(types, R, n_groups, tile, lds);
     __syncthreads();
   }
   // the last workgroup out resets the queue for the next launch (no host memset, so graph replays stay valid)
@@ -347,6 +374,7 @@ void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
                            *s);
       else
         hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, st, *m, *s);
+      hipLaunchKernelGGL(k_gjk_fixup, dim3(s->n_envs), dim3(kWave), 0, st, *m, *s);
       break;
     case 2:
       if (m->solver_variant == 1)
@@ -489,16 +517,17 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   if (hipGetDevice(&dev) != hipSuccess) return GSR_E_LAUNCH;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return GSR_E_LAUNCH;
   const int W = info->waves_per_simd == 4 ? 4 : 2;
-  const void* fn = W == 4 ? (const void*)k_fused<4> : (const void*)k_fused<2>;
+  const int G = info->gjk_inline ? 1 : 0;
+  const void* fn = W == 4 ? (G ? (const void*)k_fused<4, 1> : (const void*)k_fused<4, 0>)
+                          : (G ? (const void*)k_fused<2, 1> : (const void*)k_fused<2, 0>);
   if (lds > 0 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     return GSR_E_LAUNCH;
-  hipError_t oe = W == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused<4>, kWave, lds)
-                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused<2>, kWave, lds);
-  if (oe != hipSuccess) return GSR_E_LAUNCH;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWave, lds) != hipSuccess) return GSR_E_LAUNCH;
   out.n_groups = n_groups;
   out.n_tiles = tiles;
   out.persistent = info->persistent ? 1 : 0;
   out.waves_per_simd = W;
+  out.gjk_inline = G;
   out.grid = out.persistent ? std::max(1, std::min(tiles, n_cu * std::max(per_cu, 1))) : tiles;
   out.lds_bytes = (int32_t)lds;
   *info = out;
@@ -512,12 +541,22 @@ int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_
   const char* t = static_cast<const char*>(dev_table);
   uint32_t* queue = reinterpret_cast<uint32_t*>(const_cast<char*>(t) + 2 * sizeof(int32_t));
   for (int i = 0; i < n_substeps; i++) {
-    if (info->waves_per_simd == 4)
-      hipLaunchKernelGGL(k_fused<4>, dim3(info->grid), dim3(kWave), info->lds_bytes, stream, t, queue,
-                         (int)info->persistent);
-    else
-      hipLaunchKernelGGL(k_fused<2>, dim3(info->grid), dim3(kWave), info->lds_bytes, stream, t, queue,
-                         (int)info->persistent);
+    const dim3 grid(info->grid), block(kWave);
+    const int pers = (int)info->persistent;
+    if (info->waves_per_simd == 4) {
+      if (info->gjk_inline) hipLaunchKernelGGL((k_fused<4, 1>), grid, block, info->lds_bytes, stream, t, queue, pers);
+      else hipLaunchKernelGGL((k_fused<4, 0>), grid, block, info->lds_bytes, stream, t, queue, pers);
+    } else {
+      if (info->gjk_inline) hipLaunchKernelGGL((k_fused<2, 1>), grid, block, info->lds_bytes, stream, t, queue, pers);
+      else hipLaunchKernelGGL((k_fused<2, 0>), grid, block, info->lds_bytes, stream, t, queue, pers);
+    }
+    if (info->gjk_inline) continue;
+    for (int k = 0; k < info->n_types; k++) {
+      const gsr_model& m = info->models[k];
+      const gsr_state& st = info->states[k];
+      const size_t lds = m.solver_variant == 1 ? wave_lds_bytes(m.n_dofs, m.n_entities) : 0;
+      hipLaunchKernelGGL(k_gjk_resume, dim3(st.n_envs), dim3(kWave), lds, stream, m, st);
+    }
   }
   return launch_status();
 }

@@ -130,14 +130,31 @@ __device__ inline void dev_add_contact(const Env& e, int ga, int gb, f3 normal, 
 
 // Where the narrowphase puts the contacts of one pair. GlobalSink appends to the env's contact list at once (one
 // lane per env); LocalSink keeps them in the lane until the wavefront has counted every pair's contacts.
+//
+// The collision kernels contain no GJK/EPA code: its registers would lower the occupancy of every kernel holding the
+// narrowphase (the fused batch kernel among them). A pair that needs it is deferred instead: it contributes no
+// contact, its normal cache is left as it was, and (pair, index of the next contact) is recorded in the env's
+// deferral list. The GJK/EPA pass (wave_gjk_fixup) then recomputes each deferred pair and inserts its contacts at the
+// recorded index, which puts every contact where the one-pass narrowphase puts it.
+// kGjk: run GJK/EPA in place (the lane owns its env's workspace), instead of deferring the pair
+template <bool kGjk>
 struct GlobalSink {
   const Env& e;
+  int start;  // contacts stored before this pair
+  __device__ explicit GlobalSink(const Env& e_) : e(e_), start(e_.I(e_.s.n_contacts, 0)) {}
   __device__ void add(int ga, int gb, f3 n, f3 p, float pen) { dev_add_contact(e, ga, gb, n, p, pen); }
   // position of the k-th most recent contact (k = 0: the last one)
   __device__ f3 recent_pos(int k) const { return e.L3(e.s.contact_pos, e.I(e.s.n_contacts, 0) - 1 - k); }
   __device__ void flag(uint32_t bits) { e.I((int32_t*)e.s.errno_env, 0) |= (int)bits; }
-  __device__ static constexpr bool gjk_allowed() { return true; }
-  __device__ void defer() {}
+  __device__ static constexpr bool gjk_allowed() { return kGjk; }
+  // the pair's contacts so far (its unperturbed detection) are withdrawn: the GJK/EPA pass recomputes all of them
+  __device__ void defer(int ga, int gb) {
+    e.I(e.s.n_contacts, 0) = start;
+    int n = e.I(e.s.gjk_defer_n, 0);
+    e.I(e.s.gjk_defer, 2 * n) = (ga << 16) | gb;
+    e.I(e.s.gjk_defer, 2 * n + 1) = start;
+    e.I(e.s.gjk_defer_n, 0) = n + 1;
+  }
 };
 
 constexpr int kMaxPairContacts = 8;
@@ -166,7 +183,7 @@ struct LocalSink {
   __device__ f3 recent_pos(int k) const { return c.p[c.cnt - 1 - k]; }
   __device__ void flag(uint32_t bits) { c.err |= bits; }
   __device__ static constexpr bool gjk_allowed() { return kGjk; }
-  __device__ void defer() { c.deferred = true; }
+  __device__ void defer(int, int) { c.deferred = true; }
 };
 
 __device__ inline float dev_tolerance(const gsr_model& m, int ga, int gb, float tol) {
@@ -438,7 +455,7 @@ __device__ inline void dev_convex_convex(const Env& e, int ga, int gb, Sink& snk
           bool prefer_gjk = !guess || (m.mc_tolerance * pen >= m.mpr_to_gjk * tol);
           if (prefer_gjk) {
             if (!snk.gjk_allowed()) {
-              snk.defer();
+              snk.defer(ga, gb);
               return;
             }
             snk.flag(GSR_ERR_DEEP_PENETRATION);
@@ -526,6 +543,7 @@ __device__ inline void dev_plane_box(const Env& e, int ga, int gb, Sink& snk) {
   }
 }
 
+template <bool kGjk>
 __device__ inline void dev_narrow_phase(const Env& e) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
@@ -534,26 +552,28 @@ __device__ inline void dev_narrow_phase(const Env& e) {
     int p = e.I(s.broad_pairs, ip);
     int ga = p >> 16, gb = p & 0xffff;
     if (m.geom_type[ga] > m.geom_type[gb]) { int t = ga; ga = gb; gb = t; }
-    GlobalSink snk{e};
+    GlobalSink<kGjk> snk(e);
     if (!(m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX)) dev_convex_convex(e, ga, gb, snk);
   }
   for (int ip = 0; ip < nb; ip++) {
     int p = e.I(s.broad_pairs, ip);
     int ga = p >> 16, gb = p & 0xffff;
     if (m.geom_type[ga] > m.geom_type[gb]) { int t = ga; ga = gb; gb = t; }
-    GlobalSink snk{e};
+    GlobalSink<kGjk> snk(e);
     if (m.geom_type[ga] == GEOM_PLANE && m.geom_type[gb] == GEOM_BOX) dev_plane_box(e, ga, gb, snk);
   }
 }
 
+template <bool kGjk = false>
 __device__ inline void dev_collision(const Env& e) {
   e.I(e.s.n_contacts, 0) = 0;
   e.I(e.s.n_broad, 0) = 0;
+  e.I(e.s.gjk_defer_n, 0) = 0;
   if (!e.m.enable_collision) return;
   dev_update_aabbs(e);
   if (e.m.n_pairs > 0) {
     dev_broad_phase(e);
-    dev_narrow_phase(e);
+    dev_narrow_phase<kGjk>(e);
   }
 }
 

@@ -141,8 +141,11 @@ __device__ inline void wave_broad_phase(const Env& e, char* lds, int lane) {
 }
 
 // one pass over the broadphase pairs of one kind (convex pairs, then plane-box pairs), contacts appended at `base`
+// kGjk: pairs that need GJK/EPA are redone in place, one lane at a time (they share the env's workspace), instead of
+// being deferred to the GJK/EPA pass
+template <bool kGjk>
 __device__ inline int wave_narrow_pass(const Env& e, int lane, int nb, bool plane_box_pass, int base,
-                                       uint32_t& err) {
+                                       uint32_t& err, int& n_def) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   for (int p0 = 0; p0 < nb; p0 += 64) {
@@ -165,21 +168,34 @@ __device__ inline int wave_narrow_pass(const Env& e, int lane, int nb, bool plan
         else dev_convex_convex(e, ga, gb, snk);
       }
     }
-    // pairs that need GJK/EPA share the env's workspace: redo them one lane at a time, with the workspace
-    unsigned long long def = __ballot(pc.deferred);
-    while (def) {
-      int j = __ffsll((long long)def) - 1;
-      def &= def - 1;
-      if (lane == j) {
-        pc.cnt = 0;
-        pc.err = 0;
-        pc.deferred = false;
-        LocalSink<true> snk{pc};
-        dev_convex_convex(e, ga, gb, snk);
+    if (kGjk) {
+      unsigned long long redo = __ballot(pc.deferred);
+      while (redo) {
+        int j = __ffsll((long long)redo) - 1;
+        redo &= redo - 1;
+        if (lane == j) {
+          pc.cnt = 0;
+          pc.err = 0;
+          pc.deferred = false;
+          LocalSink<true> snk{pc};
+          dev_convex_convex(e, ga, gb, snk);
+        }
       }
     }
+    // a pair that needs GJK/EPA contributes no contact here; it is recorded with the index its contacts go to
+    if (pc.deferred) {
+      pc.cnt = 0;
+      pc.err = 0;
+    }
+    const unsigned long long def = __ballot(pc.deferred);
     int total;
     int k = base + wave_excl_scan(pc.cnt, lane, total);
+    if (pc.deferred) {
+      const int slot = n_def + __popcll(def & ((1ull << lane) - 1ull));
+      e.I(s.gjk_defer, 2 * slot) = (ga << 16) | gb;
+      e.I(s.gjk_defer, 2 * slot + 1) = min(k, m.max_contacts);
+    }
+    n_def += __popcll(def);
     float fr = fmaxf(fmaxf(m.geom_friction[ga] * 1.f, m.geom_friction[gb] * 1.f), 1e-2f);
     for (int c = 0; c < pc.cnt; c++, k++) {
       if (k < m.max_contacts) {
@@ -199,41 +215,29 @@ __device__ inline int wave_narrow_pass(const Env& e, int lane, int nb, bool plan
   return base;
 }
 
+template <bool kGjk>
 __device__ inline void wave_narrow_phase(const Env& e, int lane) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const int nb = e.I(s.n_broad, 0);
   uint32_t err = 0;
-  int n = wave_narrow_pass(e, lane, nb, false, 0, err);
-  n = wave_narrow_pass(e, lane, nb, true, n, err);
+  int n_def = 0;
+  int n = wave_narrow_pass<kGjk>(e, lane, nb, false, 0, err, n_def);
+  n = wave_narrow_pass<kGjk>(e, lane, nb, true, n, err, n_def);
   for (int o = 32; o > 0; o >>= 1) err |= (uint32_t)__shfl_xor((int)err, o, 64);
   if (lane == 0) {
     e.I(s.n_contacts, 0) = min(n, m.max_contacts);
+    e.I(s.gjk_defer_n, 0) = n_def;
     if (err) e.I((int32_t*)s.errno_env, 0) |= (int)err;
   }
   __syncthreads();
 }
 
-// equality rows, collision detection and inequality rows of one env (k_collide's work) on one wavefront
-__device__ __forceinline__ void wave_collide(const Env& e, char* lds) {
+// frictionloss rows on lane 0, the 4 rows of each contact on its own lane, limit rows on lane 0 after them
+__device__ __forceinline__ void wave_rows(const Env& e) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
   const int lane = threadIdx.x;
-  if (lane == 0) {
-    dev_equality_rows(e);
-    e.I(s.n_contacts, 0) = 0;
-    e.I(s.n_broad, 0) = 0;
-  }
-  if (m.enable_collision) {
-    wave_update_aabbs(e, lane);
-    __syncthreads();
-    if (m.n_pairs > 0) {
-      wave_broad_phase(e, lds, lane);
-      wave_narrow_phase(e, lane);
-    }
-  }
-  __syncthreads();
-  // rows: frictionloss on lane 0, contacts one per lane, limits on lane 0 after them
   int n_con = 0, n_fl = 0;
   if (lane == 0) n_con = dev_frictionloss_rows(e, e.I(s.n_con, 0), n_fl);
   n_con = __shfl(n_con, 0, 64);
@@ -248,6 +252,108 @@ __device__ __forceinline__ void wave_collide(const Env& e, char* lds) {
     e.I(s.n_con, 2) = n_fl;
   }
   __syncthreads();
+}
+
+// equality rows, collision detection and inequality rows of one env (k_collide's work) on one wavefront
+template <bool kGjk = false>
+__device__ __forceinline__ void wave_collide(const Env& e, char* lds) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const int lane = threadIdx.x;
+  if (lane == 0) {
+    dev_equality_rows(e);
+    e.I(s.n_contacts, 0) = 0;
+    e.I(s.n_broad, 0) = 0;
+    e.I(s.gjk_defer_n, 0) = 0;
+  }
+  if (m.enable_collision) {
+    wave_update_aabbs(e, lane);
+    __syncthreads();
+    if (m.n_pairs > 0) {
+      wave_broad_phase(e, lds, lane);
+      wave_narrow_phase<kGjk>(e, lane);
+    }
+  }
+  __syncthreads();
+  // with deferred pairs the contact list is not final yet: the GJK/EPA pass builds the rows after inserting them
+  if (e.I(s.gjk_defer_n, 0) > 0) return;
+  wave_rows(e);
+}
+
+// The GJK/EPA pass of one env on one wavefront: lane 0 recomputes the deferred pairs with GJK/EPA (they share the
+// env's workspace), last first; the contacts above each pair's insertion index move up across the lanes (64 at a
+// time from the top: every lane reads before any writes), and lane 0 writes the pair's contacts in the gap.
+// Contacts pushed past max_contacts are dropped with the overflow bit, as the one-pass narrowphase drops them.
+// Returns whether the env had deferred pairs (uniform).
+__device__ inline bool wave_gjk_fixup(const Env& e) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const int lane = threadIdx.x;
+  const int nd = e.I(s.gjk_defer_n, 0);
+  if (nd == 0) return false;
+  __syncthreads();
+  for (int j = nd - 1; j >= 0; j--) {
+    const int pr = e.I(s.gjk_defer, 2 * j), at = e.I(s.gjk_defer, 2 * j + 1);
+    const int ga = pr >> 16, gb = pr & 0xffff;
+    PairContacts pc;
+    pc.cnt = 0;
+    pc.err = 0;
+    pc.deferred = false;
+    if (lane == 0) {
+      LocalSink<true> snk{pc};
+      dev_convex_convex(e, ga, gb, snk);
+      if (pc.err) e.I((int32_t*)s.errno_env, 0) |= (int)pc.err;
+    }
+    const int cnt = __shfl(pc.cnt, 0, 64);
+    if (cnt == 0) continue;
+    const int n = e.I(s.n_contacts, 0);
+    const int kept = min(n + cnt, m.max_contacts);
+    for (int top = n; top > at; top -= 64) {
+      const int i = top - 1 - lane;
+      const bool act = i >= at;
+      int g0 = 0, g1 = 0;
+      f3 cp = zero3(), cn = zero3();
+      float pe = 0.f, fr = 0.f;
+      if (act) {
+        g0 = e.I(s.contact_geoms, 2 * i);
+        g1 = e.I(s.contact_geoms, 2 * i + 1);
+        cp = e.L3(s.contact_pos, i);
+        cn = e.L3(s.contact_normal, i);
+        pe = e.F(s.contact_pen, i);
+        fr = e.F(s.contact_friction, i);
+      }
+      __syncthreads();
+      const int t = i + cnt;
+      if (act && t < kept) {
+        e.I(s.contact_geoms, 2 * t) = g0;
+        e.I(s.contact_geoms, 2 * t + 1) = g1;
+        e.S3(s.contact_pos, t, cp);
+        e.S3(s.contact_normal, t, cn);
+        e.F(s.contact_pen, t) = pe;
+        e.F(s.contact_friction, t) = fr;
+      }
+      __syncthreads();
+    }
+    if (lane == 0) {
+      if (n + cnt > m.max_contacts) e.I((int32_t*)s.errno_env, 0) |= (int)GSR_ERR_CONTACT_OVERFLOW;
+      const float fr = fmaxf(fmaxf(m.geom_friction[ga] * 1.f, m.geom_friction[gb] * 1.f), 1e-2f);
+      for (int c = 0; c < cnt; c++) {
+        const int t = at + c;
+        if (t >= kept) break;
+        e.I(s.contact_geoms, 2 * t) = ga;
+        e.I(s.contact_geoms, 2 * t + 1) = gb;
+        e.S3(s.contact_pos, t, pc.p[c]);
+        e.S3(s.contact_normal, t, pc.n[c]);
+        e.F(s.contact_pen, t) = pc.pen[c];
+        e.F(s.contact_friction, t) = fr;
+      }
+      e.I(s.n_contacts, 0) = kept;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) e.I(s.gjk_defer_n, 0) = 0;
+  __syncthreads();
+  return true;
 }
 
 }  // namespace gsr

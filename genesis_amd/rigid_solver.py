@@ -50,7 +50,7 @@ MODEL_TABLES_F = [
 
 INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", "n_broad", "broad_pairs",
              "n_contacts", "contact_geoms", "active_buffer", "n_con", "jac_idx", "jac_nnz", "efc_active",
-             "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent"}
+             "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_defer_n", "gjk_defer"}
 
 # Entity counts above this cannot use the wavefront solver's pattern bitsets.
 WAVE_MAX_ENTITIES = 128
@@ -82,7 +82,7 @@ def state_sizes(M: RigidModel) -> dict:
         efc_force=R, efc_Jaref=R, efc_jv=R, efc_quad=3 * R, efc_active=R,
         qacc=D, Ma=D, grad=D, Mgrad=D, search=D, mv=D, qfrc_constraint=D, nt_vec=D, nt_H=D * (D + 1) // 2,
         solver_iters=1, qpos_next=Q, vel_next=D, ent_row_ptr=M.n_entities + 1, ent_row_list=2 * R, efc_ent=R,
-        gjk_ws=_lib.load().gsr_gjk_workspace_words(),
+        gjk_ws=_lib.load().gsr_gjk_workspace_words(), gjk_defer_n=1, gjk_defer=2 * max(M.max_broad, 1),
     )
     missing = set(_lib.STATE_POINTERS) - set(sz)
     assert not missing, f"state arrays without a size rule: {missing}"

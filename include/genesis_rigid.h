@@ -119,6 +119,9 @@ typedef struct gsr_state {
   int32_t *ent_row_ptr, *ent_row_list; /* per-entity ordered constraint-row lists (wavefront solver) */
   int32_t *efc_ent;                    /* entity pair of each row (wavefront solver) */
   float *gjk_ws;                       /* GJK/EPA polytope workspace, gsr_gjk_workspace_words() per env, env-major */
+  /* pairs the narrowphase handed to the GJK/EPA pass of this substep: count, then (ga << 16 | gb, contact index the
+     pair's contacts are inserted at) per pair, max_broad pairs per env */
+  int32_t *gjk_defer_n, *gjk_defer;
   /* integration scratch */
   float *qpos_next, *vel_next;
 } gsr_state;
@@ -172,7 +175,7 @@ typedef struct gsr_batch_info {
   int32_t n_types;           /* distinct (model, state) pairs, in first-appearance order */
   int32_t persistent;        /* input: 1 = CUs x resident workgroups pull tiles from a queue, 0 = one per tile */
   int32_t waves_per_simd;    /* input: register budget of the fused kernel, 2 (256 VGPRs) or 4 (128 VGPRs) */
-  int32_t pad;
+  int32_t gjk_inline;        /* input: 1 = GJK/EPA inside the fused kernel, 0 = deferred to a per-type resume launch */
   gsr_model models[GSR_MAX_TYPES];
   gsr_state states[GSR_MAX_TYPES];
 } gsr_batch_info;

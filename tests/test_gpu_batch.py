@@ -37,12 +37,36 @@ def _assert_same(a, b, what):
 COUNTS = [("arm", 70), ("quadruped", 90), ("pile20", 12)]
 
 
-@pytest.mark.parametrize("tiles,launch", [((64, 64, 1), dict(persistent=True, waves_per_simd=4)),
-                                          ((32, 64, 3), dict(persistent=False, waves_per_simd=4)),
-                                          ((64, 16, 4), dict(persistent=True, waves_per_simd=2))])
-def test_fused_equals_plain_bitwise(tiles, launch):
+def _overlap_pile(parts, seed=11):
+    """Random overlapping box poses for the pile part: MPR hands many pairs to GJK/EPA every step."""
+    for w, M, S, _ in parts:
+        if not w.name.startswith("pile"):
+            continue
+        rng = np.random.default_rng(seed)
+        B, n = S._B, M.n_qs // 7
+        q = np.zeros((B, n, 7))
+        q[:, :, 0:2] = rng.uniform(-0.25, 0.25, (B, n, 2))
+        q[:, :, 2] = rng.uniform(0.1, 0.6, (B, n))
+        qq = rng.normal(size=(B, n, 4))
+        q[:, :, 3:7] = qq / np.linalg.norm(qq, axis=-1, keepdims=True)
+        S.set_qpos(torch.as_tensor(q.reshape(B, -1), dtype=torch.float32))
+
+
+@pytest.mark.parametrize("tiles,launch,deep", [
+    ((64, 64, 1), dict(persistent=True, waves_per_simd=4), False),
+    ((32, 64, 3), dict(persistent=False, waves_per_simd=4), False),
+    ((64, 16, 4), dict(persistent=True, waves_per_simd=2), False),
+    ((64, 64, 1), dict(persistent=False, waves_per_simd=2, gjk_inline=False), True),
+    ((32, 64, 2), dict(persistent=True, waves_per_simd=4, gjk_inline=True), True),
+])
+def test_fused_equals_plain_bitwise(tiles, launch, deep):
+    """deep: overlapping boxes in the pile part, so the GJK/EPA pass runs (deferred to the resume launch, or inside
+    the fused kernel) while the plain step runs it in its own pass."""
     from genesis_amd.batch import FusedBatch, Group
     A, B = _parts(COUNTS), _parts(COUNTS)
+    if deep:
+        _overlap_pile(A)
+        _overlap_pile(B)
     # plain variants: lane solver for arm/quadruped, wavefront solver for the pile
     fb = FusedBatch([Group(S, envs_per_tile=t) for (_, _, S, _), t in zip(B, tiles)][::-1], **launch)
     for step in range(12):
@@ -54,6 +78,10 @@ def test_fused_equals_plain_bitwise(tiles, launch):
         fb.step()
         torch.cuda.synchronize()
         _assert_same(_state(A), _state(B), f"step {step} tiles {tiles} {launch}")
+    if deep:
+        for parts in (A, B):
+            e = parts[2][2].st.t["errno_env"][0]
+            assert bool(((e & 0x100) != 0).all()), "the GJK/EPA pass did not run in every pile env"
     for _, _, S, _ in B:
         S.check_errno()
 
