@@ -590,51 +590,12 @@ __device__ __forceinline__ int wave_col_candidate(const WaveCtx& w, int i, int e
   return -1;
 }
 
-// f(j) for every entry j of column i (entity ei) below the diagonal, in the order of wave_col_candidate: the rest of
-// ei's own dofs, then the dofs of the pattern entities after ei
-template <class F>
-__device__ __forceinline__ void wave_for_col(const WaveLds& L, int i, int ei, F f) {
-  for (int j = i + 1; j < L.ee[ei]; j++) f(j);
-  for (int word = (ei + 1) >> 6; word < 2; word++) {
-    unsigned long long bits = L.pat[2 * ei + word];
-    int lo = ei + 1 - word * 64;
-    if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
-    while (bits) {
-      int ej = word * 64 + __ffsll((long long)bits) - 1;
-      bits &= bits - 1;
-      for (int j = L.es[ej]; j < L.ee[ej]; j++) f(j);
-    }
-  }
-}
-
-// sum_{k < kmax, block(ent(a), ent(k)) and block(ent(b), ent(k)) in pattern} H[a][k] * H[b][k], k ascending
-template <class HS>
-__device__ __forceinline__ float wave_row_dot(const WaveCtx& w, const HS& h, int a, int bb, int kmax) {
-  const WaveLds& L = w.L;
-  int ea = L.ent_of_dof[a], eb = L.ent_of_dof[bb];
-  unsigned long long m0 = L.pat[2 * ea] & L.pat[2 * eb], m1 = L.pat[2 * ea + 1] & L.pat[2 * eb + 1];
-  float acc = 0.f;
-  for (int word = 0; word < 2; word++) {
-    unsigned long long t = word ? m1 : m0;
-    while (t) {
-      int ek = word * 64 + __ffsll((long long)t) - 1;
-      t &= t - 1;
-      int k0 = L.es[ek], k1 = L.ee[ek];
-      if (k0 >= kmax) return acc;
-      if (k1 > kmax) k1 = kmax;
-      const auto pa = h.blk(a, ea, ek);
-      const auto pb = h.blk(bb, eb, ek);
-      for (int k = 0; k < k1 - k0; k++) acc += pa[k] * pb[k];
-    }
-  }
-  return acc;
-}
-
 // The n entities of one level are independent; the wavefront is cut into groups of G lanes (the largest power of two
 // with G * n <= 64, at least 1), group q taking entities q, q + 64 / G, ... of the level. Inside a group, the terms
 // of every dot product are dealt round-robin to its lanes and summed by a butterfly, so all lanes of the group hold
 // the same value; each entry is written by the group's lane 0. A wavefront's LDS accesses complete in order, so an
-// entry written by one lane is seen by the reads any lane issues after it.
+// entry written by one lane is seen by the reads any lane issues after it. Terms are dealt by a running term index
+// t0 (lane r takes the terms t with t = r mod G).
 struct LaneGroup {
   int G, q, r, ng;
 };
@@ -648,52 +609,19 @@ __device__ __forceinline__ float group_sum(float v, int G) {
   return v;
 }
 
-// this lane's share (terms t with t = r mod G) of sum_{k < i in row i's pattern} a(k) * b(k): pattern entities in
-// ascending order, dofs ascending; a and b are given the entity and the dof
-template <class F>
-__device__ __forceinline__ float row_terms(const WaveLds& L, int i, int ei, int G, int r, F term) {
-  float acc = 0.f;
-  int t0 = 0;
-  for (int word = 0; word < 2; word++) {
-    unsigned long long t = L.pat[2 * ei + word];
-    while (t) {
-      int ek = word * 64 + __ffsll((long long)t) - 1;
-      t &= t - 1;
-      int k0 = L.es[ek], k1 = L.ee[ek];
-      if (k0 >= i) return acc;
-      if (k1 > i) k1 = i;
-      for (int j = k0 + ((r - t0) & (G - 1)); j < k1; j += G) acc += term(ek, k0, j);
-      t0 += k1 - k0;
-    }
-  }
-  return acc;
-}
+// Left-looking Cholesky over the pattern (func_nt_chol_factor's entries), level by level, lane groups per entity,
+// kChunk dofs of an entity at a time:
+//   1. the chunk's diagonal block less the products of its rows over the earlier columns (terms dealt round-robin
+//      to the group, kChunk (kChunk + 1) / 2 sums reduced together);
+//   2. the small Cholesky of that block, by every lane of the group alike (registers);
+//   3. every row j below the chunk (later own dofs, then the pattern entities after it), one per lane of the group:
+//      its kChunk entries less the products over the earlier columns the two rows share, then a forward solve with
+//      the chunk's factor.
+// Each row's earlier columns are read once for the whole chunk instead of once per column.
+constexpr int kChunk = 6;  // dofs of an entity the factorisation and substitutions take at a time
+constexpr int kChunkTri = kChunk * (kChunk + 1) / 2;
+__device__ __forceinline__ constexpr int tri(int r, int q) { return r * (r + 1) / 2 + q; }
 
-// f(j) for this lane's share (positions t = r mod G) of the entries of column i below the diagonal, in the order of
-// wave_col_candidate
-template <class F>
-__device__ __forceinline__ void col_terms(const WaveLds& L, int i, int ei, int G, int r, F f) {
-  int t0 = 0;
-  {
-    const int k1 = L.ee[ei];
-    for (int j = i + 1 + (r & (G - 1)); j < k1; j += G) f(j);
-    t0 = k1 - i - 1;
-  }
-  for (int word = (ei + 1) >> 6; word < 2; word++) {
-    unsigned long long bits = L.pat[2 * ei + word];
-    int lo = ei + 1 - word * 64;
-    if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
-    while (bits) {
-      int ej = word * 64 + __ffsll((long long)bits) - 1;
-      bits &= bits - 1;
-      const int k0 = L.es[ej], k1 = L.ee[ej];
-      for (int j = k0 + ((r - t0) & (G - 1)); j < k1; j += G) f(j);
-      t0 += k1 - k0;
-    }
-  }
-}
-
-// Left-looking Cholesky over the pattern (func_nt_chol_factor's entries, level by level, lane groups per entity)
 template <class HS>
 __device__ __forceinline__ void wave_chol_factor(const WaveCtx& w, const HS& h) {
   const float EPS = w.e.m.eps;
@@ -703,23 +631,117 @@ __device__ __forceinline__ void wave_chol_factor(const WaveCtx& w, const HS& h) 
     const int i0 = L.lvl_start[lv], i1 = L.lvl_start[lv + 1];
     const LaneGroup g = lane_group(i1 - i0, w.lane);
     for (int idx = i0 + g.q; idx < i1; idx += g.ng) {
-      int ei = L.lvl_order[idx];
-      int s0 = L.es[ei], s1 = L.ee[ei];
-      for (int i = s0; i < s1; i++) {
-        float sq = row_terms(L, i, ei, g.G, g.r, [&](int ek, int k0, int j) {
-          float v = h.blk(i, ei, ek)[j - k0];
-          return v * v;
-        });
-        sq = group_sum(sq, g.G);
-        auto di = h.blk(i, ei, ei);
-        float hii = sqrtf(fmaxf(di[i - s0] - sq, EPS));
-        float inv = 1.f / hii;
-        col_terms(L, i, ei, g.G, g.r, [&](int j) {
-          float dt = wave_row_dot(w, h, j, i, i);
-          auto pj = h.blk(j, L.ent_of_dof[j], ei) + (i - s0);
-          *pj = (*pj - dt) * inv;
-        });
-        if (g.r == 0) di[i - s0] = hii;
+      const int ei = L.lvl_order[idx];
+      const int es = L.es[ei], ee = L.ee[ei];
+      const int od = h.off(ei, ei);
+      for (int c0 = es; c0 < ee; c0 += kChunk) {
+        const int nc = min(kChunk, ee - c0), c1 = c0 + nc;
+        typename HS::ptr rp[kChunk];
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) rp[r] = h.row(r < nc ? c0 + r : c0, ei);
+        // 1. products of the chunk's rows over the columns before c0
+        float dacc[kChunkTri];
+#pragma unroll
+        for (int t = 0; t < kChunkTri; t++) dacc[t] = 0.f;
+        int t0 = 0;
+        for (int word = 0; word < 2; word++) {
+          unsigned long long bits = L.pat[2 * ei + word];
+          while (bits) {
+            const int ek = word * 64 + __ffsll((long long)bits) - 1;
+            bits &= bits - 1;
+            const int k0 = L.es[ek];
+            if (k0 >= c0) break;
+            const int k1 = min(L.ee[ek], c0);
+            const int off = h.off(ei, ek);
+            for (int k = k0 + ((g.r - t0) & (g.G - 1)); k < k1; k += g.G) {
+              float v[kChunk];
+#pragma unroll
+              for (int r = 0; r < kChunk; r++) v[r] = r < nc ? rp[r][off + k] : 0.f;
+#pragma unroll
+              for (int r = 0; r < kChunk; r++)
+#pragma unroll
+                for (int q = 0; q <= r; q++) dacc[tri(r, q)] += v[r] * v[q];
+            }
+            t0 += k1 - k0;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kChunkTri; t++) dacc[t] = group_sum(dacc[t], g.G);
+        // 2. the chunk's own factor
+        float lc[kChunkTri];
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) {
+#pragma unroll
+          for (int q = 0; q <= r; q++) {
+            float v = 0.f;
+            if (r < nc) {
+              v = rp[r][od + c0 + q] - dacc[tri(r, q)];
+#pragma unroll
+              for (int p2 = 0; p2 < q; p2++) v -= lc[tri(r, p2)] * lc[tri(q, p2)];
+              v = (q == r) ? sqrtf(fmaxf(v, EPS)) : v / lc[tri(q, q)];
+            }
+            lc[tri(r, q)] = v;
+          }
+        }
+        // 3. the rows below the chunk, one per lane of the group
+        int u0 = 0;
+        auto below_row = [&](int j, int ej) {
+          const auto pj = h.row(j, ej);
+          float acc[kChunk];
+#pragma unroll
+          for (int q = 0; q < kChunk; q++) acc[q] = 0.f;
+          const unsigned long long m0 = L.pat[2 * ej] & L.pat[2 * ei], m1 = L.pat[2 * ej + 1] & L.pat[2 * ei + 1];
+          for (int word = 0; word < 2; word++) {
+            unsigned long long t = word ? m1 : m0;
+            while (t) {
+              const int ek = word * 64 + __ffsll((long long)t) - 1;
+              t &= t - 1;
+              const int k0 = L.es[ek];
+              if (k0 >= c0) break;
+              const int k1 = min(L.ee[ek], c0);
+              const int offj = h.off(ej, ek), offi = h.off(ei, ek);
+              for (int k = k0; k < k1; k++) {
+                const float lj = pj[offj + k];
+#pragma unroll
+                for (int q = 0; q < kChunk; q++)
+                  if (q < nc) acc[q] += lj * rp[q][offi + k];
+              }
+            }
+          }
+          const int oji = h.off(ej, ei) + c0;
+          float y[kChunk];
+#pragma unroll
+          for (int q = 0; q < kChunk; q++) {
+            if (q < nc) {
+              float v = pj[oji + q] - acc[q];
+#pragma unroll
+              for (int p2 = 0; p2 < q; p2++) v -= y[p2] * lc[tri(q, p2)];
+              y[q] = v / lc[tri(q, q)];
+              pj[oji + q] = y[q];
+            }
+          }
+        };
+        for (int j = c1 + (g.r & (g.G - 1)); j < ee; j += g.G) below_row(j, ei);
+        u0 = ee - c1;
+        for (int word = (ei + 1) >> 6; word < 2; word++) {
+          unsigned long long bits = L.pat[2 * ei + word];
+          const int lo = ei + 1 - word * 64;
+          if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
+          while (bits) {
+            const int ej = word * 64 + __ffsll((long long)bits) - 1;
+            bits &= bits - 1;
+            const int k0 = L.es[ej], k1 = L.ee[ej];
+            for (int j = k0 + ((g.r - u0) & (g.G - 1)); j < k1; j += g.G) below_row(j, ej);
+            u0 += k1 - k0;
+          }
+        }
+        if (g.r == 0) {
+#pragma unroll
+          for (int r = 0; r < kChunk; r++)
+#pragma unroll
+            for (int q = 0; q <= r; q++)
+              if (r < nc) rp[r][od + c0 + q] = lc[tri(r, q)];
+        }
         if (HS::kFence) __threadfence_block();
       }
     }
@@ -918,8 +940,11 @@ __device__ __forceinline__ bool wave_rank1(const WaveCtx& w, const HS& h, int c,
 }
 
 // x = H^-1 x in place, x in factor space, through the factor (func_nt_chol_solve): forward over forward levels,
-// backward over backward levels, lane groups per entity. x[i] still holds its right-hand side when it is read, and
-// every x[j] subtracted from it is final (an earlier level, or an earlier dof of the same entity).
+// backward over backward levels, lane groups per entity. An entity's dofs are taken kChunk at a time: the terms
+// from outside the chunk (earlier entities and dofs going forward, later ones going back) are shared by every row of
+// the chunk, so the group deals them out once and reduces kChunk sums together; the small triangle inside the chunk
+// is then solved by every lane of the group alike. x[i] still holds its right-hand side when it is read, and every
+// x[j] subtracted from it is final.
 template <class HS>
 __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h, lds_f* x) {
   const WaveLds& L = w.L;
@@ -929,11 +954,54 @@ __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h, l
     const int i0 = L.lvl_start[lv], i1 = L.lvl_start[lv + 1];
     const LaneGroup g = lane_group(i1 - i0, w.lane);
     for (int idx = i0 + g.q; idx < i1; idx += g.ng) {
-      int ei = L.lvl_order[idx];
-      for (int i = L.es[ei]; i < L.ee[ei]; i++) {
-        float acc = row_terms(L, i, ei, g.G, g.r, [&](int ek, int k0, int j) { return h.blk(i, ei, ek)[j - k0] * x[j]; });
-        float v = (x[i] - group_sum(acc, g.G)) / h.blk(i, ei, ei)[i - L.es[ei]];
-        if (g.r == 0) x[i] = v;
+      const int ei = L.lvl_order[idx];
+      const int es = L.es[ei], ee = L.ee[ei];
+      for (int c0 = es; c0 < ee; c0 += kChunk) {
+        const int nc = min(kChunk, ee - c0);
+        typename HS::ptr rp[kChunk];
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) rp[r] = h.row(r < nc ? c0 + r : c0, ei);
+        // sum_{k < c0 in the row pattern} L[c0 + r][k] x[k] for every row r of the chunk
+        float acc[kChunk];
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) acc[r] = 0.f;
+        int t0 = 0;
+        for (int word = 0; word < 2; word++) {
+          unsigned long long t = L.pat[2 * ei + word];
+          while (t) {
+            const int ek = word * 64 + __ffsll((long long)t) - 1;
+            t &= t - 1;
+            const int k0 = L.es[ek];
+            if (k0 >= c0) break;
+            const int k1 = min(L.ee[ek], c0);
+            const int off = h.off(ei, ek);
+            for (int j = k0 + ((g.r - t0) & (g.G - 1)); j < k1; j += g.G) {
+              const float xj = x[j];
+#pragma unroll
+              for (int r = 0; r < kChunk; r++) acc[r] += rp[r][off + j] * xj;
+            }
+            t0 += k1 - k0;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) acc[r] = group_sum(acc[r], g.G);
+        // the chunk's own triangle
+        const int od = h.off(ei, ei);
+        float y[kChunk];
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) {
+          if (r < nc) {
+            float v = x[c0 + r] - acc[r];
+#pragma unroll
+            for (int q = 0; q < r; q++) v -= rp[r][od + c0 + q] * y[q];
+            y[r] = v / rp[r][od + c0 + r];
+          }
+        }
+        if (g.r == 0) {
+#pragma unroll
+          for (int r = 0; r < kChunk; r++)
+            if (r < nc) x[c0 + r] = y[r];
+        }
       }
     }
     __syncthreads();
@@ -943,13 +1011,67 @@ __device__ __forceinline__ void wave_chol_solve(const WaveCtx& w, const HS& h, l
     const int i0 = L.rlvl_start[lv], i1 = L.rlvl_start[lv + 1];
     const LaneGroup g = lane_group(i1 - i0, w.lane);
     for (int idx = i0 + g.q; idx < i1; idx += g.ng) {
-      int ei = L.rlvl_order[idx];
-      for (int i = L.ee[ei] - 1; i >= L.es[ei]; i--) {
-        const int r = i - L.es[ei];
-        float acc = 0.f;
-        col_terms(L, i, ei, g.G, g.r, [&](int j) { acc += h.blk(j, L.ent_of_dof[j], ei)[r] * x[j]; });
-        float v = (x[i] - group_sum(acc, g.G)) / h.blk(i, ei, ei)[r];
-        if (g.r == 0) x[i] = v;
+      const int ei = L.rlvl_order[idx];
+      const int es = L.es[ei], ee = L.ee[ei];
+      // chunks from the last one down; the last chunk holds the remainder
+      const int n_ch = (ee - es + kChunk - 1) / kChunk;
+      for (int ch = n_ch - 1; ch >= 0; ch--) {
+        const int c0 = es + ch * kChunk, nc = min(kChunk, ee - c0), c1 = c0 + nc;
+        // sum_{j >= c1 in column c0 + r's pattern} L[j][c0 + r] x[j] for every column r of the chunk: later own
+        // dofs, then the dofs of the pattern entities after ei
+        float acc[kChunk];
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) acc[r] = 0.f;
+        int t0 = 0;
+        {
+          const int off = h.off(ei, ei);
+          for (int j = c1 + (g.r & (g.G - 1)); j < ee; j += g.G) {
+            const float xj = x[j];
+            const auto pj = h.row(j, ei) + off + c0;
+#pragma unroll
+            for (int r = 0; r < kChunk; r++)
+              if (r < nc) acc[r] += pj[r] * xj;
+          }
+          t0 = ee - c1;
+        }
+        for (int word = (ei + 1) >> 6; word < 2; word++) {
+          unsigned long long bits = L.pat[2 * ei + word];
+          const int lo = ei + 1 - word * 64;
+          if (lo > 0) bits &= (lo >= 64) ? 0ull : (~0ull << lo);
+          while (bits) {
+            const int ej = word * 64 + __ffsll((long long)bits) - 1;
+            bits &= bits - 1;
+            const int k0 = L.es[ej], k1 = L.ee[ej];
+            const int off = h.off(ej, ei);
+            for (int j = k0 + ((g.r - t0) & (g.G - 1)); j < k1; j += g.G) {
+              const float xj = x[j];
+              const auto pj = h.row(j, ej) + off + c0;
+#pragma unroll
+              for (int r = 0; r < kChunk; r++)
+                if (r < nc) acc[r] += pj[r] * xj;
+            }
+            t0 += k1 - k0;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < kChunk; r++) acc[r] = group_sum(acc[r], g.G);
+        const int od = h.off(ei, ei);
+        float y[kChunk];
+#pragma unroll
+        for (int r = kChunk - 1; r >= 0; r--) {
+          if (r < nc) {
+            float v = x[c0 + r] - acc[r];
+#pragma unroll
+            for (int q = r + 1; q < kChunk; q++)
+              if (q < nc) v -= h.row(c0 + q, ei)[od + c0 + r] * y[q];
+            y[r] = v / h.row(c0 + r, ei)[od + c0 + r];
+          }
+        }
+        if (g.r == 0) {
+#pragma unroll
+          for (int r = 0; r < kChunk; r++)
+            if (r < nc) x[c0 + r] = y[r];
+        }
       }
     }
     __syncthreads();

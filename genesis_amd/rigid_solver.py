@@ -56,9 +56,17 @@ INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", 
 WAVE_MAX_ENTITIES = 128
 
 
-def default_solver_variant(M: RigidModel) -> int:
-    """1 = one wavefront per env (large or many-entity systems), 0 = one lane per env (small articulations)."""
-    return 1 if (M.n_dofs >= 48 and M.n_entities <= WAVE_MAX_ENTITIES) else 0
+# Up to this many envs, one wavefront per env still fills the chip (8 waves per CU at 2,048 envs) and cuts the
+# per-env latency that a few envs' step is made of; beyond it, small articulations run one lane per env.
+WAVE_MAX_SMALL_ENVS = 2048
+
+
+def default_solver_variant(M: RigidModel, n_envs: int = None) -> int:
+    """1 = one wavefront per env (large or many-entity systems, or few envs), 0 = one lane per env (many envs of a
+    small articulation)."""
+    if M.n_entities > WAVE_MAX_ENTITIES:
+        return 0
+    return 1 if (M.n_dofs >= 48 or (n_envs is not None and n_envs <= WAVE_MAX_SMALL_ENVS)) else 0
 
 
 def state_sizes(M: RigidModel) -> dict:
@@ -100,7 +108,8 @@ def finalize_model(M: RigidModel):
 class DeviceModel:
     """Static tables of one scene type uploaded once (torch-owned device memory) + the gsr_model descriptor."""
 
-    def __init__(self, M: RigidModel, device, solver_variant=None, collide_variant=None, dyn_variant=None):
+    def __init__(self, M: RigidModel, device, solver_variant=None, collide_variant=None, dyn_variant=None,
+                 n_envs=None):
         finalize_model(M)
         self.M = M
         self.device = torch.device(device)
@@ -121,7 +130,7 @@ class DeviceModel:
         d.iterations, d.ls_iterations, d.n_contacts_per_pair = o.iterations, o.ls_iterations, o.n_contacts_per_pair
         d.enable_collision, d.enable_joint_limit = int(o.enable_collision), int(o.enable_joint_limit)
         d.enable_multi_contact = int(o.enable_multi_contact)
-        d.solver_variant = default_solver_variant(M) if solver_variant is None else int(solver_variant)
+        d.solver_variant = default_solver_variant(M, n_envs) if solver_variant is None else int(solver_variant)
         if d.solver_variant == 1 and M.n_entities > WAVE_MAX_ENTITIES:
             raise ValueError(f"wavefront solver supports at most {WAVE_MAX_ENTITIES} entities")
         d.collide_variant = d.solver_variant if collide_variant is None else int(collide_variant)
@@ -184,7 +193,7 @@ class RigidSolver:
         if self.device.type != "cuda":
             raise RuntimeError("the rigid step runs on a HIP device; there is no CPU execution path")
         self._lib = _lib.load()
-        self.dm = DeviceModel(model, self.device, solver_variant, collide_variant, dyn_variant)
+        self.dm = DeviceModel(model, self.device, solver_variant, collide_variant, dyn_variant, n_envs=n_envs)
         self.st = DeviceState(self.dm, self._B)
         self._errno_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._substep = 0

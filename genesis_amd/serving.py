@@ -89,16 +89,24 @@ class Pool:
         v = int(self.S.dm.desc.solver_variant)
         self.envs_per_tile = envs_per_tile or (64 if v == 0 else 1)
         self.bytes_per_env = sum(t.numel() * t.element_size() for t in self.S.st.t.values()) // capacity
+        self.ms_per_env = 0.0  # GPU time one env adds to a tick (set from tuning; used by a tick-time budget)
 
 
 class Server:
-    """Continuous-batching request loop over per-scene-type pools under a device memory budget (bytes)."""
+    """Continuous-batching request loop over per-scene-type pools under a device memory budget (bytes).
+
+    With `tick_budget_ms`, admission also keeps the predicted tick time (sum over active requests of n_envs times
+    the pool's `ms_per_env`) within the budget, so the step latency of admitted requests stays bounded while later
+    arrivals wait; a request is always admitted when nothing is active, so a request larger than the budget still
+    runs (alone)."""
 
     def __init__(self, pools: Dict[str, Pool], memory_budget: int, record_results: bool = False,
-                 launch: Optional[dict] = None):
+                 launch: Optional[dict] = None, tick_budget_ms: Optional[float] = None):
         self.pools = pools
         self.launch = launch or {}
         self.budget = memory_budget
+        self.tick_budget = tick_budget_ms
+        self.tick_load = 0.0
         self.used = 0
         self.record = record_results
         self.order = sorted(pools, key=lambda k: -pools[k].M.n_dofs)  # heavier scene types first in the launch
@@ -106,15 +114,21 @@ class Server:
     def cost(self, r: Request) -> int:
         return r.n_envs * self.pools[r.scene].bytes_per_env
 
-    def _try_admit(self, r: Request, now: float) -> bool:
+    def tick_cost(self, r: Request) -> float:
+        return r.n_envs * self.pools[r.scene].ms_per_env
+
+    def _try_admit(self, r: Request, now: float, n_active: int) -> bool:
         p = self.pools[r.scene]
         if self.used + self.cost(r) > self.budget or r.n_envs > p.slots.capacity:
+            return False
+        if self.tick_budget is not None and n_active and self.tick_load + self.tick_cost(r) > self.tick_budget:
             return False
         b = p.slots.alloc(r.n_envs)
         if b is None:
             return False
         r.slot, r.admitted, r.steps_done = b, now, 0
         self.used += self.cost(r)
+        self.tick_load += self.tick_cost(r)
         init_state(p.w, p.M, p.S, b, r.n_envs)
         r.control = Control(p.w, p.M, p.S, r.seed, b, r.n_envs)
         return True
@@ -127,6 +141,7 @@ class Server:
         r.finished = now
         p.slots.release(r.slot, r.n_envs)
         self.used -= self.cost(r)
+        self.tick_load -= self.tick_cost(r)
         r.control = None
 
     def _groups(self, active: List[Request]) -> List[Group]:
@@ -162,7 +177,7 @@ class Server:
                 log(f"  t={now:.1f}s ticks {ticks} active {len(active)} queued {len(queue)} pending {len(pending)}")
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
-            while queue and self._try_admit(queue[0], now):
+            while queue and self._try_admit(queue[0], now, len(active)):
                 active.append(queue.popleft())
                 dirty = True
             if not active:

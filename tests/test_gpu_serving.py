@@ -76,8 +76,8 @@ def test_admission_respects_memory_budget():
     peak = []
     orig = srv._try_admit
 
-    def spy(r, now):
-        ok = orig(r, now)
+    def spy(r, now, n_active):
+        ok = orig(r, now, n_active)
         peak.append(srv.used)
         return ok
     srv._try_admit = spy
@@ -87,3 +87,20 @@ def test_admission_respects_memory_budget():
     # at most two 8-env requests fit in a 20-env budget, so admissions were staggered
     adm = sorted(r.admitted for r in reqs)
     assert adm[2] > adm[0]
+
+
+def test_tick_budget_admits_one_request_at_a_time_when_two_do_not_fit():
+    """With a tick budget of ten envs' worth of GPU time, 8-env requests run one after another, and a 30-env request
+    (beyond the budget on its own) still runs, alone."""
+    from genesis_amd.serving import Request
+    srv = _server(record=False)
+    for p in srv.pools.values():
+        p.ms_per_env = 1.0
+    srv.tick_budget = 10.0
+    reqs = [Request(i, "quadruped", 8, 4, 0.0, seed=i) for i in range(3)] + [Request(3, "arm", 30, 3, 0.0, seed=3)]
+    st = srv.run(reqs)
+    assert st["completed"] == 4 and st["failed"] == 0
+    spans = sorted((r.admitted, r.finished) for r in reqs)
+    for (a0, f0), (a1, f1) in zip(spans, spans[1:]):
+        assert a1 >= f0, "two requests were active together"
+    assert srv.tick_load == 0.0 and srv.used == 0
