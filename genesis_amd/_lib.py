@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "_build", "libgenesis_rigid.so")
 
 EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_group_size", "gsr_batch_info_size",
            "gsr_gjk_workspace_words",
-           "gsr_step", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno", "gsr_run_stage", "gsr_launch_kernel", "gsr_batch_table_bytes",
+           "gsr_step", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno",
+           "gsr_get_contacts", "gsr_links_contact_force", "gsr_get_equalities", "gsr_run_stage", "gsr_launch_kernel", "gsr_batch_table_bytes",
            "gsr_batch_upload", "gsr_step_fused", "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy"]
 
 _CTYPE = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "float": ctypes.c_float,
@@ -104,6 +105,11 @@ def load(path: str = LIB_PATH):
     L.gsr_forward_kinematics.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
     L.gsr_reset_envs.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p]
     L.gsr_get_errno.argtypes = [P(GsrState), ctypes.c_void_p, ctypes.c_void_p]
+    L.gsr_get_contacts.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.gsr_links_contact_force.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+    L.gsr_get_equalities.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.gsr_run_stage.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
     L.gsr_launch_kernel.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
     L.gsr_batch_table_bytes.restype = ctypes.c_int64

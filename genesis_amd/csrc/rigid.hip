@@ -334,6 +334,81 @@ __global__ void k_errno_reduce(gsr_state s, uint32_t* out) {
   if (threadIdx.x == 0) out[0] = part[0];
 }
 
+// collider_kernel_get_contacts (collider_decomp.py:687-724). Contact i of env b goes to output row
+// `starts[b] + i` (ragged) or `b * n_max + i` (padded, starts == nullptr). Padded rows past the env's contact count
+// get the values the reference's freshly allocated buffers hold there: -1 indices, zero floats. One thread per
+// (slot i, env b), env fastest, so the state reads of a wavefront are consecutive envs of one slot.
+__global__ void k_get_contacts(gsr_model m, gsr_state s, const int32_t* starts, int32_t n_max, int64_t out_rows,
+                               int32_t* iout, float* fout) {
+  const int B = s.n_envs;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)B * n_max) return;
+  const int i = (int)(t / B), b = (int)(t % B);
+  const Env e(m, s, b);
+  const int nc = min(e.I(s.n_contacts, 0), m.max_contacts);
+  if (starts && i >= nc) return;
+  const int64_t row = starts ? (int64_t)starts[b] + i : (int64_t)b * n_max + i;
+  if (row < 0 || row >= out_rows) return;
+  int32_t* io = iout + 4 * row;
+  float* fo = fout + 10 * row;
+  if (i >= nc) {
+    for (int k = 0; k < 4; k++) io[k] = -1;
+    for (int k = 0; k < 10; k++) fo[k] = 0.f;
+    return;
+  }
+  const int ga = e.I(s.contact_geoms, 2 * i), gb = e.I(s.contact_geoms, 2 * i + 1);
+  io[0] = m.geom_link[ga];
+  io[1] = m.geom_link[gb];
+  io[2] = ga;
+  io[3] = gb;
+  fo[0] = e.F(s.contact_pen, i);
+  const f3 p = e.L3(s.contact_pos, i), n = e.L3(s.contact_normal, i), f = e.L3(s.contact_force, i);
+  fo[1] = p.x; fo[2] = p.y; fo[3] = p.z;
+  fo[4] = n.x; fo[5] = n.y; fo[6] = n.z;
+  fo[7] = f.x; fo[8] = f.y; fo[9] = f.z;
+}
+
+// links_state.contact_force of func_update_contact_force (constraint_solver_decomp.py:1350-1379): per env, zero
+// every link, then for each contact in order subtract its force from link_a and add it to link_b, the reference's
+// order of float additions. Output row k (env envs_idx[k], or env k when envs_idx is null) holds n_links x 3 floats.
+__global__ void k_links_contact_force(gsr_model m, gsr_state s, const int32_t* envs_idx, int32_t n_idx, float* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_idx) return;
+  const int b = envs_idx ? envs_idx[k] : k;
+  if (b < 0 || b >= s.n_envs) return;
+  const Env e(m, s, b);
+  float* o = out + (size_t)k * m.n_links * 3;
+  for (int j = 0; j < 3 * m.n_links; j++) o[j] = 0.f;
+  const int nc = m.enable_collision ? min(e.I(s.n_contacts, 0), m.max_contacts) : 0;
+  for (int i = 0; i < nc; i++) {
+    const f3 f = e.L3(s.contact_force, i);
+    float* a = o + 3 * m.geom_link[e.I(s.contact_geoms, 2 * i)];
+    float* c = o + 3 * m.geom_link[e.I(s.contact_geoms, 2 * i + 1)];
+    a[0] = a[0] - f.x; a[1] = a[1] - f.y; a[2] = a[2] - f.z;
+    c[0] = c[0] + f.x; c[1] = c[1] + f.y; c[2] = c[2] + f.z;
+  }
+}
+
+// kernel_get_equality_constraints (constraint_solver_decomp.py:2510-2558) for the JOINT equalities this solver
+// builds (one row each, rows 0..n_eq-1 in model order). Every env has the model's n_equalities, so padded and ragged
+// outputs coincide: row b * n_equalities + i holds {type, obj1, obj2} and the row force in column 0 of 6.
+__global__ void k_get_equalities(gsr_model m, gsr_state s, int32_t* iout, float* fout) {
+  const int B = s.n_envs, E = m.n_equalities;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)B * E) return;
+  const int i = (int)(t / B), b = (int)(t % B);
+  const Env e(m, s, b);
+  const int64_t row = (int64_t)b * E + i;
+  iout[3 * row] = m.eq_type[i];
+  iout[3 * row + 1] = m.eq_obj1[i];
+  iout[3 * row + 2] = m.eq_obj2[i];
+  // equalities of other types are rejected when the model is compiled; their rows would be 3 (CONNECT) or 6 (WELD)
+  int c = 0;
+  for (int j = 0; j < i; j++) c += m.eq_type[j] == EQ_JOINT;
+  fout[6 * row] = m.eq_type[i] == EQ_JOINT ? e.EF(s.efc_force, c) : 0.f;
+  for (int k = 1; k < 6; k++) fout[6 * row + k] = 0.f;
+}
+
 int check(const gsr_model* m, const gsr_state* s) {
   if (!m || !s) return GSR_E_ARG;
   if (s->n_envs <= 0) return GSR_E_ARG;
@@ -434,6 +509,43 @@ int gsr_reset_envs(const gsr_model* model, const gsr_state* state, const int32_t
   int n = envs_idx ? n_idx : state->n_envs;
   if (n <= 0) return GSR_OK;
   hipLaunchKernelGGL(k_reset, grid_for(n), dim3(kBlock), 0, stream, *model, *state, envs_idx, n, what);
+  return launch_status();
+}
+
+int gsr_get_contacts(const gsr_model* model, const gsr_state* state, const int32_t* dev_starts, int32_t n_max,
+                     int64_t out_rows, int32_t* dev_iout, float* dev_fout, hipStream_t stream) {
+  int rc = check(model, state);
+  if (rc) return rc;
+  if (n_max < 0 || out_rows < 0 || ((!dev_iout || !dev_fout) && out_rows > 0)) return GSR_E_ARG;
+  if (!dev_starts && out_rows < (int64_t)state->n_envs * n_max) return GSR_E_ARG;
+  const int64_t n = (int64_t)state->n_envs * n_max;
+  if (n == 0 || out_rows == 0) return 0;
+  hipLaunchKernelGGL(k_get_contacts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *model, *state,
+                     dev_starts, n_max, out_rows, dev_iout, dev_fout);
+  return launch_status();
+}
+
+int gsr_links_contact_force(const gsr_model* model, const gsr_state* state, const int32_t* dev_envs_idx,
+                            int32_t n_idx, float* dev_out, hipStream_t stream) {
+  int rc = check(model, state);
+  if (rc) return rc;
+  const int n = dev_envs_idx ? n_idx : state->n_envs;
+  if (n < 0 || (n > 0 && !dev_out)) return GSR_E_ARG;
+  if (n == 0 || model->n_links == 0) return 0;
+  hipLaunchKernelGGL(k_links_contact_force, grid_for(n), dim3(kBlock), 0, stream, *model, *state, dev_envs_idx, n,
+                     dev_out);
+  return launch_status();
+}
+
+int gsr_get_equalities(const gsr_model* model, const gsr_state* state, int32_t* dev_iout, float* dev_fout,
+                       hipStream_t stream) {
+  int rc = check(model, state);
+  if (rc) return rc;
+  const int64_t n = (int64_t)state->n_envs * model->n_equalities;
+  if (n == 0) return 0;
+  if (!dev_iout || !dev_fout) return GSR_E_ARG;
+  hipLaunchKernelGGL(k_get_equalities, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *model, *state,
+                     dev_iout, dev_fout);
   return launch_status();
 }
 

@@ -412,6 +412,8 @@ def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel
                               inertial_pos=ipos, inertial_quat=iquat, inertial_i=iI, inertial_mass=imass,
                               is_fixed=is_fixed, entity=i_e, name=ls.name))
         for eq in ent.equalities:
+            if eq.type != EQ_JOINT:
+                raise NotImplementedError("only JOINT equalities are built (CONNECT and WELD rows are not)")
             sp = eq.sol_params if eq.sol_params is not None else default_solver_params()
             eqs.append(dict(type=eq.type, obj1=name_to_joint[eq.obj1], obj2=name_to_joint[eq.obj2],
                             data=np.pad(np.asarray(eq.data, np.float64), (0, 11 - len(eq.data))),

@@ -345,3 +345,148 @@ class RigidSolver:
         ei = e.to(torch.int32)
         self.reset_caches(ei)
         self.forward_kinematics(ei)
+
+    # -- contact and constraint readback (col:463-565, cs:235-322, rigid_entity.py:2838-2850) ---------------------
+    @property
+    def collider(self) -> "Collider":
+        return Collider(self)
+
+    @property
+    def constraint_solver(self) -> "ConstraintSolver":
+        return ConstraintSolver(self)
+
+    def get_links_net_contact_force(self, envs_idx=None, links_idx=None):
+        """Net force of the last step's contacts on each link, (n_envs, n_links, 3), or (n_links, 3) when
+        n_envs == 0: the per-link sums of func_update_contact_force (constraint_solver_decomp.py:1350-1379)."""
+        L = self.model.n_links
+        if envs_idx is None:
+            n, idx = self._B, None
+        else:
+            idx = self._idx(envs_idx, self._B).to(torch.int32).contiguous()
+            if ((idx < 0) | (idx >= self._B)).any():
+                raise IndexError("envs_idx out of range")
+            n = int(idx.numel())
+        out = torch.empty((n, L, 3), dtype=torch.float32, device=self.device)
+        rc = self._lib.gsr_links_contact_force(self.dm.desc, self.st.desc,
+                                               None if idx is None else ctypes.c_void_p(idx.data_ptr()), n,
+                                               ctypes.c_void_p(out.data_ptr()), self._stream())
+        _lib.check(rc, "gsr_links_contact_force")
+        if links_idx is not None:
+            out = out[:, self._idx(links_idx, L)]
+        return out[0] if self.n_envs == 0 else out
+
+
+def _to_numpy(values):
+    return tuple(v.cpu().numpy() for v in values)
+
+
+class Collider:
+    """Contact readback of the reference's Collider (collider_decomp.py:463-565): the contacts found by the last
+    step's collision detection, with the per-contact force the last Newton solve gave them."""
+
+    KEYS = ("link_a", "link_b", "geom_a", "geom_b", "penetration", "position", "normal", "force")
+
+    def __init__(self, solver: RigidSolver):
+        self._solver = solver
+
+    def get_contacts(self, as_tensor: bool = True, to_torch: bool = True, keep_batch_dim: bool = False):
+        """Dict of the contact fields. as_tensor: (n_envs, n_contacts_max, ...) padded with -1 / 0 (no env axis
+        when n_envs == 0 unless keep_batch_dim); otherwise one entry per env in a tuple (flat when n_envs == 0)."""
+        S = self._solver
+        n_envs, B = S.n_envs, S._B
+        counts_dev = S.st.t["n_contacts"][0].clamp(max=S.model.max_contacts)
+        counts = counts_dev.cpu()
+        n_max = int(counts.max()) if B else 0
+        if as_tensor or n_envs == 0:
+            starts, out_size = None, n_max * B
+        else:
+            starts = (torch.cumsum(counts_dev, 0) - counts_dev).to(torch.int32).contiguous()
+            out_size = int(counts.sum())
+        iout = torch.empty((out_size, 4), dtype=torch.int32, device=S.device)
+        fout = torch.empty((out_size, 10), dtype=torch.float32, device=S.device)
+        if out_size > 0:
+            rc = S._lib.gsr_get_contacts(S.dm.desc, S.st.desc, None if starts is None else ctypes.c_void_p(starts.data_ptr()),
+                                         n_max, out_size, ctypes.c_void_p(iout.data_ptr()),
+                                         ctypes.c_void_p(fout.data_ptr()), S._stream())
+            _lib.check(rc, "gsr_get_contacts")
+
+        def fields(i, f):
+            return (i[..., 0], i[..., 1], i[..., 2], i[..., 3], f[..., 0], f[..., 1:4], f[..., 4:7], f[..., 7:])
+
+        if as_tensor or n_envs == 0:
+            if n_envs > 0 or keep_batch_dim:
+                iout, fout = iout.reshape(B, n_max, 4), fout.reshape(B, n_max, 10)
+            values = fields(iout, fout)
+            if not to_torch:
+                values = _to_numpy(values)
+        else:
+            n = counts.tolist()
+            per_env = [fields(i, f) for i, f in zip(torch.split(iout, n), torch.split(fout, n))]
+            if keep_batch_dim:
+                per_env = [tuple(v[None] for v in env) for env in per_env]
+            if not to_torch:
+                per_env = [_to_numpy(env) for env in per_env]
+            values = tuple(zip(*per_env)) if per_env else tuple(() for _ in self.KEYS)
+        return dict(zip(self.KEYS, values))
+
+
+EQUALITY_WELD = 1  # gs.EQUALITY_TYPE: CONNECT 0, WELD 1, JOINT 2
+
+
+class ConstraintSolver:
+    """Equality-constraint readback of the reference's ConstraintSolver (constraint_solver_decomp.py:235-322)."""
+
+    def __init__(self, solver: RigidSolver):
+        self._solver = solver
+
+    def get_equality_constraints(self, as_tensor: bool = True, to_torch: bool = True):
+        """{type, obj_a, obj_b, force}: (n_envs, n_equalities[, 6]) with as_tensor, else a tuple per env. Every env
+        holds the model's equalities, in model order."""
+        S = self._solver
+        B, E = S._B, S.model.n_equalities
+        iout = torch.full((B * E, 3), -1, dtype=torch.int32, device=S.device)
+        fout = torch.zeros((B * E, 6), dtype=torch.float32, device=S.device)
+        if B * E > 0:
+            rc = S._lib.gsr_get_equalities(S.dm.desc, S.st.desc, ctypes.c_void_p(iout.data_ptr()),
+                                           ctypes.c_void_p(fout.data_ptr()), S._stream())
+            _lib.check(rc, "gsr_get_equalities")
+        iout, fout = iout.reshape(B, E, 3), fout.reshape(B, E, 6)
+        if as_tensor:
+            values = (iout[..., 0], iout[..., 1], iout[..., 2], fout)
+            if not to_torch:
+                values = _to_numpy(values)
+        else:
+            per_env = [(iout[b, :, 0], iout[b, :, 1], iout[b, :, 2], fout[b]) for b in range(B)]
+            if not to_torch:
+                per_env = [_to_numpy(env) for env in per_env]
+            values = tuple(zip(*per_env))
+        return dict(zip(("type", "obj_a", "obj_b", "force"), values))
+
+    def get_weld_constraints(self, as_tensor: bool = True, to_torch: bool = True):
+        """The WELD subset of get_equality_constraints, with obj_a / obj_b renamed link_a / link_b
+        (constraint_solver_decomp.py:289-322)."""
+        info = self.get_equality_constraints(as_tensor, to_torch)
+        eq_type = info.pop("type")
+        out = {}
+        if as_tensor:
+            mask = eq_type == EQUALITY_WELD
+            n_welds = mask.sum(-1) if to_torch else np.sum(mask, axis=-1)
+            n_max = int(n_welds.max()) if len(n_welds) else 0
+            for key, value in info.items():
+                shape = (len(mask), n_max, *value.shape[2:])
+                if to_torch:
+                    w = (torch.zeros(shape, dtype=value.dtype, device=value.device) if torch.is_floating_point(value)
+                         else torch.full(shape, -1, dtype=value.dtype, device=value.device))
+                else:
+                    w = np.zeros(shape, value.dtype) if np.issubdtype(value.dtype, np.floating) else np.full(shape, -1, value.dtype)
+                for b in range(len(mask)):
+                    k = int(n_welds[b])
+                    w[b, :k] = value[b][mask[b]]
+                out[key] = w
+        else:
+            masks = tuple(t == EQUALITY_WELD for t in eq_type)
+            for key, value in info.items():
+                out[key] = tuple(v[m] for m, v in zip(masks, value))
+        out["link_a"] = out.pop("obj_a")
+        out["link_b"] = out.pop("obj_b")
+        return out

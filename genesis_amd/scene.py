@@ -18,7 +18,7 @@ import torch
 from . import loaders
 from .model import (CompileOptions, EntitySpec, GeomSpec, JOINT_FIXED, JOINT_FREE, JointSpec, LinkSpec, GEOM_BOX,
                     GEOM_PLANE, GEOM_SPHERE, compile_model)
-from .rigid_solver import RigidSolver, RATE_CHECK_ERRNO
+from .rigid_solver import GenesisException, RigidSolver, RATE_CHECK_ERRNO
 
 ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "assets")
 # Reference asset paths whose compiled descriptions ship with the package.
@@ -188,6 +188,7 @@ class RigidEntity:
     def _bind(self, M):
         self.link_start, self.link_end = int(M.ent_link_start[self.idx]), int(M.ent_link_end[self.idx])
         self.dof_start, self.dof_end = int(M.ent_dof_start[self.idx]), int(M.ent_dof_end[self.idx])
+        self.geom_start, self.geom_end = int(M.ent_geom_start[self.idx]), int(M.ent_geom_end[self.idx])
         qs = [M.link_q_start[l] for l in range(self.link_start, self.link_end)]
         qe = [M.link_q_end[l] for l in range(self.link_start, self.link_end)]
         self.q_start, self.q_end = (int(min(qs)), int(max(qe))) if qs else (0, 0)
@@ -246,6 +247,41 @@ class RigidEntity:
 
     def get_quat(self, envs_idx=None):
         return self.get_links_quat(envs_idx)[..., 0, :]
+
+    def get_mass(self):
+        """Total mass of the entity's links (rigid_entity.py:2965)."""
+        return float(self._scene.model.link_inertial_mass[self.link_start:self.link_end].astype(np.float64).sum())
+
+    def get_contacts(self, with_entity=None, exclude_self_contact=False):
+        """Contacts of the last step involving this entity (rigid_entity.py:2756-2836): the collider's padded
+        fields, plus `valid_mask` (n_envs > 0) or filtered rows (n_envs == 0), with `force` split into
+        force_a = -force and force_b = +force."""
+        info = self._solver.collider.get_contacts(as_tensor=True, to_torch=True)
+        op = torch.logical_xor if exclude_self_contact else torch.logical_or
+        if with_entity is not None and self.idx == with_entity.idx:
+            if exclude_self_contact:
+                raise GenesisException("`with_entity` is self but `exclude_self_contact` is True.")
+            op = torch.logical_and
+
+        def inside(g, ent):
+            return (g >= ent.geom_start) & (g < ent.geom_end)
+
+        valid = op(inside(info["geom_a"], self), inside(info["geom_b"], self))
+        if with_entity is not None and self.idx != with_entity.idx:
+            valid = valid & (inside(info["geom_a"], with_entity) | inside(info["geom_b"], with_entity))
+        if self._solver.n_envs == 0:
+            info = {k: v[valid] for k, v in info.items()}
+        else:
+            info["valid_mask"] = valid
+        info["force_a"] = -info["force"]
+        info["force_b"] = +info["force"]
+        del info["force"]
+        return info
+
+    def get_links_net_contact_force(self, envs_idx=None):
+        """Net contact force on each link of the entity, (n_envs, n_links, 3) or (n_links, 3) when n_envs == 0
+        (rigid_entity.py:2838-2850)."""
+        return self._solver.get_links_net_contact_force(envs_idx)[..., self.link_start:self.link_end, :]
 
 
 # ------------------------------------------------------------------------------------------------------------

@@ -142,6 +142,25 @@ int gsr_reset_envs(const gsr_model* model, const gsr_state* state, const int32_t
                    uint32_t what, hipStream_t stream);
 int gsr_get_errno(const gsr_state* state, uint32_t* dev_out, hipStream_t stream);
 
+/* ---- Readback of the last step's contacts and constraint forces ---------------------------------------------- */
+/* Replaces collider_kernel_get_contacts (collider_decomp.py:687-724) behind Collider.get_contacts (:463-565).
+ * Output row r holds iout[4r..] = {link_a, link_b, geom_a, geom_b} and fout[10r..] = {penetration, position[3],
+ * normal[3], force[3]}. Padded (dev_starts == NULL): contact i of env b at row b * n_max + i, rows past the env's
+ * count set to -1 / 0; out_rows must be at least n_envs * n_max. Ragged: row dev_starts[b] + i, dev_starts being the
+ * exclusive prefix sum of the per-env counts. Rows at or past out_rows are never written. */
+int gsr_get_contacts(const gsr_model* model, const gsr_state* state, const int32_t* dev_starts, int32_t n_max,
+                     int64_t out_rows, int32_t* dev_iout, float* dev_fout, hipStream_t stream);
+/* Replaces links_state.contact_force of func_update_contact_force (constraint_solver_decomp.py:1350-1379), read by
+ * RigidEntity.get_links_net_contact_force (rigid_entity.py:2838-2850): dev_out[k][l][3] = net contact force on link l
+ * of env dev_envs_idx[k] (of env k when dev_envs_idx is NULL, n_idx then ignored). */
+int gsr_links_contact_force(const gsr_model* model, const gsr_state* state, const int32_t* dev_envs_idx,
+                            int32_t n_idx, float* dev_out, hipStream_t stream);
+/* Replaces kernel_get_equality_constraints (constraint_solver_decomp.py:2510-2558) behind
+ * ConstraintSolver.get_equality_constraints (:235-287): row b * n_equalities + i holds iout[3r..] = {type, obj1, obj2}
+ * and fout[6r..] = the equality's row forces (one for a JOINT equality, the rest zero). */
+int gsr_get_equalities(const gsr_model* model, const gsr_state* state, int32_t* dev_iout, float* dev_fout,
+                       hipStream_t stream);
+
 /* One kernel of the plain substep (0 = step_1 dynamics, 1 = collision + constraint rows, 2 = Newton solve,
  * 3 = integration + end-of-step kinematics); gsr_step launches exactly these four in order. No counterpart in the
  * reference: used to time each kernel on the caller's stream. */

@@ -187,3 +187,15 @@ class Oracle:
         out = np.zeros((max(self.m.max_broad, 1), 2), np.int32)
         n = lib().orc_broadphase_from_aabbs(self.h, b, amin.ctypes.data, amax.ctypes.data, out.ctypes.data)
         return out[:n]
+
+
+def links_net_contact_force(link_a, link_b, force, n_links):
+    """Net contact force per link of one env (constraint_solver_decomp.py:1350-1379): start from zero and, contact
+    by contact in list order, subtract the contact's force from link_a and add it to link_b. Accumulates in
+    force's dtype, so float32 inputs give the reference's float32 sums."""
+    force = np.asarray(force)
+    out = np.zeros((n_links, 3), force.dtype)
+    for la, lb, f in zip(np.asarray(link_a).tolist(), np.asarray(link_b).tolist(), force):
+        out[la] = out[la] - f
+        out[lb] = out[lb] + f
+    return out

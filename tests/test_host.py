@@ -46,6 +46,9 @@ def test_null_arguments_are_rejected_without_gpu(lib):
     # argument validation happens before any HIP call
     assert lib.gsr_step(None, None, 1, None) == -1
     assert lib.gsr_get_errno(None, None, None) == -1
+    assert lib.gsr_get_contacts(None, None, None, 0, 0, None, None, None) == -1
+    assert lib.gsr_links_contact_force(None, None, None, 0, None, None) == -1
+    assert lib.gsr_get_equalities(None, None, None, None, None) == -1
 
 
 def test_state_size_rules_cover_every_pointer():

@@ -58,6 +58,35 @@ def test_net_contact_force_equals_weight(Oracle):
     np.testing.assert_allclose(abs(fz), mass * 9.81, rtol=1e-3)
 
 
+def test_links_net_contact_force_of_resting_cube(oracle_lib, Oracle):
+    """test_rigid_physics.py:1626-1675 (the hold phase): after 50 steps at rest the net contact force on a 4 cm cube
+    is its weight, upward, within 1e-5 N; the plane takes the opposite force and the links' forces sum to zero.
+    Pins the sign of the per-contact force (normal and friction-pyramid terms) and the per-link accumulation."""
+    M = box_scene(size=(0.04, 0.04, 0.04), pos=(0.65, 0.0, 0.02))
+    o = Oracle(M, 1, 64)
+    for _ in range(50):
+        o.run(1)
+    c = o.get("contacts")[0]
+    assert len(c) == 4
+    net = oracle_lib.links_net_contact_force(c[:, 2].astype(int), c[:, 3].astype(int), c[:, 12:15], M.n_links)
+    weight = float(M.link_inertial_mass[1]) * 9.81
+    np.testing.assert_allclose(net[1], [0.0, 0.0, weight], atol=1e-5)
+    np.testing.assert_allclose(net[0], [0.0, 0.0, -weight], atol=1e-5)
+    np.testing.assert_allclose(net.sum(0), 0.0, atol=1e-12)
+    # every contact pushes the cube up and out of the plane: the force lies in the friction cone around +z
+    assert (c[:, 14] > 0).all()
+    mu = c[:, 11]
+    assert (np.hypot(c[:, 12], c[:, 13]) <= mu * c[:, 14] * (1 + 1e-9)).all()
+
+
+def test_links_net_contact_force_order_of_sums(oracle_lib):
+    """Two contacts on one link pair: a wrong sign or a swapped link index changes the result."""
+    f = np.array([[1.0, 2.0, 3.0], [0.5, -1.0, 4.0]], np.float32)
+    net = oracle_lib.links_net_contact_force([0, 2], [1, 1], f, 3)
+    np.testing.assert_array_equal(net, np.array([[-1, -2, -3], [1.5, 1, 7], [-0.5, 1, -4]], np.float32))
+    assert net.dtype == np.float32
+
+
 def test_stacked_boxes_settle(Oracle):
     """test_rigid_physics.py:852-874: three stacked unit-height boxes settle at z = 0.5 +- 2e-3 (top center)."""
     ents = [W.plane_entity()] + [W.box_entity((0.2, 0.2, 0.2), (0, 0, 0.1 + 0.2 * i + 0.001)) for i in range(3)]
