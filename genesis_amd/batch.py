@@ -5,10 +5,13 @@ no reference counterpart. It changes execution, not arithmetic: every env runs t
 drop-in (`gsr_step`), so with the same solver variant a fused result is bitwise the plain result.
 
 * `FusedBatch(groups)` uploads one device table (`gsr_batch_upload`) and steps every group with `gsr_step_fused`:
-  a persistent grid of one-wavefront workgroups pulls tiles of `envs_per_tile` envs, heaviest group first.
+  one-wavefront workgroups take tiles of `envs_per_tile` envs, heaviest group first, either in one launch that runs
+  the whole substep per tile (from a persistent work queue or one workgroup per tile) or, `phased=True`, in one
+  launch per substep stage (dynamics, collision, GJK/EPA pass, solve, integration), each over every tile.
 * `capture()` records n substeps into a hipGraph (`gsr_graph_capture`); `replay()` launches it.
 * `autotune()` times each group alone under each candidate (solver variant, collision variant, envs per tile) on a
-  copy of its state, orders the winners by per-tile cost (longest first), then times the launch forms.
+  copy of its state, for the single-kernel and the phased launches, orders the winners by per-tile cost (longest
+  first), then times the launch forms.
 """
 from __future__ import annotations
 
@@ -44,7 +47,7 @@ class FusedBatch:
     """One device table over several groups; `step()` runs fused substeps, `capture()`/`replay()` a hipGraph."""
 
     def __init__(self, groups: Sequence[Group], persistent: bool = True, waves_per_simd: int = 4,
-                 gjk_inline: bool = False):
+                 gjk_inline: bool = False, phased: bool = False):
         if not groups:
             raise ValueError("empty batch")
         self.lib = _lib.load()
@@ -68,6 +71,7 @@ class FusedBatch:
         self.info.persistent = int(bool(persistent))
         self.info.waves_per_simd = int(waves_per_simd)
         self.info.gjk_inline = int(bool(gjk_inline))
+        self.info.phased = int(bool(phased))
         torch.cuda.synchronize(self.device)  # the table's zero fill precedes the synchronous upload
         _lib.check(self.lib.gsr_batch_upload(arr, n, ctypes.c_void_p(self.table.data_ptr()), nbytes,
                                              ctypes.byref(self.info)), "gsr_batch_upload")
@@ -157,39 +161,50 @@ def time_group(group: Group, steps: int = 3, warmup: int = 1, **launch) -> float
 
 
 # launch forms: persistent queue or one workgroup per tile; 128 or 256 VGPRs; GJK/EPA inside the fused kernel (one
-# launch per substep, lower occupancy) or deferred to a per-type resume launch
+# launch per substep, lower occupancy) or deferred to a per-type resume launch; or phased: one launch per substep
+# stage over every tile, each stage with its own register allocation
 LAUNCH_CANDIDATES = [dict(persistent=p, waves_per_simd=w, gjk_inline=g)
-                     for g in (False, True) for w in (4, 2) for p in (True, False)]
+                     for g in (False, True) for w in (4, 2) for p in (True, False)] + [dict(phased=True)]
 
 
-def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: bool = False, log=print):
-    """Pick (solver variant, envs per tile) per group by timing each group alone, order the groups by per-tile cost
-    (descending, so the longest tiles start first), then pick the launch form (persistent queue or one workgroup per
-    tile; 128 or 256 VGPRs) by timing the whole batch.
-
-    Returns (tuned groups, launch kwargs for FusedBatch, report of every candidate's ms per substep)."""
-    tuned, report = [], []
+def _tune_groups(groups, launch, steps, warmup, verbose, log, report):
+    """Pick (solver variant, collision variant, envs per tile) per group by timing each group alone under `launch`,
+    then order the groups by per-tile cost, descending, so the longest tiles start first."""
+    tuned = []
     for gi, g in enumerate(groups):
         g = g.resolved()
         best = None
         for v, c, t in candidates(g.solver):
-            ms = time_group(Group(g.solver, g.env_begin, g.n_envs, t, v, c), steps, warmup, **LAUNCH_CANDIDATES[0])
-            report.append({"group": gi, "solver": v, "collide": c, "envs_per_tile": t, "ms": ms})
+            ms = time_group(Group(g.solver, g.env_begin, g.n_envs, t, v, c), steps, warmup, **launch)
+            report.append({"group": gi, "solver": v, "collide": c, "envs_per_tile": t, "ms": ms,
+                           "phased": bool(launch.get("phased"))})
             if verbose:
-                log(f"  group {gi} solver {v} collide {c} tile {t}: {ms:.3f} ms")
+                log(f"  group {gi} solver {v} collide {c} tile {t} {launch}: {ms:.3f} ms")
             if best is None or ms < best[0]:
                 best = (ms, v, c, t)
         ms, v, c, t = best
         n_tiles = (g.n_envs + t - 1) // t
         tuned.append((ms / n_tiles, Group(g.solver, g.env_begin, g.n_envs, t, v, c)))
     tuned.sort(key=lambda x: -x[0])
-    tuned = [g for _, g in tuned]
+    return [g for _, g in tuned]
+
+
+def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: bool = False, log=print):
+    """Tune the per-group choices twice, once under the single fused kernel and once under the phased launches (their
+    best variants differ: the fused kernel's register budget is shared by every stage), then time the whole batch
+    under every launch form with the matching plan and keep the fastest.
+
+    Returns (tuned groups, launch kwargs for FusedBatch, report of every candidate's ms per substep)."""
+    report = []
+    plans = {False: _tune_groups(groups, LAUNCH_CANDIDATES[0], steps, warmup, verbose, log, report),
+             True: _tune_groups(groups, dict(phased=True), steps, warmup, verbose, log, report)}
     best = None
     for lc in LAUNCH_CANDIDATES:
-        ms = time_batch(tuned, steps, warmup, **lc)
+        plan = plans[bool(lc.get("phased"))]
+        ms = time_batch(plan, steps, warmup, **lc)
         report.append(dict(group="all", launch=lc, ms=ms))
         if verbose:
             log(f"  batch {lc}: {ms:.3f} ms")
         if best is None or ms < best[0]:
-            best = (ms, lc)
-    return tuned, best[1], report
+            best = (ms, lc, plan)
+    return best[2], best[1], report

@@ -287,6 +287,68 @@ __device__ __forceinline__ void fused_tile(ConstTypes* types, const BatchRange* 
   run_tile<kGjk>(ty->m[k], ty->s[k], env0, n_here, lds);
 }
 
+// Phased form of the fused batch: one launch per stage of the plain substep, each over every tile of the batch, so
+// each stage gets its own register allocation (the single fused kernel holds every stage's code and spills) while
+// the scene types still share each launch. Stage P: 0 = step_1 dynamics, 1 = equality rows + collision + inequality
+// rows (GJK/EPA pairs deferred), 2 = the GJK/EPA pass and rows of deferred envs, 3 = Newton solve, 4 = step_2. The
+// device functions and their order per env are the plain step's, so results are bitwise those of gsr_step.
+template <int P>
+__device__ __forceinline__ void phase_tile(const gsr_model& m, const gsr_state& s, int env0, int n_here, char* lds) {
+  const int lane = threadIdx.x;
+  const bool mine = lane < n_here;
+  const bool wave = P == 0 || P == 4 ? m.dyn_variant == 1 : P == 1 ? m.collide_variant == 1 : P == 3 ? m.solver_variant == 1 : true;
+  if (!wave) {
+    if (!mine) return;
+    Env e(m, s, env0 + lane);
+    if (P == 0) {
+      dev_smooth_dynamics(e);
+    } else if (P == 1) {
+      dev_equality_rows(e);
+      dev_collision(e);
+      if (!gjk_pending(e)) dev_inequality_rows(e);
+    } else if (P == 3) {
+      dev_solve(e);
+    } else if (P == 4) {
+      dev_integrate(e);
+      dev_kinematics(e);
+    }
+    return;
+  }
+  for (int j = 0; j < n_here; j++) {
+    Env e(m, s, env0 + j);
+    if (P == 0) wave_step1(e);
+    if (P == 1) wave_collide(e, lds);
+    if (P == 2 && wave_gjk_fixup(e)) wave_rows(e);
+    if (P == 3) wave_solve(e, lds);
+    if (P == 4) wave_step2(e);
+    __syncthreads();
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(kWave) k_phase(const char* __restrict__ table) {
+  extern __shared__ char lds[];
+  const BatchHeader* __restrict__ H = reinterpret_cast<const BatchHeader*>(table);
+  ConstTypes* types = (ConstTypes*)(table + kTypesOffset);
+  const BatchRange* __restrict__ R = reinterpret_cast<const BatchRange*>(table + kRangesOffset);
+  const int n_tiles = H->n_tiles, n_groups = H->n_groups;
+  if ((int)blockIdx.x >= n_tiles) return;
+  const int tile = xcd_env(blockIdx.x, n_tiles);
+  int lo = 0, hi = n_groups - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (tile < R[mid].tile_end) hi = mid;
+    else lo = mid + 1;
+  }
+  const BatchRange g = R[lo];
+  const int T = g.envs_per_tile;
+  const int first = (tile - g.tile_begin) * T;
+  const int n_here = min(T, g.n_envs - first);
+  const int k = __builtin_amdgcn_readfirstlane(g.type);
+  const FusedTypes* ty = (const FusedTypes*)types;
+  phase_tile<P>(ty->m[k], ty->s[k], g.env_begin + first, n_here, lds);
+}
+
 // W = resident wavefronts per SIMD the register allocation must allow (2: 256 VGPRs, 4: 128 VGPRs);
 // G = GJK/EPA inside the launch (1) or deferred to k_gjk_resume (0)
 template <int W, int G>
@@ -579,6 +641,17 @@ int64_t gsr_batch_table_bytes(int32_t n_groups) {
   return (int64_t)(kRangesOffset + sizeof(BatchRange) * (size_t)n_groups);
 }
 
+// dynamic LDS of phase 1 (wavefront collision) or 3 (wavefront solver): the largest over the batch's scene types
+static int phase_lds(const gsr_batch_info& b, int phase) {
+  size_t lds = 0;
+  for (int k = 0; k < b.n_types; k++) {
+    const gsr_model& m = b.models[k];
+    if (phase == 1 && m.collide_variant == 1) lds = std::max(lds, (size_t)wave_collide_lds_bytes(m.n_geoms));
+    if (phase == 3 && m.solver_variant == 1) lds = std::max(lds, (size_t)wave_lds_bytes(m.n_dofs, m.n_entities));
+  }
+  return (int)lds;
+}
+
 int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table, int64_t table_bytes,
                      gsr_batch_info* info) {
   if (!groups || !dev_table || !info || n_groups < 1 || n_groups > GSR_MAX_GROUPS) return GSR_E_ARG;
@@ -628,6 +701,25 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   int dev = 0, n_cu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return GSR_E_LAUNCH;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return GSR_E_LAUNCH;
+  if (info->phased) {
+    const int lc = phase_lds(out, 1), ls = phase_lds(out, 3);
+    if (lc > 0 && hipFuncSetAttribute((const void*)k_phase<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lc) !=
+                      hipSuccess)
+      return GSR_E_LAUNCH;
+    if (ls > 0 && hipFuncSetAttribute((const void*)k_phase<3>, hipFuncAttributeMaxDynamicSharedMemorySize, ls) !=
+                      hipSuccess)
+      return GSR_E_LAUNCH;
+    out.n_groups = n_groups;
+    out.n_tiles = tiles;
+    out.persistent = 0;
+    out.waves_per_simd = 0;
+    out.gjk_inline = 0;
+    out.phased = 1;
+    out.grid = tiles;
+    out.lds_bytes = (int32_t)lds;
+    *info = out;
+    return GSR_OK;
+  }
   const int W = info->waves_per_simd == 4 ? 4 : 2;
   const int G = info->gjk_inline ? 1 : 0;
   const void* fn = W == 4 ? (G ? (const void*)k_fused<4, 1> : (const void*)k_fused<4, 0>)
@@ -640,6 +732,7 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   out.persistent = info->persistent ? 1 : 0;
   out.waves_per_simd = W;
   out.gjk_inline = G;
+  out.phased = 0;
   out.grid = out.persistent ? std::max(1, std::min(tiles, n_cu * std::max(per_cu, 1))) : tiles;
   out.lds_bytes = (int32_t)lds;
   *info = out;
@@ -652,6 +745,18 @@ int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_
     return GSR_E_ARG;
   const char* t = static_cast<const char*>(dev_table);
   uint32_t* queue = reinterpret_cast<uint32_t*>(const_cast<char*>(t) + 2 * sizeof(int32_t));
+  if (info->phased) {
+    const dim3 grid(info->n_tiles), block(kWave);
+    const int lc = phase_lds(*info, 1), ls = phase_lds(*info, 3);
+    for (int i = 0; i < n_substeps; i++) {
+      hipLaunchKernelGGL(k_phase<0>, grid, block, 0, stream, t);
+      hipLaunchKernelGGL(k_phase<1>, grid, block, lc, stream, t);
+      hipLaunchKernelGGL(k_phase<2>, grid, block, 0, stream, t);
+      hipLaunchKernelGGL(k_phase<3>, grid, block, ls, stream, t);
+      hipLaunchKernelGGL(k_phase<4>, grid, block, 0, stream, t);
+    }
+    return launch_status();
+  }
   for (int i = 0; i < n_substeps; i++) {
     const dim3 grid(info->grid), block(kWave);
     const int pers = (int)info->persistent;

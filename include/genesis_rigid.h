@@ -195,6 +195,9 @@ typedef struct gsr_batch_info {
   int32_t persistent;        /* input: 1 = CUs x resident workgroups pull tiles from a queue, 0 = one per tile */
   int32_t waves_per_simd;    /* input: register budget of the fused kernel, 2 (256 VGPRs) or 4 (128 VGPRs) */
   int32_t gjk_inline;        /* input: 1 = GJK/EPA inside the fused kernel, 0 = deferred to a per-type resume launch */
+  int32_t phased;            /* input: 1 = one launch per substep stage over every tile (persistent, waves_per_simd
+                                and gjk_inline are then ignored), 0 = one fused launch per substep */
+  int32_t pad;
   gsr_model models[GSR_MAX_TYPES];
   gsr_state states[GSR_MAX_TYPES];
 } gsr_batch_info;

@@ -58,6 +58,8 @@ def _overlap_pile(parts, seed=11):
     ((64, 16, 4), dict(persistent=True, waves_per_simd=2), False),
     ((64, 64, 1), dict(persistent=False, waves_per_simd=2, gjk_inline=False), True),
     ((32, 64, 2), dict(persistent=True, waves_per_simd=4, gjk_inline=True), True),
+    ((64, 64, 1), dict(phased=True), False),
+    ((32, 16, 2), dict(phased=True), True),
 ])
 def test_fused_equals_plain_bitwise(tiles, launch, deep):
     """deep: overlapping boxes in the pile part, so the GJK/EPA pass runs (deferred to the resume launch, or inside
@@ -103,11 +105,14 @@ def test_fused_group_subrange_leaves_other_envs_alone():
     assert torch.equal(qb[:, :10], before[0][0][:, :10]) and torch.equal(qb[:, 35:], before[0][0][:, 35:])
 
 
-def test_graph_replays_match_eager_fused():
+@pytest.mark.parametrize("launch", [dict(), dict(phased=True)])
+def test_graph_replays_match_eager_fused(launch):
     from genesis_amd.batch import FusedBatch, Group
     A, B = _parts(COUNTS), _parts(COUNTS)
-    fa = FusedBatch([Group(S) if S.dm.desc.solver_variant == 0 else Group(S, envs_per_tile=2) for _, _, S, _ in A])
-    fbb = FusedBatch([Group(S) if S.dm.desc.solver_variant == 0 else Group(S, envs_per_tile=2) for _, _, S, _ in B])
+    fa = FusedBatch([Group(S) if S.dm.desc.solver_variant == 0 else Group(S, envs_per_tile=2) for _, _, S, _ in A],
+                    **launch)
+    fbb = FusedBatch([Group(S) if S.dm.desc.solver_variant == 0 else Group(S, envs_per_tile=2) for _, _, S, _ in B],
+                     **launch)
     fbb.capture(1)
     for step in range(6):
         for parts in (A, B):
