@@ -181,6 +181,42 @@ class DeviceState:
         return sum(t.numel() * t.element_size() for t in self.t.values())
 
 
+# State arrays a substep writes before it reads them (constraint rows, solver work vectors, Hessian, collision
+# scratch): a checkpoint leaves them out. Everything else (poses, velocities, controls, warm starts, the contact-normal
+# and sort caches, kinematics caches, the last step's contacts and errno) is saved.
+CHECKPOINT_SKIP = frozenset({
+    "n_broad", "broad_pairs", "active_buffer", "jac_val", "jac_idx", "jac_nnz", "efc_diag", "efc_aref", "efc_D",
+    "efc_fl", "efc_force", "efc_Jaref", "efc_jv", "efc_quad", "efc_active", "n_con", "Ma", "grad", "Mgrad", "search",
+    "mv", "qfrc_constraint", "nt_vec", "nt_H", "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_ws",
+    "gjk_defer_n", "gjk_defer", "qpos_next", "vel_next",
+})
+
+
+class RigidSolverState:
+    """Dynamic state of a RigidSolver, env-major, mirroring the reference's RigidSolverState
+    (genesis/engine/states/solvers.py:46-85). The COM/mass shifts and friction ratios are the neutral values: per-env
+    model parameters are not built."""
+
+    def __init__(self, B: int, model: RigidModel, device, s_global: int = 0):
+        f = dict(dtype=torch.float32, device=device)
+        L = model.n_links
+        self.s_global = s_global
+        self.qpos = torch.zeros((B, model.n_qs), **f)
+        self.dofs_vel = torch.zeros((B, model.n_dofs), **f)
+        self.dofs_acc = torch.zeros((B, model.n_dofs), **f)
+        self.links_pos = torch.zeros((B, L, 3), **f)
+        self.links_quat = torch.zeros((B, L, 4), **f)
+        self.i_pos_shift = torch.zeros((B, L, 3), **f)
+        self.mass_shift = torch.zeros((B, L), **f)
+        self.friction_ratio = torch.ones((B, model.n_geoms), **f)
+
+    def serializable(self):
+        for k in ("qpos", "dofs_vel", "dofs_acc", "links_pos", "links_quat", "i_pos_shift", "mass_shift",
+                  "friction_ratio"):
+            setattr(self, k, getattr(self, k).detach())
+        return self
+
+
 class RigidSolver:
     """Batched rigid solver for one scene type on one device."""
 
@@ -335,16 +371,55 @@ class RigidSolver:
         return v if envs_idx is None else v[self._idx(envs_idx, self._B)]
 
     # -- state snapshot (rsd:1442-1512) ------------------------------------------------------------------------
-    def get_state(self):
-        return {"qpos": self.st.t["qpos"].clone(), "vel": self.st.t["vel"].clone()}
+    def get_state(self, s_global: int = 0) -> "RigidSolverState":
+        """Snapshot of the dynamic state (kernel_get_state, rsd:7165-7209), env-major copies."""
+        B, L = self._B, self.model.n_links
+        st = RigidSolverState(B, self.model, self.device, s_global)
+        st.qpos.copy_(self.st.t["qpos"].T)
+        st.dofs_vel.copy_(self.st.t["vel"].T)
+        st.dofs_acc.copy_(self.st.t["acc"].T)
+        st.links_pos.copy_(self.st.t["links_pos"].T.reshape(B, L, 3))
+        st.links_quat.copy_(self.st.t["links_quat"].T.reshape(B, L, 4))
+        return st
 
-    def set_state(self, state, envs_idx=None):
+    def set_state(self, state: "RigidSolverState", envs_idx=None):
+        """kernel_set_state (rsd:7212-7258) for the listed envs: qpos, dofs velocity and acceleration, controls back to
+        zero force; then forward kinematics of the new poses and cleared collider / warm-start caches and errno
+        (rsd:1478-1512). Link poses follow from qpos through the kinematics, as in the reference."""
         e = self._idx(envs_idx, self._B)
-        self.st.t["qpos"][:, e] = state["qpos"][:, e]
-        self.st.t["vel"][:, e] = state["vel"][:, e]
-        ei = e.to(torch.int32)
-        self.reset_caches(ei)
+        if bool((state.i_pos_shift != 0).any()) or bool((state.mass_shift != 0).any()) or \
+                bool((state.friction_ratio != 1).any()):
+            raise NotImplementedError("per-env COM, mass and friction shifts are not built")
+        t = self.st.t
+        dev = self.device
+        ed = e.to(dev)
+        t["qpos"][:, ed] = state.qpos.to(dev, torch.float32)[ed].T
+        t["vel"][:, ed] = state.dofs_vel.to(dev, torch.float32)[ed].T
+        t["acc"][:, ed] = state.dofs_acc.to(dev, torch.float32)[ed].T
+        t["ctrl_force"][:, ed] = 0.0
+        t["ctrl_mode"][:, ed] = CTRL_FORCE
+        ei = ed.to(torch.int32)
         self.forward_kinematics(ei)
+        self.reset_caches(ei)
+
+    # -- checkpoint (scene.py:1283-1344) -------------------------------------------------------------------------
+    def checkpoint_arrays(self):
+        """Host copies of every state array that outlives a substep (the rest is rewritten by each substep before
+        it is read), keyed "RigidSolver.<field>", like the reference's dump_ckpt_to_numpy."""
+        return {f"RigidSolver.{k}": v.detach().cpu().numpy() for k, v in self.st.t.items()
+                if k not in CHECKPOINT_SKIP}
+
+    def load_checkpoint_arrays(self, arrays):
+        t = self.st.t
+        for k, v in t.items():
+            if k in CHECKPOINT_SKIP:
+                continue
+            a = arrays.get(f"RigidSolver.{k}")
+            if a is None:
+                raise KeyError(f"checkpoint has no array for state field {k}")
+            if tuple(a.shape) != tuple(v.shape) or np.dtype(a.dtype) != np.dtype(str(v.dtype).replace("torch.", "")):
+                raise ValueError(f"checkpoint array {k}: {a.dtype}{tuple(a.shape)} does not fit {v.dtype}{tuple(v.shape)}")
+            v.copy_(torch.from_numpy(np.ascontiguousarray(a)).to(v.device))
 
     # -- contact and constraint readback (col:463-565, cs:235-322, rigid_entity.py:2838-2850) ---------------------
     @property

@@ -9,6 +9,7 @@ classes keep the reference's names and defaults for the fields this path reads (
 from __future__ import annotations
 
 import os
+import time
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -329,6 +330,7 @@ class Scene:
         for e in self.entities:
             e._bind(self.model)
         self._init_qpos = self.rigid_solver.st.t["qpos"].clone()
+        self._init_state = None
         self.is_built = True
 
     @property
@@ -341,12 +343,15 @@ class Scene:
         self.t += 1
 
     def get_state(self):
-        return self.rigid_solver.get_state()
+        return self.rigid_solver.get_state(self.t)
 
     def reset(self, state=None, envs_idx=None):
-        """Back to the build-time state (or `state`), clearing contact caches and the solver warm start."""
+        """Back to the registered initial state (scene.py:857-885): the build-time state, or the last `state` given,
+        which becomes the new initial state; contact caches, the solver warm start and errno are cleared."""
         S = self.rigid_solver
-        if state is None:
+        if state is not None:
+            self._init_state = state
+        if self._init_state is None:
             e = S._idx(envs_idx, S._B)
             S.st.init_state(e)
             S.st.t["qpos"][:, e] = self._init_qpos[:, e]
@@ -354,6 +359,24 @@ class Scene:
             S.reset_caches(ei)
             S.forward_kinematics(ei)
         else:
-            S.set_state(state, envs_idx)
-        if envs_idx is None:
-            self.t = 0
+            S.set_state(self._init_state, envs_idx)
+        self.t = 0
+
+    # -- checkpoint (scene.py:1283-1344) ---------------------------------------------------------------------------
+    def dump_ckpt_to_numpy(self):
+        """Flat {"RigidSolver.<field>": array} of every state array that outlives a substep."""
+        return self.rigid_solver.checkpoint_arrays()
+
+    def save_checkpoint(self, path):
+        """Write the physics state to one .npz file: the arrays, `step_index` and `timestamp`. The reference pickles
+        the same dictionary; an npz holds no code, so loading it runs nothing from the file."""
+        arrays = self.dump_ckpt_to_numpy()
+        with open(path, "wb") as f:
+            np.savez(f, __step_index__=np.int64(self.t), __timestamp__=np.float64(time.time()), **arrays)
+
+    def load_checkpoint(self, path):
+        """Restore a file written by save_checkpoint into this built scene (same entities and n_envs)."""
+        with np.load(path, allow_pickle=False) as z:
+            arrays = {k: z[k] for k in z.files}
+        self.rigid_solver.load_checkpoint_arrays(arrays)
+        self.t = int(arrays.get("__step_index__", self.t))

@@ -51,6 +51,14 @@ def test_null_arguments_are_rejected_without_gpu(lib):
     assert lib.gsr_get_equalities(None, None, None, None, None) == -1
 
 
+def test_checkpoint_skips_only_state_fields():
+    from genesis_amd.rigid_solver import CHECKPOINT_SKIP
+    assert CHECKPOINT_SKIP <= set(_lib.STATE_POINTERS)
+    # the persistent state and caches are saved
+    assert not {"qpos", "vel", "qacc_ws", "is_warmstart", "ctrl_mode", "ctrl_pos", "ctrl_vel", "ctrl_force",
+                "cache_normal", "sort_value", "sort_ig", "first_time", "links_pos", "links_quat"} & CHECKPOINT_SKIP
+
+
 def test_state_size_rules_cover_every_pointer():
     from genesis_amd.rigid_solver import finalize_model, state_sizes
     for w in (W.arm_workload(), W.quadruped_workload()):
