@@ -421,6 +421,134 @@ class RigidSolver:
                 raise ValueError(f"checkpoint array {k}: {a.dtype}{tuple(a.shape)} does not fit {v.dtype}{tuple(v.shape)}")
             v.copy_(torch.from_numpy(np.ascontiguousarray(a)).to(v.device))
 
+    # -- model parameters (rsd:1777-2010 setters, 2334-2452 getters) ---------------------------------------------
+    # One value per dof / link / geom shared by every env: the reference's default batch_dofs_info =
+    # batch_links_info = False (options/solvers.py:373-375). The host model arrays and their device tables are updated
+    # together (in place, so every descriptor holding the table pointers sees the new values at the next launch).
+    def _model_write(self, name, values, idx, n, width=0):
+        arr = getattr(self.model, name)
+        i = self._idx(idx, n, device="cpu").numpy()
+        if ((i < 0) | (i >= n)).any():
+            raise IndexError(f"{name}: index out of range [0, {n})")
+        v = np.asarray(values.detach().cpu() if torch.is_tensor(values) else values, np.float64)
+        v = v.reshape((len(i), width) if width else (len(i),))
+        arr[i] = v.astype(arr.dtype)
+        self.dm.tensors[name].copy_(torch.from_numpy(np.ascontiguousarray(arr, np.float32).ravel()))
+
+    def _model_read(self, name, idx, n):
+        t = self.dm.tensors[name]
+        w = t.numel() // max(n, 1)
+        t = t.reshape(n, w) if w > 1 else t
+        i = self._idx(idx, n)
+        if bool(((i < 0) | (i >= n)).any()):
+            raise IndexError(f"{name}: index out of range [0, {n})")
+        return t[i].clone()
+
+    @staticmethod
+    def _no_envs(envs_idx, what):
+        if envs_idx is not None:
+            raise GenesisException(f"`envs_idx` cannot be specified for non-batched {what} info "
+                                   f"(batched per-env {what} parameters are not built).")
+
+    def _set_dofs(self, name, values, dofs_idx, envs_idx, width=0):
+        self._no_envs(envs_idx, "dofs")
+        self._model_write(name, values, dofs_idx, self.model.n_dofs, width)
+
+    def set_dofs_kp(self, kp, dofs_idx=None, envs_idx=None):
+        self._set_dofs("dof_kp", kp, dofs_idx, envs_idx)
+
+    def set_dofs_kv(self, kv, dofs_idx=None, envs_idx=None):
+        self._set_dofs("dof_kv", kv, dofs_idx, envs_idx)
+
+    def set_dofs_stiffness(self, stiffness, dofs_idx=None, envs_idx=None):
+        self._set_dofs("dof_stiffness", stiffness, dofs_idx, envs_idx)
+
+    def set_dofs_damping(self, damping, dofs_idx=None, envs_idx=None):
+        self._set_dofs("dof_damping", damping, dofs_idx, envs_idx)
+
+    def set_dofs_frictionloss(self, frictionloss, dofs_idx=None, envs_idx=None):
+        self._set_dofs("dof_frictionloss", frictionloss, dofs_idx, envs_idx)
+
+    def set_dofs_force_range(self, lower, upper, dofs_idx=None, envs_idx=None):
+        lo, hi = (torch.as_tensor(x, dtype=torch.float64).reshape(-1).cpu() for x in (lower, upper))
+        self._set_dofs("dof_force_range", torch.stack([lo, hi], -1), dofs_idx, envs_idx, 2)
+
+    def set_dofs_limit(self, lower, upper, dofs_idx=None, envs_idx=None):
+        lo, hi = (torch.as_tensor(x, dtype=torch.float64).reshape(-1).cpu() for x in (lower, upper))
+        self._set_dofs("dof_limit", torch.stack([lo, hi], -1), dofs_idx, envs_idx, 2)
+
+    def set_dofs_armature(self, armature, dofs_idx=None, envs_idx=None):
+        """Armature enters the mass matrix, so the invweights and meaninertia are recomputed at the neutral pose
+        (rsd:1998, 1964-1969 -> _init_invweight_and_meaninertia :354-507); the envs keep their qpos and lose their
+        contact and warm-start caches, as through set_qpos."""
+        from .model import _init_invweight_and_meaninertia
+        self._set_dofs("dof_armature", armature, dofs_idx, envs_idx)
+        _init_invweight_and_meaninertia(self.model)
+        for k in ("dof_invweight", "link_invweight"):
+            self.dm.tensors[k].copy_(torch.from_numpy(np.ascontiguousarray(getattr(self.model, k), np.float32).ravel()))
+        self.dm.desc.meaninertia = float(self.model.meaninertia)
+        self.reset_caches()
+        self.forward_kinematics()
+
+    def set_links_inertial_mass(self, mass, links_idx=None, envs_idx=None):
+        """Link masses enter the COM and composite inertias that the end-of-step kinematics keep for the next step,
+        so those are recomputed here for every env."""
+        self._no_envs(envs_idx, "links")
+        self._model_write("link_inertial_mass", mass, links_idx, self.model.n_links)
+        self.forward_kinematics()
+
+    def set_geoms_friction(self, friction, geoms_idx=None):
+        self._model_write("geom_friction", friction, geoms_idx, self.model.n_geoms)
+
+    def get_dofs_kp(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read("dof_kp", dofs_idx, self.model.n_dofs)
+
+    def get_dofs_kv(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read("dof_kv", dofs_idx, self.model.n_dofs)
+
+    def get_dofs_stiffness(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read("dof_stiffness", dofs_idx, self.model.n_dofs)
+
+    def get_dofs_damping(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read("dof_damping", dofs_idx, self.model.n_dofs)
+
+    def get_dofs_frictionloss(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read("dof_frictionloss", dofs_idx, self.model.n_dofs)
+
+    def get_dofs_armature(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read("dof_armature", dofs_idx, self.model.n_dofs)
+
+    def get_dofs_invweight(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read("dof_invweight", dofs_idx, self.model.n_dofs)
+
+    def get_dofs_force_range(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        t = self._model_read("dof_force_range", dofs_idx, self.model.n_dofs)
+        return t[..., 0], t[..., 1]
+
+    def get_dofs_limit(self, dofs_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "dofs")
+        t = self._model_read("dof_limit", dofs_idx, self.model.n_dofs)
+        return t[..., 0], t[..., 1]
+
+    def get_links_inertial_mass(self, links_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "links")
+        return self._model_read("link_inertial_mass", links_idx, self.model.n_links)
+
+    def get_links_invweight(self, links_idx=None, envs_idx=None):
+        self._no_envs(envs_idx, "links")
+        return self._model_read("link_invweight", links_idx, self.model.n_links)
+
+    def get_geoms_friction(self, geoms_idx=None):
+        return self._model_read("geom_friction", geoms_idx, self.model.n_geoms)
+
     # -- contact and constraint readback (col:463-565, cs:235-322, rigid_entity.py:2838-2850) ---------------------
     @property
     def collider(self) -> "Collider":

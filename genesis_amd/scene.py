@@ -249,6 +249,66 @@ class RigidEntity:
     def get_quat(self, envs_idx=None):
         return self.get_links_quat(envs_idx)[..., 0, :]
 
+    # model parameters of the entity's dofs / links (rigid_entity.py set_dofs_kp ... get_links_inertial_mass)
+    def _dofs(self, dofs_idx_local):
+        return self._global(dofs_idx_local, self.n_dofs, self.dof_start)
+
+    def set_dofs_kp(self, kp, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_kp(kp, self._dofs(dofs_idx_local), envs_idx)
+
+    def set_dofs_kv(self, kv, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_kv(kv, self._dofs(dofs_idx_local), envs_idx)
+
+    def set_dofs_force_range(self, lower, upper, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_force_range(lower, upper, self._dofs(dofs_idx_local), envs_idx)
+
+    def set_dofs_limit(self, lower, upper, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_limit(lower, upper, self._dofs(dofs_idx_local), envs_idx)
+
+    def set_dofs_stiffness(self, stiffness, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_stiffness(stiffness, self._dofs(dofs_idx_local), envs_idx)
+
+    def set_dofs_damping(self, damping, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_damping(damping, self._dofs(dofs_idx_local), envs_idx)
+
+    def set_dofs_frictionloss(self, frictionloss, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_frictionloss(frictionloss, self._dofs(dofs_idx_local), envs_idx)
+
+    def set_dofs_armature(self, armature, dofs_idx_local=None, envs_idx=None):
+        self._solver.set_dofs_armature(armature, self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_kp(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_kp(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_kv(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_kv(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_force_range(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_force_range(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_limit(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_limit(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_stiffness(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_stiffness(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_damping(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_damping(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_frictionloss(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_frictionloss(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_dofs_armature(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_armature(self._dofs(dofs_idx_local), envs_idx)
+
+    def get_links_inertial_mass(self, links_idx_local=None, envs_idx=None):
+        return self._solver.get_links_inertial_mass(self._global(links_idx_local, self.n_links, self.link_start),
+                                                    envs_idx)
+
+    def set_links_inertial_mass(self, mass, links_idx_local=None, envs_idx=None):
+        self._solver.set_links_inertial_mass(mass, self._global(links_idx_local, self.n_links, self.link_start),
+                                             envs_idx)
+
     def get_mass(self):
         """Total mass of the entity's links (rigid_entity.py:2965)."""
         return float(self._scene.model.link_inertial_mass[self.link_start:self.link_end].astype(np.float64).sum())
