@@ -120,7 +120,9 @@ __device__ inline void dev_add_contact(const Env& e, int ga, int gb, f3 normal, 
     e.S3(s.contact_normal, i, normal);
     e.S3(s.contact_pos, i, pos);
     e.F(s.contact_pen, i) = pen;
-    float fa = m.geom_friction[ga] * 1.f, fb = m.geom_friction[gb] * 1.f;
+    // per-env friction ratios (collider_decomp.py:2161-2162, 2200-2201)
+    float fa = m.geom_friction[ga] * e.F(s.friction_ratio, ga);
+    float fb = m.geom_friction[gb] * e.F(s.friction_ratio, gb);
     e.F(s.contact_friction, i) = fmaxf(fmaxf(fa, fb), 1e-2f);
     e.I(s.n_contacts, 0) = i + 1;
   } else {

@@ -196,7 +196,8 @@ __device__ inline int wave_narrow_pass(const Env& e, int lane, int nb, bool plan
       e.I(s.gjk_defer, 2 * slot + 1) = min(k, m.max_contacts);
     }
     n_def += __popcll(def);
-    float fr = fmaxf(fmaxf(m.geom_friction[ga] * 1.f, m.geom_friction[gb] * 1.f), 1e-2f);
+    float fr = fmaxf(fmaxf(m.geom_friction[ga] * e.F(s.friction_ratio, ga),
+                           m.geom_friction[gb] * e.F(s.friction_ratio, gb)), 1e-2f);
     for (int c = 0; c < pc.cnt; c++, k++) {
       if (k < m.max_contacts) {
         e.I(s.contact_geoms, 2 * k) = ga;
@@ -336,7 +337,8 @@ __device__ inline bool wave_gjk_fixup(const Env& e) {
     }
     if (lane == 0) {
       if (n + cnt > m.max_contacts) e.I((int32_t*)s.errno_env, 0) |= (int)GSR_ERR_CONTACT_OVERFLOW;
-      const float fr = fmaxf(fmaxf(m.geom_friction[ga] * 1.f, m.geom_friction[gb] * 1.f), 1e-2f);
+      const float fr = fmaxf(fmaxf(m.geom_friction[ga] * e.F(s.friction_ratio, ga),
+                                   m.geom_friction[gb] * e.F(s.friction_ratio, gb)), 1e-2f);
       for (int c = 0; c < cnt; c++) {
         const int t = at + c;
         if (t >= kept) break;

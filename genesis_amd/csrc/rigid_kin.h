@@ -90,10 +90,11 @@ __device__ inline void dev_com_links(const Env& e, int e0 = 0, int es = 1) {
     e.F(s.crb_mass, l) = 0.f;
   }
   GSR_FOR_ENT_LINKS(ie, l) {
-    float mass = m.link_inertial_mass[l];
+    // per-env mass and COM shifts (rsd:4812-4817)
+    float mass = m.link_inertial_mass[l] + e.F(s.mass_shift, l);
     f3 lp = e.L3(s.links_pos, l);
     q4 lq = e.L4(s.links_quat, l);
-    f3 ipos_bw = lp + qrot(T3(m.link_inertial_pos, l), lq);
+    f3 ipos_bw = lp + qrot(T3(m.link_inertial_pos, l) + e.L3(s.i_pos_shift, l), lq);
     e.S4(s.links_iquat, l, qrot_q(T4(m.link_inertial_quat, l), lq));
     e.S3(s.crb_pos, l, ipos_bw);
     int r = m.link_root[l];
@@ -108,7 +109,7 @@ __device__ inline void dev_com_links(const Env& e, int e0 = 0, int es = 1) {
   }
   GSR_FOR_ENT_LINKS(ie, l) {
     f3 t = e.L3(s.crb_pos, l) - e.L3(s.root_com, l);  // i_pos
-    float mass = m.link_inertial_mass[l];
+    float mass = m.link_inertial_mass[l] + e.F(s.mass_shift, l);  // rsd:4910
     float x = t.x, y = t.y, z = t.z;
     float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
     float hhT[9] = {yy + zz, -xy, -xz, -xy, xx + zz, -yz, -xz, -yz, xx + yy};

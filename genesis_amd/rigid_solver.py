@@ -78,7 +78,7 @@ def state_sizes(M: RigidModel) -> dict:
         qpos=Q, vel=D, acc=D, acc_smooth=D, qacc_ws=D, is_warmstart=1, ctrl_mode=D, ctrl_pos=D, ctrl_vel=D,
         ctrl_force=D, dofs_pos=D, links_pos=3 * L, links_quat=4 * L, geoms_pos=3 * G, geoms_quat=4 * G,
         aabb_min=3 * G, aabb_max=3 * G, cache_normal=3 * P, sort_value=2 * G, sort_ig=2 * G, first_time=1,
-        errno_env=1,
+        errno_env=1, mass_shift=L, i_pos_shift=3 * L, friction_ratio=G,
         cdof_ang=3 * D, cdof_vel=3 * D, cdofd_ang=3 * D, cdofd_vel=3 * D, root_com=3 * L, cd_vel=3 * L,
         cd_ang=3 * L, cinr_inertial=9 * L, cinr_pos=3 * L, cinr_mass=L, links_iquat=4 * L,
         mass_blk=M.mass_blk_size, mass_L_blk=M.mass_blk_size, mass_D_inv=D, qf_smooth=D, force=D,
@@ -176,6 +176,9 @@ class DeviceState:
         self.t["is_warmstart"][:, sl] = 0
         self.t["first_time"][:, sl] = 1
         self.t["errno_env"][:, sl] = 0
+        self.t["mass_shift"][:, sl] = 0
+        self.t["i_pos_shift"][:, sl] = 0
+        self.t["friction_ratio"][:, sl] = 1
 
     def memory_bytes(self):
         return sum(t.numel() * t.element_size() for t in self.t.values())
@@ -380,22 +383,27 @@ class RigidSolver:
         st.dofs_acc.copy_(self.st.t["acc"].T)
         st.links_pos.copy_(self.st.t["links_pos"].T.reshape(B, L, 3))
         st.links_quat.copy_(self.st.t["links_quat"].T.reshape(B, L, 4))
+        st.i_pos_shift.copy_(self.st.t["i_pos_shift"].T.reshape(B, L, 3))
+        st.mass_shift.copy_(self.st.t["mass_shift"].T)
+        st.friction_ratio.copy_(self.st.t["friction_ratio"].T)
         return st
 
     def set_state(self, state: "RigidSolverState", envs_idx=None):
-        """kernel_set_state (rsd:7212-7258) for the listed envs: qpos, dofs velocity and acceleration, controls back to
-        zero force; then forward kinematics of the new poses and cleared collider / warm-start caches and errno
-        (rsd:1478-1512). Link poses follow from qpos through the kinematics, as in the reference."""
+        """kernel_set_state (rsd:7212-7258) for the listed envs: qpos, dofs velocity and acceleration, COM / mass
+        shifts and friction ratios, controls back to zero force; then forward kinematics of the new poses and cleared
+        collider / warm-start caches and errno (rsd:1478-1512). Link poses follow from qpos through the kinematics, as
+        in the reference."""
         e = self._idx(envs_idx, self._B)
-        if bool((state.i_pos_shift != 0).any()) or bool((state.mass_shift != 0).any()) or \
-                bool((state.friction_ratio != 1).any()):
-            raise NotImplementedError("per-env COM, mass and friction shifts are not built")
         t = self.st.t
+        B, L = self._B, self.model.n_links
         dev = self.device
         ed = e.to(dev)
         t["qpos"][:, ed] = state.qpos.to(dev, torch.float32)[ed].T
         t["vel"][:, ed] = state.dofs_vel.to(dev, torch.float32)[ed].T
         t["acc"][:, ed] = state.dofs_acc.to(dev, torch.float32)[ed].T
+        t["i_pos_shift"][:, ed] = state.i_pos_shift.to(dev, torch.float32).reshape(B, 3 * L)[ed].T
+        t["mass_shift"][:, ed] = state.mass_shift.to(dev, torch.float32)[ed].T
+        t["friction_ratio"][:, ed] = state.friction_ratio.to(dev, torch.float32)[ed].T
         t["ctrl_force"][:, ed] = 0.0
         t["ctrl_mode"][:, ed] = CTRL_FORCE
         ei = ed.to(torch.int32)
@@ -548,6 +556,47 @@ class RigidSolver:
 
     def get_geoms_friction(self, geoms_idx=None):
         return self._model_read("geom_friction", geoms_idx, self.model.n_geoms)
+
+    # -- per-env shifts (rsd:1755-1799 setters, 2326-2350 getters): links_state / geoms_state, (n, B) like all state
+    def _write_env(self, name, values, idx, n, envs_idx, width=1):
+        i = self._idx(idx, n)
+        if bool(((i < 0) | (i >= n)).any()):
+            raise IndexError(f"{name}: index out of range [0, {n})")
+        e = self._idx(envs_idx, self._B)
+        v = torch.as_tensor(values, dtype=torch.float32, device=self.device).reshape(len(e), len(i), width)
+        rows = (i[:, None] * width + torch.arange(width, device=self.device)[None]).reshape(-1)
+        self.st.t[name][rows[:, None], e[None, :]] = v.reshape(len(e), -1).T
+        return e
+
+    def _read_env(self, name, idx, n, envs_idx, width=1):
+        i = self._idx(idx, n)
+        rows = (i[:, None] * width + torch.arange(width, device=self.device)[None]).reshape(-1)
+        v = self.st.t[name][rows][:, self._idx(envs_idx, self._B)].T.reshape(-1, len(i), width)
+        v = v[..., 0] if width == 1 else v
+        return v[0] if self.n_envs == 0 else v
+
+    def set_links_mass_shift(self, mass, links_idx=None, envs_idx=None):
+        """Added to each listed link's mass in the listed envs; the COM caches are recomputed for those envs."""
+        e = self._write_env("mass_shift", mass, links_idx, self.model.n_links, envs_idx)
+        self.forward_kinematics(e.to(torch.int32))
+
+    def set_links_COM_shift(self, com, links_idx=None, envs_idx=None):
+        """Added to each listed link's inertial position (link frame) in the listed envs."""
+        e = self._write_env("i_pos_shift", com, links_idx, self.model.n_links, envs_idx, 3)
+        self.forward_kinematics(e.to(torch.int32))
+
+    def set_geoms_friction_ratio(self, friction_ratio, geoms_idx=None, envs_idx=None):
+        """Scales each listed geom's friction in the listed envs (read when contacts are created)."""
+        self._write_env("friction_ratio", friction_ratio, geoms_idx, self.model.n_geoms, envs_idx)
+
+    def get_links_mass_shift(self, links_idx=None, envs_idx=None):
+        return self._read_env("mass_shift", links_idx, self.model.n_links, envs_idx)
+
+    def get_links_COM_shift(self, links_idx=None, envs_idx=None):
+        return self._read_env("i_pos_shift", links_idx, self.model.n_links, envs_idx, 3)
+
+    def get_geoms_friction_ratio(self, geoms_idx=None, envs_idx=None):
+        return self._read_env("friction_ratio", geoms_idx, self.model.n_geoms, envs_idx)
 
     # -- contact and constraint readback (col:463-565, cs:235-322, rigid_entity.py:2838-2850) ---------------------
     @property

@@ -309,6 +309,32 @@ class RigidEntity:
         self._solver.set_links_inertial_mass(mass, self._global(links_idx_local, self.n_links, self.link_start),
                                              envs_idx)
 
+    # per-env shifts (rigid_entity.py:2852-2935)
+    def set_friction_ratio(self, friction_ratio, links_idx_local=None, envs_idx=None):
+        """Friction ratio per listed link, applied to every geom of the link (shape (n_envs, n_links))."""
+        links = self._global(links_idx_local, self.n_links, self.link_start).tolist()
+        gl = self._scene.model.geom_link
+        geoms = [[g for g in range(len(gl)) if gl[g] == l] for l in links]
+        counts = torch.tensor([len(g) for g in geoms])
+        ratio = torch.as_tensor(friction_ratio, dtype=torch.float32).cpu()
+        ratio = ratio.reshape(-1, len(links)) if ratio.dim() > 1 or self._solver.n_envs else ratio.reshape(1, -1)
+        self._solver.set_geoms_friction_ratio(torch.repeat_interleave(ratio, counts, dim=-1),
+                                              [g for gs_ in geoms for g in gs_], envs_idx)
+
+    def set_friction(self, friction):
+        """Friction coefficient of every geom of the entity (shared by all envs)."""
+        gl = self._scene.model.geom_link
+        geoms = [g for g in range(len(gl)) if self.link_start <= gl[g] < self.link_end]
+        self._solver.set_geoms_friction(np.full(len(geoms), float(friction)), geoms)
+
+    def set_mass_shift(self, mass_shift, links_idx_local=None, envs_idx=None):
+        self._solver.set_links_mass_shift(mass_shift, self._global(links_idx_local, self.n_links, self.link_start),
+                                          envs_idx)
+
+    def set_COM_shift(self, com_shift, links_idx_local=None, envs_idx=None):
+        self._solver.set_links_COM_shift(com_shift, self._global(links_idx_local, self.n_links, self.link_start),
+                                         envs_idx)
+
     def get_mass(self):
         """Total mass of the entity's links (rigid_entity.py:2965)."""
         return float(self._scene.model.link_inertial_mass[self.link_start:self.link_end].astype(np.float64).sum())

@@ -98,6 +98,8 @@ typedef struct gsr_state {
   float *cache_normal, *sort_value;
   int32_t *sort_ig, *first_time;
   uint32_t *errno_env;
+  /* per-env model shifts: links_state.mass_shift (L), links_state.i_pos_shift (L x 3), geoms_state.friction_ratio (G) */
+  float *mass_shift, *i_pos_shift, *friction_ratio;
   /* kinematics / dynamics intermediates */
   float *cdof_ang, *cdof_vel, *cdofd_ang, *cdofd_vel, *root_com, *cd_vel, *cd_ang, *cinr_inertial, *cinr_pos;
   float *cinr_mass, *links_iquat, *mass_blk, *mass_L_blk, *mass_D_inv, *qf_smooth, *force;

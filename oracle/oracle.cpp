@@ -154,6 +154,8 @@ struct Env {
   std::vector<int32_t> ctrl_mode;
   std::vector<T> ctrl_pos, ctrl_vel, ctrl_force;
   std::vector<V3<T>> cache_normal;  // (P)
+  std::vector<T> mass_shift, friction_ratio;  // links_state.mass_shift (L), geoms_state.friction_ratio (G)
+  std::vector<V3<T>> i_pos_shift;             // links_state.i_pos_shift (L)
   std::vector<T> sort_value;
   std::vector<int32_t> sort_ig, sort_ismax;
   int first_time = 1;
@@ -194,6 +196,7 @@ struct Env {
     qpos.assign(m.Q, 0); vel.assign(D, 0); qacc_ws.assign(D, 0);
     ctrl_mode.assign(D, CTRL_FORCE); ctrl_pos.assign(D, 0); ctrl_vel.assign(D, 0); ctrl_force.assign(D, 0);
     cache_normal.assign(std::max(m.P, 1), V3<T>());
+    mass_shift.assign(m.L, 0); i_pos_shift.assign(m.L, V3<T>()); friction_ratio.assign(m.G, 1);
     sort_value.assign(2 * G, 0); sort_ig.assign(2 * G, 0); sort_ismax.assign(2 * G, 0);
     for (auto* v : {&l_pos, &l_ipos_bw, &l_ipos, &root_com, &root_com_bw, &j_pos, &cd_vel, &cd_ang, &cdd_vel, &cdd_ang,
                     &cinr_pos, &cfrc_vel, &cfrc_ang, &crb_pos})
@@ -290,8 +293,9 @@ void com_links(const Model<T>& m, Env<T>& e) {
   const T EPS = m.EPS;
   for (int l = 0; l < m.L; l++) { e.root_com_bw[l] = V3<T>(); e.mass_sum[l] = 0; }
   for (int l = 0; l < m.L; l++) {
-    T mass = m.link_inertial_mass[l];
-    e.l_ipos_bw[l] = e.l_pos[l] + transform_by_quat(m.link_inertial_pos[l], e.l_quat[l]);
+    // per-env mass and COM shifts (rsd:4812-4817)
+    T mass = m.link_inertial_mass[l] + e.mass_shift[l];
+    e.l_ipos_bw[l] = e.l_pos[l] + transform_by_quat(m.link_inertial_pos[l] + e.i_pos_shift[l], e.l_quat[l]);
     e.l_iquat[l] = transform_quat_by_quat(m.link_inertial_quat[l], e.l_quat[l]);
     int r = m.link_root[l];
     e.mass_sum[r] += mass;
@@ -304,7 +308,7 @@ void com_links(const Model<T>& m, Env<T>& e) {
     e.l_ipos[l] = e.l_ipos_bw[l] - e.root_com[l];
     // ti_transform_inertia_by_trans_quat (geom.py:293-309)
     V3<T> t = e.l_ipos[l];
-    T mass = m.link_inertial_mass[l];
+    T mass = m.link_inertial_mass[l] + e.mass_shift[l];  // rsd:4910
     T x = t.x, y = t.y, z = t.z;
     T xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
     M3<T> hhT;
@@ -653,7 +657,8 @@ void add_contact(const Model<T>& m, Env<T>& e, int ga, int gb, V3<T> normal, V3<
   int i = e.n_contacts;
   if (i < m.max_contacts) {
     Contact<T>& c = e.contacts[i];
-    T fa = m.geom_friction[ga] * T(1), fb = m.geom_friction[gb] * T(1);
+    // per-env friction ratios (collider_decomp.py:2161-2162)
+    T fa = m.geom_friction[ga] * e.friction_ratio[ga], fb = m.geom_friction[gb] * e.friction_ratio[gb];
     c.geom_a = ga; c.geom_b = gb; c.normal = normal; c.pos = pos; c.penetration = pen;
     c.friction = std::max(std::max(fa, fb), T(1e-2));
     for (int k = 0; k < 7; k++) c.sol_params[k] = T(0.5) * (m.geom_sol_params[7 * ga + k] + m.geom_sol_params[7 * gb + k]);
@@ -2168,7 +2173,8 @@ extern "C" void orc_build(void* hp, int B, int precision) {
 }
 
 // Generic state access: field names below; arrays are env-major (B, n) float64 / int32.
-#define ORC_FIELDS_F(X) X(qpos) X(vel) X(qacc_ws) X(ctrl_pos) X(ctrl_vel) X(ctrl_force) X(acc) X(acc_smooth) X(dofs_pos) X(sort_value)
+#define ORC_FIELDS_F(X) X(qpos) X(vel) X(qacc_ws) X(ctrl_pos) X(ctrl_vel) X(ctrl_force) X(acc) X(acc_smooth) X(dofs_pos) X(sort_value) \
+  X(mass_shift) X(friction_ratio)
 #define ORC_FIELDS_I(X) X(ctrl_mode) X(sort_ig) X(sort_ismax)
 
 template <class T>
@@ -2213,6 +2219,7 @@ static void get_set_env(Env<T>& e, const Model<T>& m, const char* name, double* 
     }
   };
   if (n == "links_pos") return v3(e.l_pos);
+  if (n == "i_pos_shift") return v3(e.i_pos_shift);
   if (n == "links_quat") return q4(e.l_quat);
   if (n == "geoms_pos") return v3(e.g_pos);
   if (n == "geoms_quat") return q4(e.g_quat);

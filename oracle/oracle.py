@@ -57,7 +57,7 @@ FLOAT_TABLES = [
 
 # state field -> (kind, per-env shape builder)
 FIELDS_F = {"qpos": "Q", "vel": "D", "qacc_ws": "D", "ctrl_pos": "D", "ctrl_vel": "D", "ctrl_force": "D", "acc": "D",
-            "acc_smooth": "D", "dofs_pos": "D", "sort_value": "2G"}
+            "acc_smooth": "D", "dofs_pos": "D", "sort_value": "2G", "mass_shift": "L", "friction_ratio": "G"}
 FIELDS_I = {"ctrl_mode": "D", "sort_ig": "2G", "sort_ismax": "2G"}
 SCALAR_I = ["is_warmstart", "first_time", "errno", "n_contacts", "n_broad", "n_con", "n_newton_iters", "n_con_eq",
             "n_con_fl"]
@@ -130,7 +130,7 @@ class Oracle:
         envs = list(range(self.B)) if envs is None else list(envs)
         L = lib()
         m = self.m
-        shapes = {"links_pos": (m.n_links, 3), "links_quat": (m.n_links, 4), "geoms_pos": (m.n_geoms, 3),
+        shapes = {"links_pos": (m.n_links, 3), "i_pos_shift": (m.n_links, 3), "links_quat": (m.n_links, 4), "geoms_pos": (m.n_geoms, 3),
                   "geoms_quat": (m.n_geoms, 4), "aabb_min": (m.n_geoms, 3), "aabb_max": (m.n_geoms, 3),
                   "cdof_ang": (m.n_dofs, 3), "cdof_vel": (m.n_dofs, 3), "cd_vel": (m.n_links, 3),
                   "cd_ang": (m.n_links, 3), "root_com": (m.n_links, 3), "mass_mat": (m.n_dofs, m.n_dofs),

@@ -19,7 +19,8 @@ def gpu_fields(solver, envs=None):
         ctrl_vel=g("ctrl_vel").astype(np.float64), ctrl_force=g("ctrl_force").astype(np.float64),
         cache_normal=g("cache_normal").astype(np.float64), sort_value=g("sort_value").astype(np.float64),
         sort_ig=(sort_ig & (MAXB - 1)).astype(np.int32), sort_ismax=((sort_ig & MAXB) != 0).astype(np.int32),
-        first_time=g("first_time")[:, 0],
+        first_time=g("first_time")[:, 0], mass_shift=g("mass_shift").astype(np.float64),
+        i_pos_shift=g("i_pos_shift").astype(np.float64), friction_ratio=g("friction_ratio").astype(np.float64),
     )
 
 

@@ -100,6 +100,13 @@ def test_get_state_set_state_and_reset():
         scene.step()
     scene.reset()
     assert torch.equal(S.st.t["qpos"], st.qpos.T)
-    st.friction_ratio[0, 0] = 0.5
-    with pytest.raises(NotImplementedError):
-        S.set_state(st)
+    # per-env shifts travel with the state
+    S.set_links_mass_shift([0.5], links_idx=[2], envs_idx=[1])
+    S.set_geoms_friction_ratio([0.25], geoms_idx=[1], envs_idx=[3])
+    st2 = S.get_state()
+    assert float(st2.mass_shift[1, 2]) == 0.5 and float(st2.friction_ratio[3, 1]) == 0.25
+    scene.reset(st)
+    assert float(S.get_links_mass_shift([2], envs_idx=[1])[0, 0]) == 0.0
+    S.set_state(st2)
+    assert float(S.get_links_mass_shift([2], envs_idx=[1])[0, 0]) == 0.5
+    assert float(S.get_geoms_friction_ratio([1], envs_idx=[3])[0, 0]) == 0.25

@@ -155,6 +155,43 @@ def _brute_mass_matrix(M, o, b, eps=1e-6):
     return Mb + np.diag(M.dof_armature)
 
 
+def test_pendulum_with_mass_and_com_shifts(Oracle):
+    """Per-env mass and COM shifts (rsd:4812-4817, 4910): the point mass m + dm sits at l + dl along the arm, so
+    M = (m + dm)(l + dl)^2 (+ the 1e-6 inertia) and the gravity torque is (m + dm) g (l + dl) cos(theta); env 0 is
+    unshifted."""
+    opt = CompileOptions(dt=0.01, enable_collision=False)
+    M = compile_model([_pendulum()], opt)
+    o = Oracle(M, 3, 64)
+    th = np.array([0.3, 0.4, -1.1])
+    dm, dl = np.array([0.0, 0.5, 2.0]), np.array([0.0, 0.1, -0.2])
+    o.set("qpos", th[:, None])
+    o.set("mass_shift", np.stack([np.zeros(3), dm], 1))
+    shift = np.zeros((3, 2, 3))
+    shift[:, 1, 0] = dl
+    o.set("i_pos_shift", shift)
+    o.run(stage=2)
+    m_, l_ = 1.0 + dm, 0.5 + dl
+    np.testing.assert_allclose(o.get("mass_mat")[:, 0, 0], m_ * l_ ** 2 + 1e-6, rtol=1e-9)
+    np.testing.assert_allclose(o.get("acc_smooth")[:, 0], 9.81 * m_ * l_ * np.cos(th) / (m_ * l_ ** 2 + 1e-6),
+                               rtol=1e-6)
+    np.testing.assert_allclose(o.get("mass_shift")[:, 1], dm)
+    np.testing.assert_allclose(o.get("i_pos_shift")[:, 1, 0], dl)
+
+
+def test_friction_ratio_scales_contact_friction(Oracle):
+    """collider_decomp.py:2161-2164: a contact's friction is max(mu_a * ratio_a, mu_b * ratio_b, 1e-2) with the
+    env's ratios."""
+    M = box_scene(size=(0.2, 0.2, 0.2), pos=(0, 0, 0.09))
+    o = Oracle(M, 3, 64)
+    o.set("friction_ratio", np.array([[1.0, 1.0], [0.2, 0.3], [1e-3, 1e-3]]))
+    o.run(1)
+    mu = M.geom_friction.astype(np.float64)
+    want = [max(mu[0], mu[1]), max(0.2 * mu[0], 0.3 * mu[1]), 1e-2]
+    for b, c in enumerate(o.get("contacts")):
+        assert len(c) == 4
+        np.testing.assert_allclose(c[:, 11], want[b], rtol=1e-12)
+
+
 def test_crba_matches_brute_force_jacobians(Oracle):
     """a7 CRBA vs M = sum J^T I J with finite-difference Jacobians on the 9-DoF arm (FP64 oracle)."""
     w = W.arm_workload()
