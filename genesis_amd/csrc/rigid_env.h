@@ -23,6 +23,8 @@ struct Env {
   __device__ __forceinline__ size_t ix(int i) const { return (size_t)i * (size_t)B + (size_t)b; }
   __device__ __forceinline__ float& F(float* a, int i) const { return a[ix(i)]; }
   __device__ __forceinline__ int32_t& I(int32_t* a, int i) const { return a[ix(i)]; }
+  // per-dof parameter table entry i of this env: shared by every env, or env-minor when the model batches dofs
+  __device__ __forceinline__ float DP(const float* a, int i) const { return m.batch_dofs ? a[ix(i)] : a[i]; }
   __device__ __forceinline__ f3 L3(const float* a, int i) const {
     return f3{a[ix(3 * i)], a[ix(3 * i + 1)], a[ix(3 * i + 2)]};
   }

@@ -303,9 +303,9 @@ __device__ inline void dev_smooth_dynamics(const Env& e, int e0 = 0, int es = 1)
     // armature, then the implicit damping terms of approximate_implicitfast
     for (int d = st; d < en; d++) e.Mm(ie, d, d) = e.Mm(ie, d, d) + m.dof_armature[d];
     for (int d = st; d < en; d++) {
-      float v = e.Mm(ie, d, d) + m.dof_damping[d] * dt;
+      float v = e.Mm(ie, d, d) + e.DP(m.dof_damping, d) * dt;
       int cm = e.I(s.ctrl_mode, d);
-      if (cm == CTRL_POSITION || cm == CTRL_VELOCITY) v = v + m.dof_kv[d] * dt;
+      if (cm == CTRL_POSITION || cm == CTRL_VELOCITY) v = v + e.DP(m.dof_kv, d) * dt;
       e.Mm(ie, d, d) = v;
     }
     // --- LDL^T (rsd:3697-3760) ---
@@ -334,10 +334,12 @@ __device__ inline void dev_smooth_dynamics(const Env& e, int e0 = 0, int es = 1)
       float f = 0.f;
       int cm = e.I(s.ctrl_mode, d);
       if (cm == CTRL_FORCE) f = e.F(s.ctrl_force, d);
-      else if (cm == CTRL_VELOCITY) f = m.dof_kv[d] * (e.F(s.ctrl_vel, d) - e.F(s.vel, d));
+      else if (cm == CTRL_VELOCITY) f = e.DP(m.dof_kv, d) * (e.F(s.ctrl_vel, d) - e.F(s.vel, d));
       else if (cm == CTRL_POSITION && !(jt == JOINT_FREE && d >= dstart + 3))
-        f = m.dof_kp[d] * (e.F(s.ctrl_pos, d) - e.F(s.dofs_pos, d)) + m.dof_kv[d] * (e.F(s.ctrl_vel, d) - e.F(s.vel, d));
-      e.F(s.qf_applied, d) = fminf(fmaxf(f, m.dof_force_range[2 * d]), m.dof_force_range[2 * d + 1]);
+        f = e.DP(m.dof_kp, d) * (e.F(s.ctrl_pos, d) - e.F(s.dofs_pos, d)) +
+            e.DP(m.dof_kv, d) * (e.F(s.ctrl_vel, d) - e.F(s.vel, d));
+      e.F(s.qf_applied, d) =
+          fminf(fmaxf(f, e.DP(m.dof_force_range, 2 * d)), e.DP(m.dof_force_range, 2 * d + 1));
     }
     if (jt == JOINT_FREE && (e.I(s.ctrl_mode, dstart + 3) == CTRL_POSITION || e.I(s.ctrl_mode, dstart + 4) == CTRL_POSITION ||
                              e.I(s.ctrl_mode, dstart + 5) == CTRL_POSITION)) {
@@ -347,18 +349,19 @@ __device__ inline void dev_smooth_dynamics(const Env& e, int e0 = 0, int es = 1)
       f3 rv = quat_to_rotvec(qd, EPS);
       for (int j = 0; j < 3; j++) {
         int d = dstart + 3 + j;
-        float f = m.dof_kp[d] * comp(rv, j) - m.dof_kv[d] * e.F(s.vel, d);
-        e.F(s.qf_applied, d) = fminf(fmaxf(f, m.dof_force_range[2 * d]), m.dof_force_range[2 * d + 1]);
+        float f = e.DP(m.dof_kp, d) * comp(rv, j) - e.DP(m.dof_kv, d) * e.F(s.vel, d);
+        e.F(s.qf_applied, d) =
+          fminf(fmaxf(f, e.DP(m.dof_force_range, 2 * d)), e.DP(m.dof_force_range, 2 * d + 1));
       }
     }
   }
-  GSR_FOR_ENT_DOFS(ie3, d) e.F(s.qf_passive, d) = -m.dof_damping[d] * e.F(s.vel, d);
+  GSR_FOR_ENT_DOFS(ie3, d) e.F(s.qf_passive, d) = -e.DP(m.dof_damping, d) * e.F(s.vel, d);
   GSR_FOR_ENT_LINKS(ie4, l) {
     if (m.link_dof_end[l] - m.link_dof_start[l] <= 0) continue;
     int jt = m.joint_type[m.link_joint_start[l]];
     if (jt != JOINT_FREE && jt != JOINT_FIXED)
       for (int d = m.link_dof_start[l]; d < m.link_dof_end[l]; d++)
-        e.F(s.qf_passive, d) = e.F(s.qf_passive, d) + -e.F(s.dofs_pos, d) * m.dof_stiffness[d];
+        e.F(s.qf_passive, d) = e.F(s.qf_passive, d) + -e.F(s.dofs_pos, d) * e.DP(m.dof_stiffness, d);
   }
   // --- bias accelerations (rsd:6180-6281) ---
   for (int ie = e0; ie < E; ie += es) {

@@ -104,7 +104,7 @@ __device__ inline int dev_frictionloss_rows(const Env& e, int n_con, int& n_fl) 
   for (int l = 0; l < m.n_links; l++)
     for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++)
       for (int d = m.joint_dof_start[j]; d < m.joint_dof_end[j]; d++) {
-        if (m.dof_frictionloss[d] > m.eps) {
+        if (e.DP(m.dof_frictionloss, d) > m.eps) {
           float imp, aref;
           imp_aref(m.joint_sol_params + 7 * j, 0.f, 1.f * e.F(s.vel, d), 0.f, imp, aref);
           float diag = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
@@ -113,7 +113,7 @@ __device__ inline int dev_frictionloss_rows(const Env& e, int n_con, int& n_fl) 
           e.EF(s.efc_diag, c) = diag;
           e.EF(s.efc_aref, c) = aref;
           e.EF(s.efc_D, c) = 1.f / diag;
-          e.EF(s.efc_fl, c) = m.dof_frictionloss[d];
+          e.EF(s.efc_fl, c) = e.DP(m.dof_frictionloss, d);
           RowBuf r;
           r.n = 1; r.idx[0] = d; r.val[0] = 1.f;
           row_store(e, c, r);
@@ -182,7 +182,7 @@ __device__ inline int dev_limit_rows(const Env& e, int n_con) {
       if (jt != JOINT_REVOLUTE && jt != JOINT_PRISMATIC) continue;
       int q = m.joint_q_start[j], d = m.joint_dof_start[j];
       float qv = e.F(s.qpos, q);
-      float dmin = qv - m.dof_limit[2 * d], dmax = m.dof_limit[2 * d + 1] - qv;
+      float dmin = qv - e.DP(m.dof_limit, 2 * d), dmax = e.DP(m.dof_limit, 2 * d + 1) - qv;
       float pd = fminf(dmin, dmax);
       if (pd < 0.f) {
         float jac = (dmin < dmax) ? 1.f : -1.f;

@@ -289,6 +289,7 @@ class CompileOptions:
     enable_multi_contact: bool = True
     max_dynamic_constraints: int = 8
     n_contacts_per_pair: int = 5
+    batch_dofs_info: bool = False  # per-env kp, kv, force range, stiffness, damping, frictionloss, limits
 
 
 def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel:

@@ -105,6 +105,11 @@ def finalize_model(M: RigidModel):
     M.mass_blk_size = int((ns * ns).sum())
 
 
+# per-dof tables that batch_dofs_info makes per-env (env-minor, like the state)
+BATCHED_DOF_TABLES = ("dof_kp", "dof_kv", "dof_force_range", "dof_stiffness", "dof_damping", "dof_frictionloss",
+                      "dof_limit")
+
+
 class DeviceModel:
     """Static tables of one scene type uploaded once (torch-owned device memory) + the gsr_model descriptor."""
 
@@ -117,10 +122,14 @@ class DeviceModel:
         for k in MODEL_TABLES_I:
             a = np.ascontiguousarray(np.asarray(getattr(M, k)), dtype=np.int32).ravel()
             self.tensors[k] = torch.from_numpy(a if a.size else np.zeros(1, np.int32)).to(self.device)
+        o = M.options
+        self.batch_dofs = bool(getattr(o, "batch_dofs_info", False))
+        self.B = max(1, int(n_envs or 0))
         for k in MODEL_TABLES_F:
             a = np.ascontiguousarray(np.asarray(getattr(M, k)), dtype=np.float32).ravel()
+            if self.batch_dofs and k in BATCHED_DOF_TABLES:
+                a = np.ascontiguousarray(np.repeat(a[:, None], self.B, axis=1)).ravel()  # (n, B) env-minor
             self.tensors[k] = torch.from_numpy(a if a.size else np.zeros(1, np.float32)).to(self.device)
-        o = M.options
         d = _lib.GsrModel()
         d.n_links, d.n_joints, d.n_dofs, d.n_qs = M.n_links, M.n_joints, M.n_dofs, M.n_qs
         d.n_geoms, d.n_entities, d.n_equalities, d.n_verts, d.n_pairs = (M.n_geoms, M.n_entities, M.n_equalities,
@@ -135,6 +144,7 @@ class DeviceModel:
             raise ValueError(f"wavefront solver supports at most {WAVE_MAX_ENTITIES} entities")
         d.collide_variant = d.solver_variant if collide_variant is None else int(collide_variant)
         d.dyn_variant = d.solver_variant if dyn_variant is None else int(dyn_variant)
+        d.batch_dofs = int(self.batch_dofs)
         d.dt, d.tolerance, d.ls_tolerance = o.dt, o.tolerance, o.ls_tolerance
         d.eps = float(np.finfo(np.float32).eps)
         d.meaninertia = float(M.meaninertia)
@@ -459,8 +469,35 @@ class RigidSolver:
                                    f"(batched per-env {what} parameters are not built).")
 
     def _set_dofs(self, name, values, dofs_idx, envs_idx, width=0):
+        if self.dm.batch_dofs and name in BATCHED_DOF_TABLES:
+            # one value per (env, dof): the table is (n * width, B) env-minor; values (n_envs, n[, width]) or (n[, w])
+            n, D = len(self._idx(dofs_idx, self.model.n_dofs)), self.model.n_dofs
+            e = self._idx(envs_idx, self._B)
+            v = torch.as_tensor(values, dtype=torch.float32, device=self.device).reshape(-1, n, max(width, 1))
+            self._write_env_table(self.dm.tensors[name].view(D * max(width, 1), self._B),
+                                  v.expand(len(e), n, max(width, 1)), dofs_idx, D, e, max(width, 1))
+            return
         self._no_envs(envs_idx, "dofs")
         self._model_write(name, values, dofs_idx, self.model.n_dofs, width)
+
+    def _write_env_table(self, table, v, idx, n, e, width):
+        i = self._idx(idx, n)
+        if bool(((i < 0) | (i >= n)).any()):
+            raise IndexError(f"index out of range [0, {n})")
+        rows = (i[:, None] * width + torch.arange(width, device=self.device)[None]).reshape(-1)
+        table[rows[:, None], e[None, :]] = v.reshape(len(e), -1).T
+
+    def _get_dofs(self, name, dofs_idx, envs_idx, width=1):
+        if self.dm.batch_dofs and name in BATCHED_DOF_TABLES:
+            D = self.model.n_dofs
+            i = self._idx(dofs_idx, D)
+            rows = (i[:, None] * width + torch.arange(width, device=self.device)[None]).reshape(-1)
+            v = self.dm.tensors[name].view(D * width, self._B)[rows][:, self._idx(envs_idx, self._B)].T
+            v = v.reshape(-1, len(i), width)
+            v = v[..., 0] if width == 1 else v
+            return v[0] if self.n_envs == 0 else v
+        self._no_envs(envs_idx, "dofs")
+        return self._model_read(name, dofs_idx, self.model.n_dofs)
 
     def set_dofs_kp(self, kp, dofs_idx=None, envs_idx=None):
         self._set_dofs("dof_kp", kp, dofs_idx, envs_idx)
@@ -478,11 +515,11 @@ class RigidSolver:
         self._set_dofs("dof_frictionloss", frictionloss, dofs_idx, envs_idx)
 
     def set_dofs_force_range(self, lower, upper, dofs_idx=None, envs_idx=None):
-        lo, hi = (torch.as_tensor(x, dtype=torch.float64).reshape(-1).cpu() for x in (lower, upper))
+        lo, hi = (torch.as_tensor(x, dtype=torch.float64).cpu() for x in (lower, upper))
         self._set_dofs("dof_force_range", torch.stack([lo, hi], -1), dofs_idx, envs_idx, 2)
 
     def set_dofs_limit(self, lower, upper, dofs_idx=None, envs_idx=None):
-        lo, hi = (torch.as_tensor(x, dtype=torch.float64).reshape(-1).cpu() for x in (lower, upper))
+        lo, hi = (torch.as_tensor(x, dtype=torch.float64).cpu() for x in (lower, upper))
         self._set_dofs("dof_limit", torch.stack([lo, hi], -1), dofs_idx, envs_idx, 2)
 
     def set_dofs_armature(self, armature, dofs_idx=None, envs_idx=None):
@@ -509,24 +546,19 @@ class RigidSolver:
         self._model_write("geom_friction", friction, geoms_idx, self.model.n_geoms)
 
     def get_dofs_kp(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read("dof_kp", dofs_idx, self.model.n_dofs)
+        return self._get_dofs("dof_kp", dofs_idx, envs_idx)
 
     def get_dofs_kv(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read("dof_kv", dofs_idx, self.model.n_dofs)
+        return self._get_dofs("dof_kv", dofs_idx, envs_idx)
 
     def get_dofs_stiffness(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read("dof_stiffness", dofs_idx, self.model.n_dofs)
+        return self._get_dofs("dof_stiffness", dofs_idx, envs_idx)
 
     def get_dofs_damping(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read("dof_damping", dofs_idx, self.model.n_dofs)
+        return self._get_dofs("dof_damping", dofs_idx, envs_idx)
 
     def get_dofs_frictionloss(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read("dof_frictionloss", dofs_idx, self.model.n_dofs)
+        return self._get_dofs("dof_frictionloss", dofs_idx, envs_idx)
 
     def get_dofs_armature(self, dofs_idx=None, envs_idx=None):
         self._no_envs(envs_idx, "dofs")
@@ -537,13 +569,11 @@ class RigidSolver:
         return self._model_read("dof_invweight", dofs_idx, self.model.n_dofs)
 
     def get_dofs_force_range(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        t = self._model_read("dof_force_range", dofs_idx, self.model.n_dofs)
+        t = self._get_dofs("dof_force_range", dofs_idx, envs_idx, 2)
         return t[..., 0], t[..., 1]
 
     def get_dofs_limit(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        t = self._model_read("dof_limit", dofs_idx, self.model.n_dofs)
+        t = self._get_dofs("dof_limit", dofs_idx, envs_idx, 2)
         return t[..., 0], t[..., 1]
 
     def get_links_inertial_mass(self, links_idx=None, envs_idx=None):

@@ -63,6 +63,9 @@ class RigidOptions:
     box_box_detection: bool = False
     use_gjk_collision: Optional[bool] = None
     max_dynamic_constraints: int = 8
+    batch_dofs_info: bool = False
+    batch_links_info: bool = False
+    batch_joints_info: bool = False
 
 
 class options:  # noqa: N801 - namespace mirroring gs.options
@@ -402,7 +405,8 @@ class Scene:
             multiplier_collision_broad_phase=ro.multiplier_collision_broad_phase,
             enable_collision=ro.enable_collision and not ro.disable_constraint, enable_joint_limit=ro.enable_joint_limit,
             enable_self_collision=ro.enable_self_collision, enable_adjacent_collision=ro.enable_adjacent_collision,
-            enable_multi_contact=ro.enable_multi_contact, max_dynamic_constraints=ro.max_dynamic_constraints)
+            enable_multi_contact=ro.enable_multi_contact, max_dynamic_constraints=ro.max_dynamic_constraints,
+            batch_dofs_info=ro.batch_dofs_info)
 
     def build(self, n_envs: int = 0, device="cuda", solver_variant=None):
         ro = self.rigid_options

@@ -56,7 +56,9 @@ typedef struct gsr_model {
   int32_t solver_variant;  /* 0: one lane per env (rows env-minor), 1: one wavefront per env (rows env-major) */
   int32_t collide_variant; /* 0: one lane per env, 1: one wavefront per env (pairs across lanes) */
   int32_t dyn_variant;     /* step_1 / step_2: 0 one lane per env, 1 one wavefront per env (entities across lanes) */
-  int32_t pad_variant;
+  int32_t batch_dofs;      /* 1: dof_kp, dof_kv, dof_force_range, dof_stiffness, dof_damping, dof_frictionloss and
+                              dof_limit hold one value per env, env-minor like the state (component c of dof d of env b
+                              at [(d * k + c) * n_envs + b]); 0: one value per dof shared by every env */
   float dt, tolerance, ls_tolerance, eps, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
   float gravity[4];
   /* links */

@@ -172,3 +172,62 @@ def test_entity_mass_shift_changes_resting_contact_force():
         np.testing.assert_allclose(fr[1], max(mu[0], 0.5 * mu[1]), rtol=1e-6)
     finally:
         gs.destroy()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_batched_dofs_info_matches_per_env_oracles(variant):
+    """batch_dofs_info=True (options/solvers.py:375): per-env kp, kv, force range, damping, stiffness and
+    frictionloss. Env b is held over resynced steps to an oracle of the shared-parameter model carrying env b's
+    values."""
+    _need_gpu()
+    import copy
+    import dataclasses
+    from oracle.oracle import Oracle
+    B = 6
+    w = W.arm_workload(n_envs=B)
+    M = compile_model(w.entities, dataclasses.replace(w.options, batch_dofs_info=True))
+    S = RigidSolver(M, B, "cuda", solver_variant=variant)
+    rng = np.random.default_rng(11)
+    arm7 = np.arange(7)
+    S.set_dofs_kp(rng.uniform(1000.0, 6000.0, (B, 7)), dofs_idx=arm7)
+    S.set_dofs_kv(rng.uniform(50.0, 300.0, (B, 7)), dofs_idx=arm7)
+    f = rng.uniform(20.0, 80.0, (B, 3))
+    S.set_dofs_force_range(-f, f, dofs_idx=[0, 1, 2])
+    S.set_dofs_damping(rng.uniform(0.0, 3.0, (B, M.n_dofs)))
+    S.set_dofs_stiffness(rng.uniform(0.0, 5.0, (B, M.n_dofs)))
+    S.set_dofs_frictionloss(rng.uniform(0.0, 0.5, (B, M.n_dofs)))
+    S.set_dofs_kp(np.full(7, 100.0), dofs_idx=arm7, envs_idx=[2])  # one env, values broadcast
+    kp = S.get_dofs_kp()
+    assert tuple(kp.shape) == (B, M.n_dofs)
+    assert bool((kp[2, :7] == 100.0).all()) and not bool((kp[3, :7] == 100.0).all())
+    lo, hi = S.get_dofs_force_range(dofs_idx=[0, 1, 2])
+    np.testing.assert_allclose(hi.cpu().numpy(), f.astype(np.float32))
+    oracles = []
+    for b in range(B):
+        Mb = copy.copy(M)
+        for name, get in (("dof_kp", S.get_dofs_kp), ("dof_kv", S.get_dofs_kv), ("dof_damping", S.get_dofs_damping),
+                          ("dof_stiffness", S.get_dofs_stiffness), ("dof_frictionloss", S.get_dofs_frictionloss)):
+            setattr(Mb, name, get()[b].cpu().numpy().astype(np.float32))
+        for name, get in (("dof_force_range", S.get_dofs_force_range), ("dof_limit", S.get_dofs_limit)):
+            a, c = get()
+            setattr(Mb, name, np.stack([a[b].cpu().numpy(), c[b].cpu().numpy()], -1).astype(np.float32))
+        oracles.append((Oracle(Mb, 1, 32), Oracle(Mb, 1, 64)))
+    lo_, hi_ = (np.where(np.isfinite(M.dof_limit[:, k]), M.dof_limit[:, k], (-1, 1)[k]) for k in (0, 1))
+    S.control_dofs_position(torch.as_tensor(rng.uniform(lo_, hi_, (B, M.n_dofs)), dtype=torch.float32))
+    for step in range(40):
+        torch.cuda.synchronize()
+        for b, (O, O64) in enumerate(oracles):
+            gpu_to_oracle(S, O, envs=slice(b, b + 1))
+            gpu_to_oracle(S, O64, envs=slice(b, b + 1))
+        S.substep()
+        for O, O64 in oracles:
+            O.run(1)
+            O64.run(1)
+        torch.cuda.synchronize()
+        qg, vg = S.st.t["qpos"].T.cpu().numpy(), S.st.t["vel"].T.cpu().numpy()
+        for b, (O, O64) in enumerate(oracles):
+            for name, g_ in (("qpos", qg[b]), ("vel", vg[b])):
+                o_, o64 = O.get(name)[0], O64.get(name)[0]
+                err = np.abs(g_ - o_)
+                bound = np.maximum(TOL + TOL * np.abs(o_), 4 * np.abs(o_ - o64).max() + TOL)
+                assert (err <= bound).all(), f"step {step} env {b} {name}: max excess {(err - bound).max():.3g}"
