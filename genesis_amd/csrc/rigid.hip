@@ -478,7 +478,8 @@ int check(const gsr_model* m, const gsr_state* s) {
   if (m->n_geoms >= (1 << 15)) return GSR_E_SHAPE;
   if (m->solver_variant == 1) {
     if (m->n_entities > kMaxPatternEntities) return GSR_E_SHAPE;
-    if (wave_lds_bytes(m->n_dofs, m->n_entities) > 160 * 1024) return GSR_E_SHAPE;
+    if (m->wave_h_floats < 0 || wave_lds_bytes(m->n_dofs, m->n_entities, m->wave_h_floats) > 160 * 1024)
+      return GSR_E_SHAPE;
   }
   if (m->collide_variant == 1) {
     if (m->n_geoms > kWaveMaxGeoms || m->n_contacts_per_pair > kMaxPairContacts) return GSR_E_SHAPE;
@@ -515,7 +516,8 @@ void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
       break;
     case 2:
       if (m->solver_variant == 1)
-        hipLaunchKernelGGL(k_solve_wave, dim3(s->n_envs), dim3(kWave), wave_lds_bytes(m->n_dofs, m->n_entities), st,
+        hipLaunchKernelGGL(k_solve_wave, dim3(s->n_envs), dim3(kWave),
+                           wave_lds_bytes(m->n_dofs, m->n_entities, m->wave_h_floats), st,
                            *m, *s);
       else
         hipLaunchKernelGGL(k_solve, g, dim3(kBlock), 0, st, *m, *s);
@@ -647,7 +649,8 @@ static int phase_lds(const gsr_batch_info& b, int phase) {
   for (int k = 0; k < b.n_types; k++) {
     const gsr_model& m = b.models[k];
     if (phase == 1 && m.collide_variant == 1) lds = std::max(lds, (size_t)wave_collide_lds_bytes(m.n_geoms));
-    if (phase == 3 && m.solver_variant == 1) lds = std::max(lds, (size_t)wave_lds_bytes(m.n_dofs, m.n_entities));
+    if (phase == 3 && m.solver_variant == 1)
+      lds = std::max(lds, (size_t)wave_lds_bytes(m.n_dofs, m.n_entities, m.wave_h_floats));
   }
   return (int)lds;
 }
@@ -678,7 +681,8 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
       out.models[type] = G.model;
       out.states[type] = G.state;
     }
-    if (G.model.solver_variant == 1) lds = std::max(lds, wave_lds_bytes(G.model.n_dofs, G.model.n_entities));
+    if (G.model.solver_variant == 1)
+      lds = std::max(lds, wave_lds_bytes(G.model.n_dofs, G.model.n_entities, G.model.wave_h_floats));
     if (G.model.collide_variant == 1) lds = std::max(lds, wave_collide_lds_bytes(G.model.n_geoms));
     int n_t = (G.n_envs + G.envs_per_tile - 1) / G.envs_per_tile;
     ranges[g] = BatchRange{type, G.env_begin, G.n_envs, G.envs_per_tile, tiles, tiles + n_t, 0, 0};
@@ -771,7 +775,7 @@ int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_
     for (int k = 0; k < info->n_types; k++) {
       const gsr_model& m = info->models[k];
       const gsr_state& st = info->states[k];
-      const size_t lds = m.solver_variant == 1 ? wave_lds_bytes(m.n_dofs, m.n_entities) : 0;
+      const size_t lds = m.solver_variant == 1 ? wave_lds_bytes(m.n_dofs, m.n_entities, m.wave_h_floats) : 0;
       hipLaunchKernelGGL(k_gjk_resume, dim3(st.n_envs), dim3(kWave), lds, stream, m, st);
     }
   }

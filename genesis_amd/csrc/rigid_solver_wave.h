@@ -8,7 +8,7 @@
 //     row order through per-entity row lists;
 //   * the Hessian H = M + J^T D J lives only on its entity-block pattern: the symbolic Cholesky fill of the entity
 //     graph of all constraint rows, so every entry outside it is an exact zero in the dense factorisation too and
-//     skipping it changes no bit of the factors. When the pattern fits (kWaveHBudget floats) H is kept in LDS,
+//     skipping it changes no bit of the factors. When the pattern fits (gsr_model.wave_h_floats) H is kept in LDS,
 //     row-compressed per entity; otherwise in the env's packed dense lower triangle in global memory.
 // Vectors over dofs live in LDS; constraint rows live in global memory, env-major.
 #pragma once
@@ -103,7 +103,7 @@ struct WaveLds {
   // Hessian in LDS (when the pattern fits): row i of entity e starts at hoff[e] + r * below[e] + r * (r + 1) / 2
   // (r = i - es[e]) and holds the dofs of the pattern entities below e (entity k at pre[pstart[e] + rank of k in
   // the row]) followed by e's own dofs up to i.
-  lds_f* hl;                // [kWaveHBudget]
+  lds_f* hl;                // [gsr_model.wave_h_floats]
   lds_i* es;                  // [E] first factor dof of the entity at each position
   lds_i* ee;                  // [E] end factor dof of the entity at each position
   lds_i* hoff;                // [E]
@@ -114,16 +114,17 @@ struct WaveLds {
   lds_i* moff;                // [E] offset of each entity's mass block (copy of ent_mass_offset, model numbering)
 };
 
-constexpr int kWaveHBudget = 5120;   // floats of LDS for the Hessian of one env
 constexpr int kWavePreSlots = 256;   // off-diagonal pattern blocks addressable in LDS mode
 
-__host__ __device__ inline size_t wave_lds_bytes(int D, int E) {
-  return sizeof(float) * (6 * (size_t)D + kWaveHBudget) + sizeof(unsigned long long) * 2 * (size_t)E +
+// H = floats of LDS for the Hessian of one env (gsr_model.wave_h_floats): a pattern that needs more keeps H in global
+// memory; a smaller budget lets more envs share a CU
+__host__ __device__ inline size_t wave_lds_bytes(int D, int E, int H) {
+  return sizeof(float) * (6 * (size_t)D + (size_t)H) + sizeof(unsigned long long) * 2 * (size_t)E +
          sizeof(int) * 3 * (size_t)D + sizeof(int) * (9 * (size_t)E + 6) + sizeof(int) * (6 * (size_t)E + kWavePreSlots + 1) +
          64;
 }
 
-__device__ inline WaveLds wave_lds_carve(lds_c* base, int D, int E) {
+__device__ inline WaveLds wave_lds_carve(lds_c* base, int D, int E, int H) {
   WaveLds w;
   lds_f* f = reinterpret_cast<lds_f*>(base);
   w.qacc = f; f += D;
@@ -132,7 +133,7 @@ __device__ inline WaveLds wave_lds_carve(lds_c* base, int D, int E) {
   w.mv = f; f += D;
   w.qfrc = f; f += D;
   w.vec = f; f += D;
-  w.hl = f; f += kWaveHBudget;
+  w.hl = f; f += H;
   uint32_t p = (uint32_t)reinterpret_cast<uintptr_t>(f);
   lds_c* pc = reinterpret_cast<lds_c*>(f) + (((p + 7u) & ~7u) - p);
   w.pat = reinterpret_cast<lds_u64*>(pc);
@@ -429,7 +430,7 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
       L.below[ie] = acc;
       total += n * acc + n * (n + 1) / 2;
     }
-    L.hmode[0] = (ok && total <= kWaveHBudget) ? 1 : 0;
+    L.hmode[0] = (ok && total <= w.e.m.wave_h_floats) ? 1 : 0;
   }
   __syncthreads();
 }
@@ -1297,7 +1298,7 @@ __device__ __forceinline__ void wave_solve(const Env& e, char* lds) {
   const gsr_state& s = e.s;
   const int lane = threadIdx.x;
   const int D = m.n_dofs, E = m.n_entities;
-  WaveLds L = wave_lds_carve((lds_c*)lds, D, E);
+  WaveLds L = wave_lds_carve((lds_c*)lds, D, E, m.wave_h_floats);
   int n_con = e.I(s.n_con, 0), ne = e.I(s.n_con, 1), nef = ne + e.I(s.n_con, 2);
   WaveCtx w{e, L, lane, D, E, n_con, ne, nef};
   int ws = e.I(s.is_warmstart, 0);

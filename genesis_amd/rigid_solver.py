@@ -105,6 +105,18 @@ def finalize_model(M: RigidModel):
     M.mass_blk_size = int((ns * ns).sum())
 
 
+# LDS floats the wavefront solver may reserve for one env's Hessian: enough for every 64-box pile Hessian measured
+WAVE_H_FLOATS = 5120
+
+
+def wave_h_floats_default(M: RigidModel) -> int:
+    """Hessian LDS budget of one env: 4x the entity-diagonal blocks (room for the contact-coupled off-diagonal
+    blocks), in [1024, WAVE_H_FLOATS]. A Hessian that does not fit is kept in global memory instead (same bits,
+    slower); a smaller budget lets more envs share a CU (a 20-box pile: 19 KB of LDS per env instead of 27 KB)."""
+    n = (np.asarray(M.ent_dof_end) - np.asarray(M.ent_dof_start)).astype(np.int64)
+    diag = int((n * (n + 1) // 2).sum())
+    return int(min(WAVE_H_FLOATS, max(1024, -(-4 * diag // 256) * 256)))
+
 # per-dof tables that batch_dofs_info makes per-env (env-minor, like the state)
 BATCHED_DOF_TABLES = ("dof_kp", "dof_kv", "dof_force_range", "dof_stiffness", "dof_damping", "dof_frictionloss",
                       "dof_limit")
@@ -114,7 +126,7 @@ class DeviceModel:
     """Static tables of one scene type uploaded once (torch-owned device memory) + the gsr_model descriptor."""
 
     def __init__(self, M: RigidModel, device, solver_variant=None, collide_variant=None, dyn_variant=None,
-                 n_envs=None):
+                 n_envs=None, wave_h_floats=None):
         finalize_model(M)
         self.M = M
         self.device = torch.device(device)
@@ -145,6 +157,7 @@ class DeviceModel:
         d.collide_variant = d.solver_variant if collide_variant is None else int(collide_variant)
         d.dyn_variant = d.solver_variant if dyn_variant is None else int(dyn_variant)
         d.batch_dofs = int(self.batch_dofs)
+        d.wave_h_floats = wave_h_floats_default(M) if wave_h_floats is None else int(wave_h_floats)
         d.dt, d.tolerance, d.ls_tolerance = o.dt, o.tolerance, o.ls_tolerance
         d.eps = float(np.finfo(np.float32).eps)
         d.meaninertia = float(M.meaninertia)
@@ -234,7 +247,7 @@ class RigidSolver:
     """Batched rigid solver for one scene type on one device."""
 
     def __init__(self, model: RigidModel, n_envs: int, device="cuda", solver_variant=None, collide_variant=None,
-                 dyn_variant=None):
+                 dyn_variant=None, wave_h_floats=None):
         self.model = model
         self.n_envs = n_envs
         self._B = max(1, n_envs)
@@ -242,7 +255,8 @@ class RigidSolver:
         if self.device.type != "cuda":
             raise RuntimeError("the rigid step runs on a HIP device; there is no CPU execution path")
         self._lib = _lib.load()
-        self.dm = DeviceModel(model, self.device, solver_variant, collide_variant, dyn_variant, n_envs=n_envs)
+        self.dm = DeviceModel(model, self.device, solver_variant, collide_variant, dyn_variant, n_envs=n_envs,
+                              wave_h_floats=wave_h_floats)
         self.st = DeviceState(self.dm, self._B)
         self._errno_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._substep = 0

@@ -253,10 +253,13 @@ class Control:
 
 
 def init_state(w: Workload, M, S, env_begin=0, n_envs=None):
-    """Initial poses of the workload for envs [env_begin, env_begin + n_envs) (qpos0 unless the workload says)."""
+    """Fresh state for envs [env_begin, env_begin + n_envs): zero velocities, force control, neutral shifts, cleared
+    caches and errno, and the workload's initial poses (qpos0 unless the workload says). Nothing of a previous
+    occupant of those envs survives (a reused serving slot starts like a new env)."""
     import torch
     n = S._B - env_begin if n_envs is None else n_envs
     envs = torch.arange(env_begin, env_begin + n, device=S.device)
+    S.st.init_state(envs)
     if w.init_qpos is not None:
         S.set_qpos(torch.as_tensor(w.init_qpos(n, M.n_qs), dtype=torch.float32), envs_idx=envs)
     elif w.name == "quadruped":

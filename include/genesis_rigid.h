@@ -59,6 +59,9 @@ typedef struct gsr_model {
   int32_t batch_dofs;      /* 1: dof_kp, dof_kv, dof_force_range, dof_stiffness, dof_damping, dof_frictionloss and
                               dof_limit hold one value per env, env-minor like the state (component c of dof d of env b
                               at [(d * k + c) * n_envs + b]); 0: one value per dof shared by every env */
+  int32_t wave_h_floats;   /* wavefront solver: LDS floats reserved for one env's Hessian (a larger pattern keeps it
+                              in global memory); sets the solver's LDS per env, so how many envs share a CU */
+  int32_t pad_model;
   float dt, tolerance, ls_tolerance, eps, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
   float gravity[4];
   /* links */

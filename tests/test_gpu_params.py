@@ -231,3 +231,22 @@ def test_batched_dofs_info_matches_per_env_oracles(variant):
                 err = np.abs(g_ - o_)
                 bound = np.maximum(TOL + TOL * np.abs(o_), 4 * np.abs(o_ - o64).max() + TOL)
                 assert (err <= bound).all(), f"step {step} env {b} {name}: max excess {(err - bound).max():.3g}"
+
+
+def test_hessian_lds_budget_changes_no_bit():
+    """The wavefront solver keeps an env's Hessian in LDS when it fits the model's budget and in global memory
+    otherwise; both stores give the same bits, so the budget only moves envs between them."""
+    _need_gpu()
+    w = W.pile_workload(n_boxes=8, n_envs=32, max_collision_pairs=200)
+    M = compile_model(w.entities, w.options)
+    q0 = torch.as_tensor(w.init_qpos(32, M.n_qs), dtype=torch.float32)
+    out = []
+    for H in (5120, 256, 0):
+        S = RigidSolver(M, 32, "cuda", solver_variant=1, wave_h_floats=H)
+        S.set_qpos(q0)
+        for _ in range(80):
+            S.substep()
+        torch.cuda.synchronize()
+        out.append((S.st.t["qpos"].clone(), S.st.t["vel"].clone()))
+    for q, v in out[1:]:
+        assert torch.equal(q, out[0][0]) and torch.equal(v, out[0][1])

@@ -46,6 +46,21 @@ def test_request_same_alone_and_in_mix():
         _same(alone[r.rid], got[r.rid].result, f"request {r.rid} ({r.scene}) alone vs in the mix")
 
 
+def test_reused_slots_give_the_alone_result():
+    """A request admitted into slots another request has just left gets the result it gets in a fresh pool."""
+    from genesis_amd.serving import Request
+    probe = Request(11, "pile20", 4, 12, 0.0, seed=5)
+    srv = _server(caps=(("pile20", 4),))
+    fresh = Request(probe.rid, probe.scene, probe.n_envs, probe.horizon, 0.0, probe.seed)
+    srv.run([fresh])
+    srv = _server(caps=(("pile20", 4),))
+    first = Request(10, "pile20", 4, 30, 0.0, seed=4)
+    second = Request(probe.rid, probe.scene, probe.n_envs, probe.horizon, 0.0, probe.seed)
+    st = srv.run([first, second])
+    assert st["completed"] == 2 and second.slot == first.slot == 0
+    _same(fresh.result, second.result, "request in reused slots vs in a fresh pool")
+
+
 def test_request_matches_plain_drop_in():
     from genesis_amd.model import compile_model
     from genesis_amd.rigid_solver import RigidSolver

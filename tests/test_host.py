@@ -59,6 +59,17 @@ def test_checkpoint_skips_only_state_fields():
                 "cache_normal", "sort_value", "sort_ig", "first_time", "links_pos", "links_quat"} & CHECKPOINT_SKIP
 
 
+def test_wave_hessian_budget_default():
+    from genesis_amd.rigid_solver import WAVE_H_FLOATS, wave_h_floats_default
+    pile64, pile20 = (compile_model(w.entities, w.options) for w in (W.pile_workload(n_boxes=64),
+                                                                   W.pile_workload(n_boxes=20)))
+    assert wave_h_floats_default(pile64) == WAVE_H_FLOATS
+    # 20 free boxes: 20 x 21 diagonal floats, 4x that rounded up to 256
+    assert wave_h_floats_default(pile20) == 1792
+    arm = compile_model(W.arm_workload().entities, W.arm_workload().options)
+    assert wave_h_floats_default(arm) == 1024
+
+
 def test_state_size_rules_cover_every_pointer():
     from genesis_amd.rigid_solver import finalize_model, state_sizes
     for w in (W.arm_workload(), W.quadruped_workload()):
