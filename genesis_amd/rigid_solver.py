@@ -607,7 +607,10 @@ class RigidSolver:
         if bool(((i < 0) | (i >= n)).any()):
             raise IndexError(f"{name}: index out of range [0, {n})")
         e = self._idx(envs_idx, self._B)
-        v = torch.as_tensor(values, dtype=torch.float32, device=self.device).reshape(len(e), len(i), width)
+        v = torch.as_tensor(values, dtype=torch.float32, device=self.device)
+        if v.numel() == len(i) * width:  # one value set for every listed env
+            v = v.reshape(1, len(i), width).expand(len(e), len(i), width)
+        v = v.reshape(len(e), len(i), width)
         rows = (i[:, None] * width + torch.arange(width, device=self.device)[None]).reshape(-1)
         self.st.t[name][rows[:, None], e[None, :]] = v.reshape(len(e), -1).T
         return e
