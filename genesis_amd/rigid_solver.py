@@ -601,6 +601,37 @@ class RigidSolver:
     def get_geoms_friction(self, geoms_idx=None):
         return self._model_read("geom_friction", geoms_idx, self.model.n_geoms)
 
+    def get_mass_mat(self, dofs_idx=None, envs_idx=None, decompose=False):
+        """Joint-space mass matrix (B, n, n) at the current state (rsd:2436-2449), as the step's dynamics builds it:
+        entity blocks with armature and the implicit damping terms (dof damping, and kv for position / velocity
+        controlled dofs, times dt). decompose: (L, D_inv) with unit-diagonal L and M = L^T diag(1 / D_inv) L. The
+        dynamics kernel is run on the cached kinematics; its other outputs are recomputed from the same inputs by
+        the next step, so the call changes no result."""
+        self.launch_kernel(0)
+        Mo, B = self.model, self._B
+        src = self.st.t["mass_L_blk" if decompose else "mass_blk"]
+        out = torch.zeros((B, Mo.n_dofs, Mo.n_dofs), dtype=torch.float32, device=self.device)
+        for ie in range(Mo.n_entities):
+            st, en = int(Mo.ent_dof_start[ie]), int(Mo.ent_dof_end[ie])
+            n = en - st
+            if n == 0:
+                continue
+            off = int(Mo.ent_mass_offset[ie])
+            blk = src[off:off + n * n].T.reshape(B, n, n)
+            out[:, st:en, st:en] = torch.tril(blk) if decompose else blk
+        e = None if envs_idx is None else self._idx(envs_idx, B)
+        if e is not None:
+            out = out[e]
+        if dofs_idx is not None:
+            d = self._idx(dofs_idx, Mo.n_dofs)
+            out = out[:, d][:, :, d]
+        if decompose:
+            dinv = self.st.t["mass_D_inv"].T
+            dinv = dinv if e is None else dinv[e]
+            dinv = dinv if dofs_idx is None else dinv[:, self._idx(dofs_idx, Mo.n_dofs)]
+            return (out[0], dinv[0]) if self.n_envs == 0 else (out, dinv)
+        return out[0] if self.n_envs == 0 else out
+
     # -- per-env shifts (rsd:1755-1799 setters, 2326-2350 getters): links_state / geoms_state, (n, B) like all state
     def _write_env(self, name, values, idx, n, envs_idx, width=1):
         i = self._idx(idx, n)

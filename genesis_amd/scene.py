@@ -338,6 +338,10 @@ class RigidEntity:
         self._solver.set_links_COM_shift(com_shift, self._global(links_idx_local, self.n_links, self.link_start),
                                          envs_idx)
 
+    def get_mass_mat(self, envs_idx=None, decompose=False):
+        """The entity's block of the mass matrix (rigid_entity.py:2715-2717)."""
+        return self._solver.get_mass_mat(self._dofs(None), envs_idx, decompose)
+
     def get_mass(self):
         """Total mass of the entity's links (rigid_entity.py:2965)."""
         return float(self._scene.model.link_inertial_mass[self.link_start:self.link_end].astype(np.float64).sum())

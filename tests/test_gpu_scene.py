@@ -97,3 +97,36 @@ def test_go2_urdf_standing():
         scene.step()
     z = robot.get_pos().cpu().numpy()[:, 2]
     assert (z > 0.2).all() and (z < 0.45).all()
+
+
+def test_mass_mat_of_identical_robots_and_its_factor():
+    """test_rigid_physics.py:1788-1820: two identical arms have the same mass matrix, and L^T diag(1 / D_inv) L
+    rebuilds it; the matrix also matches the oracle's at the same state."""
+    from oracle.oracle import Oracle
+    from tests.sync import gpu_to_oracle
+    scene = gs.Scene(sim_options=gs.options.SimOptions(dt=0.01, substeps=1))
+    scene.add_entity(gs.morphs.Plane())
+    a1 = scene.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 0, 0)))
+    a2 = scene.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 2, 0)))
+    scene.build()
+    m1, m2 = a1.get_mass_mat(), a2.get_mass_mat()
+    assert tuple(m1.shape) == (a1.n_dofs, a1.n_dofs)
+    torch.testing.assert_close(m1, m2, atol=1e-6, rtol=1e-6)
+    L, dinv = a1.get_mass_mat(decompose=True)
+    torch.testing.assert_close(L.T @ torch.diag(1.0 / dinv) @ L, m1, atol=1e-5, rtol=1e-5)
+    S = scene.rigid_solver
+    O = Oracle(S.model, 1, 64)
+    gpu_to_oracle(S, O)
+    O.run(stage=2)
+    np.testing.assert_allclose(S.get_mass_mat().cpu().numpy(), O.get("mass_mat")[0], atol=1e-5, rtol=1e-5)
+    # reading it between steps changes no result
+    other = gs.Scene(sim_options=gs.options.SimOptions(dt=0.01, substeps=1))
+    other.add_entity(gs.morphs.Plane())
+    other.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 0, 0)))
+    other.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 2, 0)))
+    other.build()
+    for _ in range(10):
+        S.get_mass_mat()
+        scene.step()
+        other.step()
+    assert torch.equal(S.st.t["qpos"], other.rigid_solver.st.t["qpos"])
