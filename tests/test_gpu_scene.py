@@ -189,3 +189,19 @@ def test_125_cubes_stay_on_their_grid():
         q = c.get_dofs_position().cpu().numpy()
         np.testing.assert_allclose(q[:3], (i * (1.0 - 1e-3), j * (1.0 - 1e-3), k * (1.0 - 1e-3) + 0.5), atol=0.05)
         np.testing.assert_allclose(q[3:], 0.0, atol=0.03)
+
+
+@pytest.mark.parametrize("multi,count", [(True, 4), (False, 1)])
+def test_box_on_plane_contact_count(multi, count):
+    """test_gstaichi.py:95-124, 151: a box resting on the plane has 4 contacts with multi-contact and 1 without."""
+    scene = gs.Scene(sim_options=gs.options.SimOptions(dt=0.01),
+                     rigid_options=gs.options.RigidOptions(enable_multi_contact=multi))
+    scene.add_entity(gs.morphs.Plane())
+    box = scene.add_entity(gs.morphs.Box(size=(0.2, 0.2, 0.2), pos=(0.0, 0.0, 0.1)))
+    scene.build(n_envs=2)
+    for _ in range(20):
+        scene.step()
+    c = box.get_contacts()
+    assert bool(c["valid_mask"].all()) and c["valid_mask"].shape == (2, count)
+    if multi:  # one contact point cannot hold the box flat: it rocks on it
+        np.testing.assert_allclose(box.get_pos().cpu().numpy()[:, 2], 0.1, atol=2e-3)
