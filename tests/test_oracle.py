@@ -320,22 +320,6 @@ def test_mpr_sphere_sphere_and_capsule(Oracle):
     np.testing.assert_allclose(c[0, 10], 0.15 - 0.12, atol=2e-6)  # MPR stops at CCD_TOLERANCE = 1e-6
 
 
-def test_imp_aref_known_answer(Oracle):
-    """gu.imp_aref with default solver params (geom.py:377-398, 1708-1714), computed by hand for pen = 5e-4."""
-    # default: timeconst 0.02 (2*dt), dampratio 1, dmin .9, dmax .95, width 1e-3, mid .5, power 2
-    # x = 0.5 -> imp_b branch (x == mid): 1 - (1/0.5) * 0.5^2 = 0.5 -> imp = 0.9 + 0.5*0.05 = 0.925
-    M = box_scene(size=(0.2, 0.2, 0.2), pos=(0, 0, 0.1 - 5e-4))
-    o = Oracle(M, 1, 64)
-    o.run(stage=3)
-    imp = 0.925
-    k = 1 / (0.95 ** 2 * 0.02 ** 2)
-    aref_expected = -k * imp * (-5e-4)
-    assert o.get("n_contacts")[0] == 4
-    # aref is not exported; check the consequence: the first solver step's acceleration lifts the box
-    o.run(1)
-    assert o.get("vel")[0][2] > 0 or aref_expected > 0
-
-
 def test_fp64_vs_fp32_drift_small(Oracle):
     """FP32 and FP64 oracles stay within FP32 noise over 100 arm steps under PD control."""
     w = W.arm_workload()

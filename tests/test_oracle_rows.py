@@ -1,0 +1,281 @@
+"""Known answers for the oracle's constraint rows, build-time inverse weights and box-box multi-contact (no GPU).
+
+Each test builds a scene whose row inputs have closed forms (Jacobian, J v, inverse weight, penetration), reads the
+rows the FP64 oracle builds (efc_aref, efc_D, efc_diag, efc_jac) and compares them with numbers worked out by hand
+from the reference's definitions:
+
+* impedance and reference acceleration, `imp_aref` (genesis/utils/geom.py:377-398) with the default solver
+  parameters (timeconst 0 -> 2 dt, dampratio 1, dmin 0.9, dmax 0.95, width 1e-3, mid 0.5, power 2;
+  geom.py:1708-1714 sanitised by rigid_solver_decomp.py:49-77);
+* contact rows (constraint_solver_decomp.py:428-525): J v, diag = (iw + mu^2 iw) 2 mu^2 (1 - imp) / imp, D = 1/diag;
+* joint-limit rows (constraint_solver_decomp.py:1017-1072) and the JOINT equality row (:641-718), whose quartic
+  coupling and residual are evaluated by hand;
+* link and dof inverse weights (rigid_solver_decomp.py:354-507): J M^-1 J^T from finite-difference Jacobians and a
+  brute-force mass matrix, and the free-body closed form.
+
+Impedance by hand (x = |pos| / width; y = x^2 / mid^(p-1) for x < mid, else 1 - (1-x)^2 / (1-mid)^(p-1);
+imp = dmin + y (dmax - dmin), imp = dmax for x > 1):
+    x = 0.2 -> y = 0.08 -> imp = 0.904       x = 0.5 -> y = 0.5 -> imp = 0.925
+    x = 0.7 -> y = 0.82 -> imp = 0.941       x = 2.5 -> imp = dmax = 0.95
+With dt = 0.01 the time constant is 0.02: b = 2 / (0.95 * 0.02), k = 1 / (0.95^2 * 0.02^2), aref = -b J v - k imp pos.
+"""
+import numpy as np
+import pytest
+
+from genesis_amd import workloads as W
+from genesis_amd.model import (EQ_JOINT, CompileOptions, EntitySpec, EqualitySpec, GeomSpec, JointSpec, LinkSpec,
+                               JOINT_FIXED, JOINT_PRISMATIC, JOINT_REVOLUTE, compile_model, quat_to_R)
+
+IMP = {0.2: 0.904, 0.5: 0.925, 0.7: 0.941, 2.5: 0.95}
+WIDTH = 1e-3
+B_DAMP = 2.0 / (0.95 * 0.02)
+K_SPRING = 1.0 / (0.95 ** 2 * 0.02 ** 2)
+
+
+@pytest.fixture(scope="module")
+def Oracle(oracle_lib):
+    return oracle_lib.Oracle
+
+
+def f32(x):
+    """Model tables are stored in float32: the oracle sees the float32 value of every constant."""
+    return float(np.float32(x))
+
+
+@pytest.mark.parametrize("x", [0.2, 0.5, 0.7, 2.5])
+def test_contact_rows_known_answer(Oracle, x):
+    """A 0.2 m box sinking into the plane at vz = -0.05 m/s with penetration x * width. Every pyramid row of every
+    corner contact has J v = vz (the directions d mu - n have unit z and horizontal d), inverse weight 1/m (the plane's
+    link has none) and the contact friction mu = max(1.5, 1.0)."""
+    pen = x * WIDTH
+    vz = -0.05
+    ents = [W.plane_entity(), W.box_entity((0.2, 0.2, 0.2), (0, 0, 0.1 - pen), friction=1.5)]
+    M = compile_model(ents, CompileOptions(dt=0.01))
+    o = Oracle(M, 1, 64)
+    o.set("vel", np.array([[0, 0, vz, 0, 0, 0]]))
+    o.run(1, stage=4)
+    assert o.get("n_contacts")[0] == 4
+    c = o.get("contacts")[0]
+    np.testing.assert_allclose(c[:, 10], pen, atol=1e-8)  # float32 box extents
+    mu = c[0, 11]
+    assert mu == f32(1.5)
+    m = 200.0 * 0.2 ** 3  # default density 200 kg/m^3
+    iw = 1.0 / m
+    assert o.get("n_con")[0] == 16 and o.get("n_con_eq")[0] == 0 and o.get("n_con_fl")[0] == 0
+    J = o.get("efc_jac")[0]
+    v = np.array([0, 0, vz, 0, 0, 0])
+    np.testing.assert_allclose(J @ v, vz, rtol=1e-6)
+    imp = IMP[x]
+    pen_c = c[:, 10].repeat(4)  # the rows of contact i use that contact's own penetration
+    aref_expected = -B_DAMP * vz - K_SPRING * imp * (-pen_c)
+    diag_expected = (iw + mu * mu * iw) * 2 * mu * mu * (1 - imp) / imp
+    np.testing.assert_allclose(o.get("efc_aref")[0], aref_expected, rtol=2e-5)
+    np.testing.assert_allclose(o.get("efc_diag")[0], diag_expected, rtol=2e-5)
+    np.testing.assert_allclose(o.get("efc_D")[0], 1.0 / diag_expected, rtol=2e-5)
+    # the four rows of a contact differ only in the friction direction: same aref, same diag
+    a = o.get("efc_aref")[0].reshape(4, 4)
+    np.testing.assert_array_equal(a, a[:, :1].repeat(4, 1))
+
+
+def _hinge(limit=(-0.5, 0.5), length=0.5, mass=1.0):
+    base = LinkSpec("base", -1, np.zeros(3), np.array([1.0, 0, 0, 0]), [JointSpec("fix", JOINT_FIXED)], [])
+    arm = LinkSpec("arm", 0, np.array([0, 0, 1.0]), np.array([1.0, 0, 0, 0]),
+                   [JointSpec("hinge", JOINT_REVOLUTE, axis=np.array([0, 1.0, 0]), limit=np.array([limit]))], [],
+                   inertial_pos=np.array([length, 0, 0]), inertial_quat=np.array([1.0, 0, 0, 0]),
+                   inertial_i=np.eye(3) * 1e-6, inertial_mass=mass)
+    return EntitySpec([base, arm], name="hinge")
+
+
+@pytest.mark.parametrize("x", [0.2, 0.7, 2.5])
+@pytest.mark.parametrize("side", [1, -1])
+def test_joint_limit_row_known_answer(Oracle, x, side):
+    """A hinge past its upper (side 1) or lower (side -1) limit of +-0.5 rad by x * width, turning at w = 0.3 rad/s.
+    pos_delta = -x width; the row's Jacobian is -1 past the upper limit and +1 past the lower one; the dof inverse
+    weight is 1 / (m l^2 + I_yy) for the point-mass hinge."""
+    M = compile_model([_hinge()], CompileOptions(dt=0.01, enable_collision=False))
+    o = Oracle(M, 1, 64)
+    delta = x * WIDTH
+    w = 0.3
+    o.set("qpos", np.array([[side * (0.5 + delta)]]))
+    o.set("vel", np.array([[w]]))
+    o.run(1, stage=4)
+    assert o.get("n_con")[0] == 1
+    jac = -1.0 if side == 1 else 1.0
+    np.testing.assert_allclose(o.get("efc_jac")[0], [[jac]])
+    # pos_delta in float arithmetic on the float32 limit
+    pos_delta = f32(0.5) - (0.5 + delta) if side == 1 else -(0.5 + delta) - f32(-0.5)
+    imp = IMP[x]
+    iw = 1.0 / (f32(1.0) * f32(0.5) ** 2 + f32(1e-6))
+    assert M.dof_invweight[0] == pytest.approx(iw, rel=1e-6)
+    aref_expected = -B_DAMP * (jac * w) - K_SPRING * imp * pos_delta
+    diag_expected = iw * (1 - imp) / imp
+    np.testing.assert_allclose(o.get("efc_aref")[0], [aref_expected], rtol=2e-5)
+    np.testing.assert_allclose(o.get("efc_diag")[0], [diag_expected], rtol=2e-5)
+    np.testing.assert_allclose(o.get("efc_D")[0], [1.0 / diag_expected], rtol=2e-5)
+
+
+POLY = np.array([0.01, 0.5, -0.3, 0.2, 0.1])  # y - y0 = a0 + a1 d + a2 d^2 + a3 d^3 + a4 d^4
+
+
+def _slides(m1=1.5, m2=0.5):
+    """Slide x carrying slide y: M = [[m1 + m2, 0], [0, m2]], so the dof inverse weights are 1/(m1+m2) and 1/m2."""
+    I3 = np.eye(3) * 1e-3
+    base = LinkSpec("base", -1, np.zeros(3), np.array([1.0, 0, 0, 0]), [JointSpec("fix", JOINT_FIXED)], [])
+    l1 = LinkSpec("l1", 0, np.zeros(3), np.array([1.0, 0, 0, 0]),
+                  [JointSpec("j1", JOINT_PRISMATIC, axis=np.array([1.0, 0, 0]))], [],
+                  inertial_pos=np.zeros(3), inertial_quat=np.array([1.0, 0, 0, 0]), inertial_i=I3, inertial_mass=m1)
+    l2 = LinkSpec("l2", 1, np.zeros(3), np.array([1.0, 0, 0, 0]),
+                  [JointSpec("j2", JOINT_PRISMATIC, axis=np.array([0, 1.0, 0]))], [],
+                  inertial_pos=np.zeros(3), inertial_quat=np.array([1.0, 0, 0, 0]), inertial_i=I3, inertial_mass=m2)
+    eq = EqualitySpec(EQ_JOINT, "j1", "j2", data=np.concatenate([POLY, np.zeros(6)]))
+    return EntitySpec([base, l1, l2], equalities=[eq], name="slides")
+
+
+@pytest.mark.parametrize("resid", [2e-4, -7e-4, 2.5e-3])
+def test_joint_equality_row_known_answer(Oracle, resid):
+    """JOINT equality q1 - q1_0 = poly(q2 - q2_0) on two orthogonal slides. At q2 = 0.2 (hand-evaluated):
+    poly(0.2) = 0.01 + 0.1 - 0.012 + 0.0016 + 0.00016 = 0.09976, poly'(0.2) = 0.5 - 0.12 + 0.024 + 0.0032 = 0.4072.
+    q1 = 0.09976 + resid gives the residual `resid`; the row is [1, -poly'], J v = v1 - poly' v2, aref =
+    -b J v - k imp resid with imp from |resid| / width, diag = (iw1 + iw2) (1 - imp) / imp."""
+    M = compile_model([_slides()], CompileOptions(dt=0.01, enable_collision=False, gravity=(0.0, 0.0, 0.0)))
+    o = Oracle(M, 1, 64)
+    q2 = 0.2
+    a = [f32(c) for c in POLY]
+    poly = sum(a[i] * q2 ** i for i in range(5))
+    deriv = sum(a[i + 1] * (i + 1) * q2 ** i for i in range(4))
+    assert poly == pytest.approx(0.09976, abs=1e-8) and deriv == pytest.approx(0.4072, abs=1e-8)
+    v = np.array([0.07, -0.2])
+    o.set("qpos", np.array([[poly + resid, q2]]))
+    o.set("vel", v[None])
+    o.run(1, stage=4)
+    assert o.get("n_con")[0] == 1 and o.get("n_con_eq")[0] == 1
+    J = o.get("efc_jac")[0]
+    np.testing.assert_allclose(J, [[1.0, -deriv]], rtol=1e-7)
+    iw1, iw2 = 1.0 / (1.5 + 0.5), 1.0 / 0.5
+    np.testing.assert_allclose(M.dof_invweight, [iw1, iw2], rtol=1e-6)
+    imp = IMP[round(abs(resid) / WIDTH, 6)]
+    jv = v[0] - deriv * v[1]
+    aref_expected = -B_DAMP * jv - K_SPRING * imp * resid
+    diag_expected = (iw1 + iw2) * (1 - imp) / imp
+    np.testing.assert_allclose(o.get("efc_aref")[0], [aref_expected], rtol=1e-4)
+    np.testing.assert_allclose(o.get("efc_diag")[0], [diag_expected], rtol=2e-5)
+    np.testing.assert_allclose(o.get("efc_D")[0], [1.0 / diag_expected], rtol=2e-5)
+
+
+def _link_jacobians(M, o, b, eps=1e-6):
+    """Central-difference Jacobians (6 x D: COM linear velocity, angular velocity) of every link at the env's qpos
+    (hinge / slide models, so qpos and dofs coincide)."""
+    q0 = o.get("qpos", [b])[0].copy()
+    D = M.n_dofs
+    assert M.n_qs == D
+
+    def pose(q):
+        o.set("qpos", q[None], [b])
+        o.run(stage=1, envs=(b, b + 1))
+        p, qt = o.get("links_pos", [b])[0], o.get("links_quat", [b])[0]
+        R = [quat_to_R(qt[l]) for l in range(M.n_links)]
+        c = [p[l] + R[l] @ M.link_inertial_pos[l] for l in range(M.n_links)]
+        return c, R
+
+    c0, R0 = pose(q0)
+    J = np.zeros((M.n_links, 6, D))
+    for d in range(D):
+        qp, qm = q0.copy(), q0.copy()
+        qp[d] += eps
+        qm[d] -= eps
+        cp, Rp = pose(qp)
+        cm, Rm = pose(qm)
+        for l in range(M.n_links):
+            J[l, :3, d] = (cp[l] - cm[l]) / (2 * eps)
+            Wm = (Rp[l] - Rm[l]) / (2 * eps) @ R0[l].T
+            J[l, 3:, d] = [Wm[2, 1], Wm[0, 2], Wm[1, 0]]
+    o.set("qpos", q0[None], [b])
+    return J, R0
+
+
+def test_invweight_matches_brute_force_on_arm(Oracle):
+    """rigid_solver_decomp.py:354-507 at qpos0: link invweight = (mean of the linear, mean of the angular diagonal
+    of J M^-1 J^T) for the link's COM Jacobian; 1-dof joints' dof invweight = diag(M^-1). M is assembled by brute
+    force from the finite-difference Jacobians, sum m Jp^T Jp + Jr^T I_w Jr, plus armature (no implicit damping)."""
+    w = W.arm_workload()
+    M = compile_model(w.entities, w.options)
+    o = Oracle(M, 1, 64)
+    o.set("qpos", M.qpos0[None].astype(np.float64))
+    J, R0 = _link_jacobians(M, o, 0)
+    D = M.n_dofs
+    Mb = np.zeros((D, D))
+    for l in range(M.n_links):
+        R = R0[l] @ quat_to_R(M.link_inertial_quat[l])
+        Iw = R @ M.link_inertial_i[l].reshape(3, 3).astype(np.float64) @ R.T
+        Mb += float(M.link_inertial_mass[l]) * J[l, :3].T @ J[l, :3] + J[l, 3:].T @ Iw @ J[l, 3:]
+    Mb += np.diag(M.dof_armature.astype(np.float64))
+    Minv = np.linalg.inv(Mb)
+    for l in range(M.n_links):
+        A = np.diag(J[l] @ Minv @ J[l].T)
+        np.testing.assert_allclose(M.link_invweight[l], [A[:3].mean(), A[3:].mean()], rtol=2e-5, atol=1e-9,
+                                   err_msg=f"link {l}")
+    np.testing.assert_allclose(M.dof_invweight, np.diag(Minv), rtol=2e-5)
+    # the arm's links carry real weight: a plausible mistake (the link origin instead of its COM, angular terms
+    # dropped) changes these by far more than the tolerance
+    assert (M.link_invweight[2:, 0] > 0).all() and (M.link_invweight[2:, 1] > 0).all()
+
+
+def test_invweight_free_box_closed_form():
+    """A free box (joint at its COM): link invweight = (1/m, mean(1/I_xx, 1/I_yy, 1/I_zz)), dof invweight 1/m on the
+    three translations and the same angular mean on the three rotations; the fixed plane's link has none. Mass and
+    inertia start from EPS before the geoms are added (rigid_link.py:160-162)."""
+    M = compile_model([W.plane_entity(), W.box_entity((0.1, 0.2, 0.3), (0, 0, 1.0))], CompileOptions(dt=0.01))
+    eps = float(np.finfo(np.float32).eps)
+    mb = 200.0 * 0.1 * 0.2 * 0.3
+    m = mb + eps
+    I = mb / 12.0 * np.array([0.2 ** 2 + 0.3 ** 2, 0.1 ** 2 + 0.3 ** 2, 0.1 ** 2 + 0.2 ** 2]) + eps
+    ang = np.mean(1.0 / I)
+    np.testing.assert_allclose(M.link_invweight[1], [1 / m, ang], rtol=1e-5)
+    np.testing.assert_allclose(M.link_invweight[0], [0.0, 0.0], atol=0)
+    np.testing.assert_allclose(M.dof_invweight, [1 / m] * 3 + [ang] * 3, rtol=1e-5)
+
+
+@pytest.mark.parametrize("pen", [1e-3, 4e-3])
+def test_box_on_box_multi_contact_known_answer(Oracle, pen):
+    """A 0.2 x 0.3 x 0.1 box resting centred and axis-aligned on a fixed 0.6 x 0.6 x 0.2 box, sunk by `pen`
+    (collider_decomp.py:2328-2725: MPR, then 4 detections with geom a rotated by +-mc_perturbation about two
+    axes orthogonal to the normal, rotated back, de-duplicated within 0.5 * mc_tolerance * min AABB diagonal):
+    the first contact sits on the axis of the boxes with depth pen; the four perturbed detections land on the small
+    box's four bottom corners, each at depth pen, at z = 0.2 - pen / 2 (midway between the two surfaces), with
+    normal -z to first order (|n| - 1 = O(mc_perturbation^2))."""
+    big = W.box_entity((0.6, 0.6, 0.2), (0, 0, 0.1), fixed=True)
+    small = W.box_entity((0.2, 0.3, 0.1), (0, 0, 0.25 - pen))
+    M = compile_model([big, small], CompileOptions(dt=0.01, gravity=(0.0, 0.0, 0.0)))
+    o = Oracle(M, 1, 64)
+    o.run(1, stage=3)
+    c = o.get("contacts")[0]
+    assert len(c) == 5
+    np.testing.assert_allclose(c[0, 10], pen, rtol=1e-5)
+    # the perturbed depths are measured along the first-order corrected normal: pen (1 + O(mc_perturbation^2))
+    np.testing.assert_allclose(c[1:, 10], pen, rtol=2e-4)
+    np.testing.assert_allclose(c[:, 6], 0.2 - pen / 2, atol=1e-6)
+    np.testing.assert_allclose(c[0, 7:10], [0, 0, -1], atol=1e-6)
+    np.testing.assert_allclose(c[1:, 7:10], np.tile([0, 0, -1.0], (4, 1)), atol=2e-4)
+    assert np.hypot(c[0, 4], c[0, 5]) < 0.16  # inside the overlap
+    corners = sorted(map(tuple, np.sign(c[1:, 4:6]).astype(int).tolist()))
+    assert corners == [(-1, -1), (-1, 1), (1, -1), (1, 1)]
+    np.testing.assert_allclose(np.abs(c[1:, 4:6]), np.tile([0.1, 0.15], (4, 1)), atol=1e-4)
+
+
+def test_multi_contact_dedup_keeps_one_point_contact(Oracle):
+    """De-duplication radius: a sphere resting on a box touches at one point; the four perturbed detections find
+    that point again (within far less than the radius 0.5 * 1e-2 * min AABB diagonal), so exactly one contact is
+    kept. A missing or zero radius would keep five."""
+    from genesis_amd.model import GEOM_SPHERE, JOINT_FREE
+    big = W.box_entity((0.6, 0.6, 0.2), (0, 0, 0.1), fixed=True)
+    pos = np.array([0.05, -0.02, 0.2 + 0.1 - 5e-4])
+    ball = EntitySpec([LinkSpec("b", -1, pos, np.array([1.0, 0, 0, 0]),
+                                [JointSpec("f", JOINT_FREE, init_qpos=np.concatenate([pos, [1, 0, 0, 0]]))],
+                                [GeomSpec(GEOM_SPHERE, data=(0.1,))])])
+    M = compile_model([big, ball], CompileOptions(dt=0.01, gravity=(0.0, 0.0, 0.0)))
+    o = Oracle(M, 1, 64)
+    o.run(1, stage=3)
+    c = o.get("contacts")[0]
+    assert len(c) == 1
+    np.testing.assert_allclose(c[0, 10], 5e-4, rtol=1e-3)
+    np.testing.assert_allclose(c[0, 4:6], pos[:2], atol=1e-6)
