@@ -1,16 +1,25 @@
-"""Benchmark of the batched rigid step on one MI355X (plain drop-in form).
+"""Benchmark of the batched rigid step on one MI355X.
 
 Prints ONE JSON line. A step is one scene.step() (one substep) of every env of the workload, inputs resident in HBM.
 Workloads (seeded synthetic scenes, random-init state; no datasets):
+  mixed      [default] batching form: 16384 envs split (seed 3) among the 9-DoF arm, the go2 quadruped and a 20-box
+             pile, dt = 0.01. From one snapshot of the contact-rich window (steps 150.. after the piles have landed)
+             it times (1) the plain drop-in with the fused plan's per-type execution variants (one gsr_step per
+             scene type: the like-for-like baseline), (2) the plain drop-in with each scene's default variants, and
+             (3) the fused hipGraph-captured launch; `value` is (3), `speedup_vs_plain` is (3) / (1), and the fused
+             run's final state is compared bitwise with (1)'s
   pile       64 free boxes per env (edges U[0.05, 0.2] m, 1000 kg/m^3, friction 1.0) falling in a 1 m column onto a
-             plane, 4096 envs, dt = 0.01, timed over its 500 steps           [default: the headline configuration]
+             plane, 4096 envs, dt = 0.01, plain drop-in
   arm        9-DoF Panda arm + plane, 64 envs, dt = 0.01, PD targets U(limits) resampled every 100 steps
   quadruped  go2 quadruped + plane, 32768 envs, dt = 0.005, PD targets standing + U(-0.3, 0.3) every step
-  mixed      batching form: 16384 envs split (seed 3) among arm / quadruped / 20-box pile, dt = 0.01; times the plain
-             drop-in (one gsr_step per scene type) and the fused hipGraph-captured launch from the same state
   serve      serving form: Poisson rollout requests (scene uniform over arm / quadruped / 20-box pile, n_envs
              log-uniform in [1, 4096], horizon U{50..500}) served by continuous batching under a memory budget;
              capacity measured saturated, then offered load swept 20-100 % of it; p50/p99 step and request latency
+
+Timed window: every workload first advances untimed from its seeded start to step WINDOW_START (150: the piles have
+landed and settle under ~contact-rich load), snapshots, runs the warm-up steps, restores the snapshot and times
+steps WINDOW_START .. WINDOW_START + steps - 1 (clipped to the workload's horizon; the whole horizon from step 0
+when `--steps` covers it).
 """
 import argparse
 import json
@@ -25,12 +34,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 KERNELS = ["k_step1", "k_collide", "k_solve", "k_step2"]
-HORIZON = {"pile": 500, "arm": 1000, "quadruped": 1000}  # steps of each configuration
+HORIZON = {"pile": 500, "arm": 1000, "quadruped": 1000, "pile20": 500, "mixed": 500}
+WINDOW_START = 150
 
 
-def profiled_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter summary (FETCH_SIZE and
-    WRITE_SIZE per dispatch, each scaled by what the same counter reads for a device copy of known size), or None."""
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def window(name, steps):
+    """(first timed step, timed steps) of a workload for --steps (0: its horizon)."""
+    h = HORIZON[name]
+    if steps <= 0 or steps >= h:
+        return 0, steps if steps > 0 else h
+    return min(WINDOW_START, h - steps), steps
+
+
+def hbm_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` in `workload` from the committed rocprofv3 counter summary of the same bench
+    command (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the same counter reads for a device copy of
+    known size), or None when no summary for that kernel and workload is committed."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not os.path.exists(path):
         return None
@@ -38,7 +61,7 @@ def profiled_traffic(workload, kernel):
         rec = json.load(f).get(workload)
     if not rec or rec.get("kernel") != kernel:
         return None
-    return rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]
+    return {"bytes": rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"], "source": rec.get("source")}
 
 
 def b_alg(M, n_ctrl):
@@ -46,149 +69,387 @@ def b_alg(M, n_ctrl):
     return 4 * (2 * M.n_qs + 4 * M.n_dofs + n_ctrl)
 
 
-def make(workload, n_envs):
-    from genesis_amd import workloads as W
-    from genesis_amd.model import compile_model
-    if workload == "pile":
-        w = W.pile_workload(n_envs=n_envs or 4096)
-    elif workload == "arm":
-        w = W.arm_workload(n_envs=n_envs or 64)
-    elif workload == "quadruped":
-        w = W.quadruped_workload(n_envs=n_envs or 32768)
-    else:
-        raise SystemExit(f"unknown workload {workload}")
-    return w, compile_model(w.entities, w.options)
+class Part:
+    """One scene type of a workload: its model, solver, controls, and the snapshot the timed runs start from."""
+
+    def __init__(self, w, n_envs, seed, solver_variant=None):
+        import torch
+        from genesis_amd.model import compile_model
+        from genesis_amd.rigid_solver import RigidSolver
+        from genesis_amd.workloads import Control, init_state
+        self.w, self.name = w, w.name
+        self.M = compile_model(w.entities, w.options)
+        self.S = RigidSolver(self.M, n_envs, "cuda", solver_variant=solver_variant)
+        init_state(w, self.M, self.S)
+        self.ctrl = Control(w, self.M, self.S, seed)
+        d = self.S.dm.desc
+        self.default_variants = (int(d.solver_variant), int(d.collide_variant), int(d.dyn_variant))
+        self.snap = None
+        self.gen_state = None
+        self.torch = torch
+
+    @property
+    def n(self):
+        return self.S._B
+
+    def balg(self):
+        return b_alg(self.M, self.ctrl.n_ctrl)
+
+    def save(self):
+        self.snap = {k: v.clone() for k, v in self.S.st.t.items()}
+        self.gen_state = self.ctrl.gen.get_state()
+
+    def restore(self):
+        for k, v in self.snap.items():
+            self.S.st.t[k].copy_(v)
+        self.ctrl.gen.set_state(self.gen_state)
 
 
-def cpu_baseline(w, M, steps, seconds=30.0, min_seconds=10.0):
-    """The CPU oracle (scalar restatement of the reference step, FP32) on the host cores, one env per thread, over
-    the same trajectory as the timed GPU run: batches of `threads` envs (the workload's envs 0, 1, ... in order) each
-    run from their seeded initial state for `steps` steps, until at least `min_seconds` of CPU time are sampled; a
-    batch still running after `seconds` stops early."""
+class Runner:
+    """Steps a list of parts through the workload's trajectory and times windows of it."""
+
+    def __init__(self, parts):
+        import torch
+        self.parts = parts
+        self.torch = torch
+        self.step = 0
+
+    def controls(self):
+        for p in self.parts:
+            p.ctrl(self.step)
+
+    def plain_step(self):
+        for p in self.parts:
+            p.S._lib.gsr_step(p.S.dm.desc, p.S.st.desc, 1, p.S._stream())
+
+    def advance(self, n):
+        for _ in range(n):
+            self.controls()
+            self.plain_step()
+            self.step += 1
+            if self.step % 50 == 0:
+                log(f"bench: advanced to step {self.step}")
+        self.torch.cuda.synchronize()
+        self.check()
+
+    def check(self):
+        for p in self.parts:
+            p.S.check_errno()
+
+    def snapshot(self):
+        self.torch.cuda.synchronize()
+        for p in self.parts:
+            p.save()
+        self.step0 = self.step
+
+    def restore(self):
+        self.torch.cuda.synchronize()
+        for p in self.parts:
+            p.restore()
+        self.step = self.step0
+        self.torch.cuda.synchronize()
+
+    def timed(self, one_step, warmup, steps, label):
+        """Warm-up steps from the snapshot, restore, then `steps` timed steps (controls, the step, an errno poll
+        every 10 steps as Simulator.step does). Returns seconds."""
+        torch = self.torch
+        self.restore()
+        for _ in range(warmup):
+            self.controls()
+            one_step(None)
+            self.step += 1
+        self.restore()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            self.controls()
+            one_step(i)
+            self.step += 1
+            if self.step % 10 == 0:
+                self.check()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"bench: {label}: {steps} steps in {dt:.3f} s")
+        return dt
+
+
+def cpu_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def oracle_rate(part, first_step, min_seconds, threads, n_envs):
+    """Env-steps/s of the CPU oracle (scalar FP32 restatement of the reference step) on `n_envs` envs of the part's
+    scene with `threads` host threads, over steps first_step.. of the scene's own trajectory (seeded start, the
+    workload's control schedule drawn with numpy): untimed up to first_step, then timed until min_seconds pass."""
     from oracle.oracle import Oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
-    n = threads
-    env_steps, t_total, batches, done = 0, 0.0, 0, 0
+    from genesis_amd.workloads import GO2_INIT_BASE, GO2_STANDING
+    w, M = part.w, part.M
+    O = Oracle(M, n_envs, 32)
+    if w.init_qpos is not None:
+        O.set("qpos", w.init_qpos(n_envs, M.n_qs))
+    elif w.name == "quadruped":
+        O.set("qpos", np.tile(np.concatenate([GO2_INIT_BASE, GO2_STANDING]), (n_envs, 1)))
+    mode = np.zeros((n_envs, M.n_dofs), np.int32)
+    rng = np.random.default_rng(0)
+    if w.name == "arm":
+        mode[:] = 2
+    elif w.name == "quadruped":
+        mode[:, 6:] = 2
+    O.set("ctrl_mode", mode)
+
+    def ctrl(step):
+        if w.name == "arm" and step % 100 == 0:
+            O.set("ctrl_pos", rng.uniform(M.dof_limit[:, 0], M.dof_limit[:, 1], (n_envs, M.n_dofs)))
+        elif w.name == "quadruped":
+            t = np.zeros((n_envs, M.n_dofs))
+            t[:, 6:] = GO2_STANDING + rng.uniform(-0.3, 0.3, (n_envs, 12))
+            O.set("ctrl_pos", t)
+
+    for s in range(first_step):
+        ctrl(s)
+        O.run(1, n_threads=cpu_threads())
+    t_total, done = 0.0, 0
     while t_total < min_seconds:
-        O = Oracle(M, n, 32)
-        if w.init_qpos is not None:
-            O.set("qpos", w.init_qpos((batches + 1) * n, M.n_qs)[batches * n:])
-        ctrl_mode = np.full((n, M.n_dofs), 2 if w.name != "pile" else 0, np.int32)
-        if w.name == "quadruped":
-            ctrl_mode[:, :6] = 0
-        O.set("ctrl_mode", ctrl_mode)
-        done, t0 = 0, time.time()
-        while done < steps and t_total + time.time() - t0 < seconds:
-            O.run(1, n_threads=threads)
-            done += 1
-        t_total += time.time() - t0
-        env_steps += n * done
-        batches += 1
-        if done < steps:
-            break
-    return {"value": env_steps / t_total, "unit": "env-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{batches} batch(es) of {n} envs x steps 0-{steps - 1} of the same workload on the CPU oracle"
-                      + (f" (the last batch to step {done - 1})" if done < steps else "") + f", {t_total:.1f} s"}
+        ctrl(first_step + done)
+        t0 = time.perf_counter()
+        O.run(1, n_threads=threads)
+        t_total += time.perf_counter() - t0
+        done += 1
+    return n_envs * done / t_total, done
+
+
+def cpu_baseline(parts, first_step, seconds=4.0):
+    """The oracle timed on the host: all cores (env-chunked threads, like the reference's GS_PARA_LEVEL=2 with
+    TI_NUM_THREADS=N) and one thread (the reference's CPU default), per scene type over the same window of its
+    trajectory as the GPU run; a mixed batch's rate is the split-weighted combination (time per mixed step =
+    sum over scene types of envs / rate)."""
+    T = cpu_threads()
+    per, t_all, t_one = {}, 0.0, 0.0
+    for p in parts:
+        r_all, n_all = oracle_rate(p, first_step, seconds, T, T)
+        r_one, n_one = oracle_rate(p, first_step, seconds / 2, 1, max(1, T // 4))
+        per[p.name] = {"all_cores": round(r_all, 1), "one_thread": round(r_one, 1)}
+        t_all += p.n / r_all
+        t_one += p.n / r_one
+    n = sum(p.n for p in parts)
+    return {"value": round(n / t_all, 1), "unit": "env-steps/s", "cores": T, "kind": "port",
+            "value_one_thread": round(n / t_one, 1), "per_scene": per, "host": host_cpu(),
+            "sample": f"CPU oracle (FP32), per scene type {T} envs on {T} threads and {max(1, T // 4)} envs on 1 "
+                      f"thread, steps {first_step}.. of each scene's seeded trajectory for >= {seconds:g} s "
+                      f"(1 thread: >= {seconds / 2:g} s); mixed rate = split-weighted"}
+
+
+def plain_kernel_events(runner, K):
+    """HIP events around every plain kernel of the timed steps, on the launch stream (torch's current stream)."""
+    import torch
+    ev = [[[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in KERNELS]
+           for _ in runner.parts] for _ in range(K)]
+
+    def one_step(i):
+        for j, p in enumerate(runner.parts):
+            for k in range(4):
+                if i is not None:
+                    ev[i][j][k][0].record()
+                p.S.launch_kernel(k)
+                if i is not None:
+                    ev[i][j][k][1].record()
+            p.S._substep += 1
+
+    def kernel_ms():
+        out = {}
+        for j, p in enumerate(runner.parts):
+            for k in range(4):
+                ms = sum(e[j][k][0].elapsed_time(e[j][k][1]) for e in ev) / K
+                out[f"{p.name}:{p.S.kernel_name(k)}"] = round(ms, 4)
+        return out
+
+    return one_step, kernel_ms
 
 
 def bench_mixed(args):
     import torch
     from genesis_amd import batch as FB
-    from genesis_amd.model import compile_model
-    from genesis_amd.rigid_solver import RigidSolver
-    from genesis_amd.workloads import Control, init_state, mixed_workloads
+    from genesis_amd.workloads import mixed_workloads
 
     n_total = args.n_envs or 16384
-    parts = []
-    for i, (w, n) in enumerate(mixed_workloads(n_total)):
-        if n == 0:
-            continue
-        M = compile_model(w.entities, w.options)
-        S = RigidSolver(M, n, "cuda")
-        init_state(w, M, S)
-        parts.append((w, M, S, Control(w, M, S, args.seed + i)))
-    step = 0
+    parts = [Part(w, n, args.seed + i) for i, (w, n) in enumerate(mixed_workloads(n_total)) if n > 0]
+    runner = Runner(parts)
+    first, K = window("mixed", args.steps)
+    W = args.warmup
+    log(f"bench: mixed {[(p.name, p.n) for p in parts]}, window steps {first}..{first + K - 1}")
+    runner.advance(first)
+    runner.snapshot()
 
-    def controls():
-        for w, M, S, c in parts:
-            c(step)
+    # the fused plan: tiles and variants per scene type and the launch form, tuned on the snapshot state
+    if args.plan:
+        spec = json.loads(args.plan)
+        plan = [FB.Group(next(p.S for p in parts if p.name == g["scene"]), 0, None, g["envs_per_tile"],
+                         g["solver"], g["collide"]) for g in spec["groups"]]
+        launch, report = spec["launch"], []
+    else:
+        plan, launch, report = FB.autotune([FB.Group(p.S) for p in parts])
+    plan = [g.resolved() for g in plan]
+    names = {id(p.S): p.name for p in parts}
+    variants = {id(g.solver): (g.solver_variant, g.collide_variant) for g in plan}
+    plan_json = {"groups": [{"scene": names[id(g.solver)], "solver": g.solver_variant, "collide": g.collide_variant,
+                             "envs_per_tile": g.envs_per_tile} for g in plan], "launch": launch}
+    log("bench: plan " + json.dumps(plan_json))
 
-    for _ in range(args.warmup):
-        controls()
-        for w, M, S, c in parts:
-            S.substep()
-        step += 1
-    torch.cuda.synchronize()
-    snap = [FB._snapshot(S) for _, _, S, _ in parts]
-    step0 = step
-
-    def timed(run):
-        nonlocal step
-        step = step0
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for it in range(args.steps):
-            controls()
-            run()
-            step += 1
-            if step % 10 == 0:
-                for _, _, S, _ in parts:
-                    S.check_errno()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    def plain():
-        for _, _, S, _ in parts:
-            S._lib.gsr_step(S.dm.desc, S.st.desc, 1, S._stream())
-
-    t_plain = timed(plain)
-    # tile and launch choices are tuned on the state the timed window reaches halfway (boxes landing), then the
-    # fused run starts from the same state as the plain run
-    for (_, _, S, _), sn in zip(parts, snap):
-        FB._restore(S, sn)
-    step = step0
-    for _ in range(args.steps // 2):
-        controls()
-        plain()
-        step += 1
-    tuned, launch, report = FB.autotune([FB.Group(S) for _, _, S, _ in parts])
-    for (_, _, S, _), sn in zip(parts, snap):
-        FB._restore(S, sn)
-    fb = FB.FusedBatch(tuned, **launch)
+    res = {}
+    forms = args.forms.split(",")
+    if "plain" in forms:
+        for p in parts:
+            p.S.set_variants(*variants[id(p.S)])
+        one, kms = plain_kernel_events(runner, K)
+        t = runner.timed(one, W, K, "plain (fused plan's variants)")
+        res["plain"] = {"value": round(n_total * K / t, 1), "ms_per_step": round(1e3 * t / K, 4),
+                        "kernel_ms": kms(),
+                        "variants": {p.name: ["lane", "wave"][variants[id(p.S)][0]] for p in parts}}
+        final_plain = [(p.S.st.t["qpos"].clone(), p.S.st.t["vel"].clone()) for p in parts]
+    if "default" in forms:
+        for p in parts:
+            p.S.set_variants(*p.default_variants)
+        one, kms = plain_kernel_events(runner, K)
+        t = runner.timed(one, W, K, "plain (default variants)")
+        res["plain_default_variants"] = {"value": round(n_total * K / t, 1), "ms_per_step": round(1e3 * t / K, 4),
+                                         "kernel_ms": kms(),
+                                         "variants": {p.name: ["lane", "wave"][p.default_variants[0]] for p in parts}}
+    fb = FB.FusedBatch(plan, **launch)
+    knames = fb.kernel_names()
     fb.capture(1)
-    t_fused = timed(fb.replay)
-    B = sum(S._B for _, _, S, _ in parts)
-    v_plain, v_fused = B * args.steps / t_plain, B * args.steps / t_fused
-    names = {id(S): w.name for w, _, S, _ in parts}
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+
+    def fused_step(i):
+        if i is not None:
+            ev[i][0].record()
+        fb.replay()
+        if i is not None:
+            ev[i][1].record()
+
+    t_fused = runner.timed(fused_step, W, K, "fused graph")
+    substep_ms = sum(a.elapsed_time(b) for a, b in ev) / K
+    bitwise = None
+    if "plain" in forms:
+        bitwise = all(torch.equal(q, p.S.st.t["qpos"]) and torch.equal(v, p.S.st.t["vel"])
+                      for (q, v), p in zip(final_plain, parts))
+    # per-kernel times of the fused substep: the graph's only kernel, or (several kernels) a pass that launches
+    # each kernel of the same steps with events around it
+    if len(knames) == 1:
+        kernel_ms = {knames[0]: substep_ms}
+    else:
+        kev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in knames]
+               for _ in range(K)]
+
+        def fused_kernels(i):
+            for k in range(len(knames)):
+                if i is not None:
+                    kev[i][k][0].record()
+                fb.launch_kernel(k)
+                if i is not None:
+                    kev[i][k][1].record()
+
+        runner.timed(fused_kernels, 0, K, "fused kernels (per-kernel events)")
+        kernel_ms = {}
+        for k, nm in enumerate(knames):
+            kernel_ms[nm] = kernel_ms.get(nm, 0.0) + sum(e[k][0].elapsed_time(e[k][1]) for e in kev) / K
+    dom = max(kernel_ms, key=kernel_ms.get)
+    balg_launch = sum(p.balg() * p.n for p in parts)
+    achieved = balg_launch / (kernel_ms[dom] * 1e-3) / 1e9
+    value = n_total * K / t_fused
+    tr = hbm_traffic("mixed", dom)
     out = {
-        "metric": "env-steps/s (fused batched vs plain drop-in, 1 GPU)",
-        "value": round(v_fused, 1), "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * t_fused / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scenes, random-init state)",
-        "config": {"workload": "mixed batch arm/quadruped/20-box pile", "n_envs": B, "dt": 0.01, "form": "fused",
-                   "split": {w.name: S._B for w, _, S, _ in parts},
-                   "tiles": [{"scene": names[id(g.solver)], "solver": ["lane-per-env", "wavefront-per-env"][g.solver_variant],
-                              "envs_per_tile": g.envs_per_tile} for g in tuned],
-                   "grid": fb.info.grid, "lds_bytes": fb.info.lds_bytes, "launch": launch},
-        "plain": {"value": round(v_plain, 1), "ms_per_step": round(1e3 * t_plain / args.steps, 4)},
-        "speedup_vs_plain": round(v_fused / v_plain, 3),
-        "autotune": [dict(r, scene=parts[r["group"]][0].name if r["group"] != "all" else "all", ms=round(r["ms"], 4))
+        "metric": "env-steps/s (plain vs fused batched, 1 GPU)",
+        "value": round(value, 1), "unit": "env-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(1e3 * t_fused / K, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scenes and controls, random-init state)",
+        "config": {"workload": "mixed batch: 9-DoF arm / go2 quadruped / 20-box pile", "n_envs": n_total,
+                   "dt": 0.01, "form": "fused hipGraph", "window": [first, first + K - 1],
+                   "split": {p.name: p.n for p in parts}, "plan": plan_json,
+                   "grid": int(fb.info.grid), "lds_bytes": int(fb.info.lds_bytes)},
+        "plain": res.get("plain"),
+        "plain_default_variants": res.get("plain_default_variants"),
+        "speedup_vs_plain": round(value / res["plain"]["value"], 3) if "plain" in res else None,
+        "speedup_vs_plain_default_variants": (round(value / res["plain_default_variants"]["value"], 3)
+                                              if "plain_default_variants" in res else None),
+        "fused_bitwise_equals_plain": bitwise,
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "fused_substep_gpu_ms": round(substep_ms, 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tr["bytes"] if tr else None, "traffic_source": tr["source"] if tr else None,
+                     "b_alg_per_launch": balg_launch,
+                     "b_alg_per_env_step": {p.name: p.balg() for p in parts},
+                     "step_frac": balg_launch / n_total * value / 1e9 / HBM_PEAK_GBS},
+        "autotune": [dict(r, scene=parts[r["group"]].name if r["group"] != "all" else "all", ms=round(r["ms"], 4))
                      for r in report],
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(parts, first)
+    print(json.dumps(out))
+
+
+def bench_single(args):
+    import torch
+    from genesis_amd import workloads as Wk
+    n = args.n_envs or {"pile": 4096, "arm": 64, "quadruped": 32768}[args.workload]
+    w = {"pile": lambda: Wk.pile_workload(n_envs=n), "arm": lambda: Wk.arm_workload(n_envs=n),
+         "quadruped": lambda: Wk.quadruped_workload(n_envs=n)}[args.workload]()
+    p = Part(w, n, args.seed, solver_variant=args.solver_variant)
+    runner = Runner([p])
+    first, K = window(w.name, args.steps)
+    runner.advance(first)
+    runner.snapshot()
+    one, kms = plain_kernel_events(runner, K)
+    t = runner.timed(one, args.warmup, K, "plain")
+    value = n * K / t
+    k_ms = kms()
+    dom = max(k_ms, key=k_ms.get)
+    balg = p.balg()
+    achieved = balg * n / (k_ms[dom] * 1e-3) / 1e9
+    kern = dom.split(":")[1]
+    tr = hbm_traffic(w.name, kern)
+    out = {
+        "metric": "env-steps/s (plain drop-in, 1 GPU)", "value": round(value, 1), "unit": "env-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": round(1e3 * t / K, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene and controls, random-init state)",
+        "config": {"workload": {"pile": "box pile 64 boxes/env", "arm": "9-DoF arm + plane",
+                                "quadruped": "go2 quadruped + plane"}[w.name],
+                   "n_envs": n, "dt": w.options.dt, "form": "plain", "window": [first, first + K - 1],
+                   "n_dofs": p.M.n_dofs, "n_geoms": p.M.n_geoms,
+                   "solver": ["lane-per-env", "wavefront-per-env"][p.S.dm.desc.solver_variant]},
+        "kernel_ms": k_ms,
+        "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tr["bytes"] if tr else None, "traffic_source": tr["source"] if tr else None,
+                     "b_alg_per_env_step": balg, "step_frac": balg * value / 1e9 / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline([p], first)
     print(json.dumps(out))
 
 
 def bench_serve(args):
-    import numpy as np
-    import torch
     from genesis_amd.batch import FusedBatch, Group, autotune, time_group
     from genesis_amd.serving import Pool, Server, poisson_requests
     from genesis_amd.workloads import init_state
 
     cap_envs = args.n_envs or 8192
     pools = {s: Pool(s, cap_envs, dt=0.01) for s in ("arm", "quadruped", "pile20")}
-    # execution choices per scene type, tuned once on a 1024-env slice of each pool
-    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    # execution choices per scene type, tuned once on a slice of each pool
     log("serve: tuning tiles")
     n_tune = min(512, cap_envs)
     for p in pools.values():
@@ -201,7 +462,7 @@ def bench_serve(args):
         for p in pools.values():
             if p.S is g.solver:
                 p.envs_per_tile = g.envs_per_tile
-                p.S.dm.desc.solver_variant = g.solver_variant
+                p.S.set_variants(g.solver_variant, g.collide_variant)
                 p.ms_per_env = time_group(g, steps=3, warmup=1, **launch) / n_tune
     log("serve: ms per env per tick " + ", ".join(f"{p.scene} {p.ms_per_env * 1e3:.3f} us" for p in pools.values()))
     budget = int(args.serve_budget_gb * 2**30)
@@ -233,7 +494,8 @@ def bench_serve(args):
         "config": {"workload": "serving arm/quadruped/20-box pile requests", "pool_envs_per_scene": cap_envs,
                    "memory_budget_gb": args.serve_budget_gb, "tick_budget_ms": args.serve_tick_ms or None,
                    "dt": 0.01, "form": "serving",
-                   "tiles": {p.scene: {"envs_per_tile": p.envs_per_tile, "solver": int(p.S.dm.desc.solver_variant)}
+                   "tiles": {p.scene: {"envs_per_tile": p.envs_per_tile, "solver": int(p.S.dm.desc.solver_variant),
+                                       "collide": int(p.S.dm.desc.collide_variant)}
                              for p in pools.values()}},
         "step_ms_p50": at80["step_ms_p50"], "step_ms_p99": at80["step_ms_p99"],
         "request_ms_p50": at80["request_ms_p50"], "request_ms_p99": at80["request_ms_p99"],
@@ -259,9 +521,14 @@ def calibrate_copy(mb):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's horizon)")
+    ap.add_argument("--steps", type=int, default=100, help="timed steps (0: the workload's whole horizon)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="pile", choices=["pile", "arm", "quadruped", "mixed", "serve"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "pile", "arm", "quadruped", "serve"])
+    ap.add_argument("--forms", default="plain,default,fused",
+                    help="mixed: the forms to time besides the fused graph (plain = fused plan's variants, "
+                         "default = each scene's default variants)")
+    ap.add_argument("--plan", default="", help="mixed: a fused plan (JSON as printed in config.plan) instead of "
+                                               "autotuning")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window per offered-load level")
     ap.add_argument("--serve-budget-gb", type=float, default=12.0, help="device memory budget for admission")
     ap.add_argument("--serve-saturate", type=int, default=24, help="requests queued at t=0 to measure capacity")
@@ -276,89 +543,14 @@ def main():
     if args.calibrate_copy > 0:
         return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (envs are independent: "
+                                   "a multi-GPU run would shard the env axis, one process per GPU)"}))
         sys.exit(2)
-
     if args.workload == "mixed":
-        args.steps = args.steps or 100
         return bench_mixed(args)
     if args.workload == "serve":
         return bench_serve(args)
-
-    import torch
-    from genesis_amd.rigid_solver import RigidSolver
-    from genesis_amd.workloads import Control, init_state
-
-    w, M = make(args.workload, args.n_envs)
-    # the timed steps are the workload's own trajectory from its seeded initial state (the pile: free fall and
-    # settling); warm-up steps run first on the same state, which is then put back
-    args.steps = args.steps or HORIZON[w.name]
-    S = RigidSolver(M, w.n_envs if not args.n_envs else args.n_envs, "cuda", solver_variant=args.solver_variant)
-    init_state(w, M, S)
-    snap = {k: v.clone() for k, v in S.st.t.items()}
-    ctrl = Control(w, M, S, args.seed)
-    for step in range(args.warmup):
-        ctrl(step)
-        S.substep()
-    torch.cuda.synchronize()
-    for k, v in snap.items():
-        S.st.t[k].copy_(v)
-    del snap
-    ctrl = Control(w, M, S, args.seed)
-    step = 0
-    torch.cuda.synchronize()
-
-    # every kernel launch of the timed region is bracketed by events on the launch stream (torch's current stream)
-    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in KERNELS]
-          for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for it in range(args.steps):
-        ctrl(step)
-        for k in range(4):
-            ev[it][k][0].record()
-            S.launch_kernel(k)
-            ev[it][k][1].record()
-        S._substep += 1
-        if S._substep % 10 == 0:
-            S.check_errno()
-        step += 1
-        if step % 100 == 0:
-            print(f"bench: {step} / {args.steps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    B = S._B
-    value = B * args.steps / elapsed
-    k_avg = np.array([sum(e[k][0].elapsed_time(e[k][1]) for e in ev) for k in range(4)]) / args.steps
-    dom = int(np.argmax(k_avg))
-    balg = b_alg(M, ctrl.n_ctrl)
-    achieved = balg * B / (k_avg[dom] * 1e-3) / 1e9
-    out = {
-        "metric": "env-steps/s (plain drop-in, 1 GPU)",
-        "value": round(value, 1),
-        "unit": "env-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded scene, random-init state)",
-        "config": {"workload": {"pile": "box pile 64 boxes/env", "arm": "9-DoF arm + plane",
-                                "quadruped": "go2 quadruped + plane"}[w.name],
-                   "n_envs": B, "dt": w.options.dt, "form": "plain", "n_dofs": M.n_dofs, "n_geoms": M.n_geoms,
-                   "solver": ["lane-per-env", "wavefront-per-env"][S.dm.desc.solver_variant]},
-        "kernel_ms": {k: round(float(v), 4) for k, v in zip(KERNELS, k_avg)},
-        "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(w.name, S.kernel_name(dom)),
-                     "step_frac": balg * value / 1e9 / HBM_PEAK_GBS, "b_alg_per_env_step": balg},
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, M, args.steps)
-    print(json.dumps(out))
+    return bench_single(args)
 
 
 if __name__ == "__main__":

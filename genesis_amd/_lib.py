@@ -18,7 +18,8 @@ EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_group_siz
            "gsr_gjk_workspace_words",
            "gsr_step", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno",
            "gsr_get_contacts", "gsr_links_contact_force", "gsr_get_equalities", "gsr_run_stage", "gsr_launch_kernel", "gsr_batch_table_bytes",
-           "gsr_batch_upload", "gsr_step_fused", "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy"]
+           "gsr_batch_upload", "gsr_step_fused", "gsr_fused_kernel_count", "gsr_launch_fused_kernel",
+           "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy"]
 
 _CTYPE = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "float": ctypes.c_float,
           "int64_t": ctypes.c_int64}
@@ -116,6 +117,8 @@ def load(path: str = LIB_PATH):
     L.gsr_batch_table_bytes.argtypes = [ctypes.c_int32]
     L.gsr_batch_upload.argtypes = [P(GsrGroup), ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, P(GsrBatchInfo)]
     L.gsr_step_fused.argtypes = [ctypes.c_void_p, P(GsrBatchInfo), ctypes.c_int32, ctypes.c_void_p]
+    L.gsr_fused_kernel_count.argtypes = [P(GsrBatchInfo)]
+    L.gsr_launch_fused_kernel.argtypes = [ctypes.c_void_p, P(GsrBatchInfo), ctypes.c_int32, ctypes.c_void_p]
     L.gsr_graph_capture.argtypes = [ctypes.c_void_p, P(GsrBatchInfo), ctypes.c_int32, ctypes.c_void_p,
                                     P(ctypes.c_void_p)]
     L.gsr_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]

@@ -52,6 +52,15 @@ class FusedBatch:
             raise ValueError("empty batch")
         self.lib = _lib.load()
         self.groups = [g.resolved() for g in groups]
+        # the solver's own descriptor takes the group's variants, so plain launches and readbacks of the same state
+        # (constraint rows and Hessian are laid out by the solver variant) see the layout the fused step wrote
+        chosen = {}
+        for g in self.groups:
+            v = (g.solver_variant, g.collide_variant)
+            if chosen.setdefault(id(g.solver), v) != v:
+                raise ValueError("groups of one solver must use the same solver and collision variants")
+        for g in self.groups:
+            g.solver.set_variants(g.solver_variant, g.collide_variant)
         self.device = self.groups[0].solver.device
         n = len(self.groups)
         nbytes = int(self.lib.gsr_batch_table_bytes(n))
@@ -60,10 +69,7 @@ class FusedBatch:
         self.table = torch.zeros(nbytes, dtype=torch.uint8, device=self.device)
         arr = (_lib.GsrGroup * n)()
         for i, g in enumerate(self.groups):
-            m = _lib.GsrModel.from_buffer_copy(g.solver.dm.desc)
-            m.solver_variant = g.solver_variant
-            m.collide_variant = g.collide_variant
-            arr[i].model = m
+            arr[i].model = g.solver.dm.desc
             arr[i].state = g.solver.st.desc
             arr[i].env_begin, arr[i].n_envs, arr[i].envs_per_tile = g.env_begin, g.n_envs, g.envs_per_tile
         self._arr = arr
@@ -88,6 +94,21 @@ class FusedBatch:
     def step(self, n_substeps: int = 1):
         _lib.check(self.lib.gsr_step_fused(ctypes.c_void_p(self.table.data_ptr()), ctypes.byref(self.info),
                                            n_substeps, self._stream()), "gsr_step_fused")
+
+    def kernel_names(self) -> List[str]:
+        """The kernels of one fused substep, in launch order (`launch_kernel(k)` launches the k-th)."""
+        i = self.info
+        if i.phased:
+            return [f"k_phase<{p}>" for p in range(5)]
+        names = [f"k_fused<{i.waves_per_simd}, {i.gjk_inline}>"]
+        if not i.gjk_inline:
+            names += ["k_gjk_resume"] * i.n_types
+        assert len(names) == self.lib.gsr_fused_kernel_count(ctypes.byref(i))
+        return names
+
+    def launch_kernel(self, k: int):
+        _lib.check(self.lib.gsr_launch_fused_kernel(ctypes.c_void_p(self.table.data_ptr()), ctypes.byref(self.info),
+                                                    k, self._stream()), "gsr_launch_fused_kernel")
 
     def capture(self, n_substeps: int = 1):
         """Record n_substeps fused launches into a hipGraph on a side stream; `replay()` launches the latest."""

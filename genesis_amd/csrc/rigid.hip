@@ -1,45 +1,26 @@
 // gfx950 kernels and C-ABI entry points of the batched rigid step (libgenesis_rigid.so).
 //
-// Plain drop-in form: one substep = five stream-ordered launches, one lane per environment over env-minor SoA state
-// (consecutive lanes touch consecutive addresses for every per-env field):
-//   k_step1      kinematics refresh (only when the pose cache is stale) + smooth dynamics (kernel_step_1)
-//   k_collide    equality rows, AABBs, sweep-and-prune, narrowphase, inequality rows (_func_constraint_force)
-//   k_solve      Newton solve + qacc/contact-force epilogue (ConstraintSolver.resolve)
-//   k_step2      integration, NaN-guarded commit, end-of-step kinematics (kernel_step_2)
+// Plain drop-in form: one substep = four stages launched in stream order over env-minor SoA state (consecutive lanes
+// or workgroups touch consecutive envs for every per-env field):
+//   step_1     smooth dynamics (kernel_step_1): k_step1 (one lane per env) or k_step1_wave (one wavefront per env)
+//   collision  equality rows, AABBs, sweep-and-prune, narrowphase, inequality rows (_func_constraint_force):
+//              k_collide / k_collide_wave, then the GJK/EPA pass k_gjk_fixup (rigid_k_collide.hip)
+//   solve      Newton solve + qacc/contact-force epilogue (ConstraintSolver.resolve): k_solve / k_solve_wave
+//              (rigid_k_solve.hip)
+//   step_2     integration, NaN-guarded commit, end-of-step kinematics (kernel_step_2): k_step2 / k_step2_wave
+// This unit holds step_1, step_2, the kinematics refresh, resets and readback kernels and every C-ABI entry point;
+// the batching form's kernels live in rigid_k_fused_*.hip, rigid_k_phase_*.hip and rigid_k_resume.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
-#include "genesis_rigid.h"
-#include "rigid_kin.h"
-#include "rigid_collide.h"
-#include "rigid_solver.h"
-#include "rigid_solver_wave.h"
-#include "rigid_collide_wave.h"
+#include "rigid_tiles.h"
 
 using namespace gsr;
 
 namespace {
-
-constexpr int kBlock = 64;
-
-__device__ __forceinline__ int env_id() { return blockIdx.x * blockDim.x + threadIdx.x; }
-
-// Workgroup -> env for the one-env-per-workgroup kernels. The dispatcher deals workgroups to the 8 XCDs round-robin
-// (workgroup i to XCD i % 8) and every XCD has its own L2, while the env-minor state arrays put 32 neighbouring envs
-// in one 128-byte line. Within each chunk of 256 workgroups, XCD x therefore gets a run of 32 consecutive envs, so
-// the waves it holds at once share their lines in its L2 instead of each XCD fetching every line. A bijection on
-// [0, n): a partial last chunk is split the same way over its m workgroups.
-__device__ __forceinline__ int xcd_env(int blk, int n) {
-  constexpr int kRun = 32, kChunk = 8 * kRun;
-  const int base = blk - blk % kChunk;
-  const int m = min(kChunk, n - base);
-  const int j = blk - base;
-  const int q = m >> 3, r = m & 7, x = j & 7, k = j >> 3;
-  return base + x * q + min(x, r) + k;
-}
 
 __global__ void __launch_bounds__(kBlock) k_kinematics(gsr_model m, gsr_state s, const int32_t* envs_idx, int n_idx) {
   int t = env_id();
@@ -59,22 +40,6 @@ __global__ void __launch_bounds__(kBlock) k_step1(gsr_model m, gsr_state s, int 
   dev_smooth_dynamics(e);
 }
 
-// step_1 / step_2 of one env on one wavefront: the env's entities are spread over the lanes (no entity reads another's
-// state); only the geom poses, split by geom index, need the link poses of every lane, hence the barrier.
-__device__ __forceinline__ void wave_step1(const Env& e) { dev_smooth_dynamics(e, threadIdx.x, kWave); }
-
-__device__ __forceinline__ void wave_step2(const Env& e) {
-  const int lane = threadIdx.x;
-  bool valid = dev_integrate_next(e, lane, kWave);
-  valid = __all(valid);
-  dev_integrate_commit(e, lane, kWave, valid, lane == 0);
-  dev_forward_kinematics(e, lane, kWave);
-  dev_com_links(e, lane, kWave);
-  __syncthreads();
-  dev_update_geoms(e, lane, kWave);
-  dev_forward_velocity(e, lane, kWave);
-  __syncthreads();
-}
 
 __global__ void __launch_bounds__(kWave) k_step1_wave(gsr_model m, gsr_state s) {
   if ((int)blockIdx.x >= s.n_envs) return;  // uniform across the block
@@ -90,49 +55,6 @@ __global__ void __launch_bounds__(kWave) k_step2_wave(gsr_model m, gsr_state s) 
   wave_step2(e);
 }
 
-// an env whose narrowphase deferred pairs to the GJK/EPA pass: its contact list and rows are completed there
-__device__ __forceinline__ bool gjk_pending(const Env& e) { return e.I(e.s.gjk_defer_n, 0) > 0; }
-
-__global__ void __launch_bounds__(kBlock) k_collide(gsr_model m, gsr_state s) {
-  int b = env_id();
-  if (b >= s.n_envs) return;
-  Env e(m, s, b);
-  dev_equality_rows(e);
-  dev_collision(e);
-  if (!gjk_pending(e)) dev_inequality_rows(e);
-}
-
-// the GJK/EPA pass of the plain step, one wavefront per env: deferred pairs, contact insertion, then the env's
-// inequality rows; envs without deferred pairs return at once
-__global__ void __launch_bounds__(kWave) k_gjk_fixup(gsr_model m, gsr_state s) {
-  if ((int)blockIdx.x >= s.n_envs) return;
-  const int b = xcd_env(blockIdx.x, s.n_envs);
-  Env e(m, s, b);
-  if (wave_gjk_fixup(e)) wave_rows(e);
-}
-
-__global__ void __launch_bounds__(kWave) k_collide_wave(gsr_model m, gsr_state s) {
-  extern __shared__ char lds[];
-  if ((int)blockIdx.x >= s.n_envs) return;  // uniform across the block
-  const int b = xcd_env(blockIdx.x, s.n_envs);
-  Env e(m, s, b);
-  wave_collide(e, lds);
-}
-
-__global__ void __launch_bounds__(kBlock) k_solve(gsr_model m, gsr_state s) {
-  int b = env_id();
-  if (b >= s.n_envs) return;
-  Env e(m, s, b);
-  dev_solve(e);
-}
-
-__global__ void __launch_bounds__(kWave) k_solve_wave(gsr_model m, gsr_state s) {
-  extern __shared__ char lds[];
-  if ((int)blockIdx.x >= s.n_envs) return;  // uniform across the block
-  const int b = xcd_env(blockIdx.x, s.n_envs);
-  Env e(m, s, b);
-  wave_solve(e, lds);
-}
 
 __global__ void __launch_bounds__(kBlock) k_step2(gsr_model m, gsr_state s) {
   int b = env_id();
@@ -160,227 +82,6 @@ __global__ void __launch_bounds__(kBlock) k_reset(gsr_model m, gsr_state s, cons
   e.I((int32_t*)s.errno_env, 0) = 0;
 }
 
-
-// ---- batching form ----------------------------------------------------------------------------------------------
-// Device table of a heterogeneous batch: a header (work queue) then one record per group (env range). The model and
-// state descriptors of the batch's scene types are kernel arguments, so every field read is a scalar load from the
-// kernel-argument segment, as in the plain kernels.
-struct BatchHeader {
-  int32_t n_groups, n_tiles;
-  uint32_t counter, done;  // work queue: next tile, finished workgroups (the last one resets both)
-};
-struct BatchRange {
-  int32_t type, env_begin, n_envs, envs_per_tile, tile_begin, tile_end, pad0, pad1;
-};
-struct FusedTypes {
-  gsr_model m[GSR_MAX_TYPES];
-  gsr_state s[GSR_MAX_TYPES];
-};
-constexpr size_t kTypesOffset = 256;
-constexpr size_t kRangesOffset = (kTypesOffset + sizeof(FusedTypes) + 255) & ~(size_t)255;
-// The descriptors are read through the constant address space: every field read with a uniform type index is a
-// scalar, invariant load, like a kernel argument, without copying the descriptors anywhere.
-typedef const FusedTypes __attribute__((address_space(4))) ConstTypes;
-
-// One substep of envs [first, first + n_here) of a group; the workgroup is one wavefront. Per-env stages run one
-// lane per env; the wavefront collision and solver variants take the tile's envs one after the other.
-// kGjk: GJK/EPA runs inside the collision stage (no env stops early, no resume launch) instead of being deferred
-template <bool kGjk>
-__device__ __forceinline__ void run_tile(const gsr_model& m, const gsr_state& s, int env0, int n_here, char* lds) {
-  const int lane = threadIdx.x;
-  const bool mine = lane < n_here;
-  const bool cw = m.collide_variant == 1, sw = m.solver_variant == 1, dv = m.dyn_variant == 1;
-  // an env with pairs deferred to GJK/EPA stops after its collision; k_gjk_resume finishes its substep
-  if (!cw && !sw && !dv) {
-    if (mine) {
-      Env e(m, s, env0 + lane);
-      dev_smooth_dynamics(e);
-      dev_equality_rows(e);
-      dev_collision<kGjk>(e);
-      if (!gjk_pending(e)) {
-        dev_inequality_rows(e);
-        dev_solve(e);
-        dev_integrate(e);
-        dev_kinematics(e);
-      }
-    }
-    return;
-  }
-  // collision reads only the pose state of the previous step, so it may run before or after the dynamics
-  if (mine && (!dv || !cw)) {
-    Env e(m, s, env0 + lane);
-    if (!dv) dev_smooth_dynamics(e);
-    if (!cw) {
-      dev_equality_rows(e);
-      dev_collision<kGjk>(e);
-      if (!gjk_pending(e)) dev_inequality_rows(e);
-    }
-  }
-  __syncthreads();
-  for (int j = 0; j < n_here; j++) {
-    Env e(m, s, env0 + j);
-    if (dv) wave_step1(e);
-    if (cw) wave_collide<kGjk>(e, lds);
-    __syncthreads();
-    if (sw && !gjk_pending(e)) wave_solve(e, lds);
-    __syncthreads();
-  }
-  if (mine && (!sw || !dv)) {
-    Env e(m, s, env0 + lane);
-    if (!gjk_pending(e)) {
-      if (!sw) dev_solve(e);
-      if (!dv) {
-        dev_integrate(e);
-        dev_kinematics(e);
-      }
-    }
-  }
-  if (dv) {
-    __syncthreads();
-    for (int j = 0; j < n_here; j++) {
-      Env e(m, s, env0 + j);
-      if (!gjk_pending(e)) wave_step2(e);
-    }
-  }
-}
-
-// The rest of the substep for the envs of one scene type that the fused launch left at their collision: the GJK/EPA
-// pass and inequality rows on lane 0, then the model's solver and integration variants. One wavefront per env; envs
-// without deferred pairs return at once.
-__global__ void __launch_bounds__(kWave) k_gjk_resume(gsr_model m, gsr_state s) {
-  extern __shared__ char lds[];
-  if ((int)blockIdx.x >= s.n_envs) return;
-  const int b = xcd_env(blockIdx.x, s.n_envs);
-  Env e(m, s, b);
-  if (!gjk_pending(e)) return;  // uniform across the block
-  wave_gjk_fixup(e);
-  wave_rows(e);
-  if (m.solver_variant == 1) wave_solve(e, lds);
-  else if (threadIdx.x == 0) dev_solve(e);
-  __syncthreads();
-  if (m.dyn_variant == 1) {
-    wave_step2(e);
-  } else if (threadIdx.x == 0) {
-    dev_integrate(e);
-    dev_kinematics(e);
-  }
-}
-
-template <bool kGjk>
-__device__ __forceinline__ void fused_tile(ConstTypes* types, const BatchRange* __restrict__ R, int n_groups,
-                                           int tile, char* lds) {
-  // the range holding the tile: binary search over the cumulative tile ends
-  int lo = 0, hi = n_groups - 1;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (tile < R[mid].tile_end) hi = mid;
-    else lo = mid + 1;
-  }
-  const BatchRange g = R[lo];
-  const int T = g.envs_per_tile;
-  const int first = (tile - g.tile_begin) * T;
-  const int n_here = min(T, g.n_envs - first);
-  const int env0 = g.env_begin + first;
-  // the type index is uniform, so the descriptor fields are scalar loads from the kernel-argument segment
-  const int k = __builtin_amdgcn_readfirstlane(g.type);
-  const FusedTypes* ty = (const FusedTypes*)types;
-  run_tile<kGjk>(ty->m[k], ty->s[k], env0, n_here, lds);
-}
-
-// Phased form of the fused batch: one launch per stage of the plain substep, each over every tile of the batch, so
-// each stage gets its own register allocation (the single fused kernel holds every stage's code and spills) while
-// the scene types still share each launch. Stage P: 0 = step_1 dynamics, 1 = equality rows + collision + inequality
-// rows (GJK/EPA pairs deferred), 2 = the GJK/EPA pass and rows of deferred envs, 3 = Newton solve, 4 = step_2. The
-// device functions and their order per env are the plain step's, so results are bitwise those of gsr_step.
-template <int P>
-__device__ __forceinline__ void phase_tile(const gsr_model& m, const gsr_state& s, int env0, int n_here, char* lds) {
-  const int lane = threadIdx.x;
-  const bool mine = lane < n_here;
-  const bool wave = P == 0 || P == 4 ? m.dyn_variant == 1 : P == 1 ? m.collide_variant == 1 : P == 3 ? m.solver_variant == 1 : true;
-  if (!wave) {
-    if (!mine) return;
-    Env e(m, s, env0 + lane);
-    if (P == 0) {
-      dev_smooth_dynamics(e);
-    } else if (P == 1) {
-      dev_equality_rows(e);
-      dev_collision(e);
-      if (!gjk_pending(e)) dev_inequality_rows(e);
-    } else if (P == 3) {
-      dev_solve(e);
-    } else if (P == 4) {
-      dev_integrate(e);
-      dev_kinematics(e);
-    }
-    return;
-  }
-  for (int j = 0; j < n_here; j++) {
-    Env e(m, s, env0 + j);
-    if (P == 0) wave_step1(e);
-    if (P == 1) wave_collide(e, lds);
-    if (P == 2 && wave_gjk_fixup(e)) wave_rows(e);
-    if (P == 3) wave_solve(e, lds);
-    if (P == 4) wave_step2(e);
-    __syncthreads();
-  }
-}
-
-template <int P>
-__global__ void __launch_bounds__(kWave) k_phase(const char* __restrict__ table) {
-  extern __shared__ char lds[];
-  const BatchHeader* __restrict__ H = reinterpret_cast<const BatchHeader*>(table);
-  ConstTypes* types = (ConstTypes*)(table + kTypesOffset);
-  const BatchRange* __restrict__ R = reinterpret_cast<const BatchRange*>(table + kRangesOffset);
-  const int n_tiles = H->n_tiles, n_groups = H->n_groups;
-  if ((int)blockIdx.x >= n_tiles) return;
-  const int tile = xcd_env(blockIdx.x, n_tiles);
-  int lo = 0, hi = n_groups - 1;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (tile < R[mid].tile_end) hi = mid;
-    else lo = mid + 1;
-  }
-  const BatchRange g = R[lo];
-  const int T = g.envs_per_tile;
-  const int first = (tile - g.tile_begin) * T;
-  const int n_here = min(T, g.n_envs - first);
-  const int k = __builtin_amdgcn_readfirstlane(g.type);
-  const FusedTypes* ty = (const FusedTypes*)types;
-  phase_tile<P>(ty->m[k], ty->s[k], g.env_begin + first, n_here, lds);
-}
-
-// W = resident wavefronts per SIMD the register allocation must allow (2: 256 VGPRs, 4: 128 VGPRs);
-// G = GJK/EPA inside the launch (1) or deferred to k_gjk_resume (0)
-template <int W, int G>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(W, 8)))
-k_fused(const char* __restrict__ table, uint32_t* queue, int persistent) {
-  extern __shared__ char lds[];
-  const BatchHeader* __restrict__ H = reinterpret_cast<const BatchHeader*>(table);
-  ConstTypes* types = (ConstTypes*)(table + kTypesOffset);
-  const BatchRange* __restrict__ R = reinterpret_cast<const BatchRange*>(table + kRangesOffset);
-  const int n_tiles = H->n_tiles, n_groups = H->n_groups;
-  if (!persistent) {
-    if ((int)blockIdx.x < n_tiles) fused_tile<G != 0>(types, R, n_groups, xcd_env(blockIdx.x, n_tiles), lds);
-    return;
-  }
-  for (;;) {
-    uint32_t t = 0;
-    if (threadIdx.x == 0) t = atomicAdd(&queue[0], 1u);
-    const int tile = __builtin_amdgcn_readfirstlane((int)__shfl((int)t, 0, kWave));
-    if (tile >= n_tiles) break;
-    fused_tile<G != 0>(types, R, n_groups, tile, lds);
-    __syncthreads();
-  }
-  // the last workgroup out resets the queue for the next launch (no host memset, so graph replays stay valid)
-  if (threadIdx.x == 0) {
-    __threadfence();
-    uint32_t d = atomicAdd(&queue[1], 1u);
-    if (d == gridDim.x - 1) {
-      atomicExch(&queue[0], 0u);
-      atomicExch(&queue[1], 0u);
-    }
-  }
-}
 
 __global__ void k_errno_reduce(gsr_state s, uint32_t* out) {
   // one block of 256 threads ORs all env words
@@ -507,20 +208,10 @@ void launch_one(const gsr_model* m, const gsr_state* s, int k, hipStream_t st) {
       else hipLaunchKernelGGL(k_step1, g, dim3(kBlock), 0, st, *m, *s, 0);
       break;
     case 1:
-      if (m->collide_variant == 1)
-        hipLaunchKernelGGL(k_collide_wave, dim3(s->n_envs), dim3(kWave), wave_collide_lds_bytes(m->n_geoms), st, *m,
-                           *s);
-      else
-        hipLaunchKernelGGL(k_collide, g, dim3(kBlock), 0, st, *m, *s);
-      hipLaunchKernelGGL(k_gjk_fixup, dim3(s->n_envs), dim3(kWave), 0, st, *m, *s);
+      launch_collide(*m, *s, st);
       break;
     case 2:
-      if (m->solver_variant == 1)
-        hipLaunchKernelGGL(k_solve_wave, dim3(s->n_envs), dim3(kWave),
-                           wave_lds_bytes(m->n_dofs, m->n_entities, m->wave_h_floats), st,
-                           *m, *s);
-      else
-        hipLaunchKernelGGL(k_solve, g, dim3(kBlock), 0, st, *m, *s);
+      launch_solve(*m, *s, st);
       break;
     default:
       if (m->dyn_variant == 1) hipLaunchKernelGGL(k_step2_wave, dim3(s->n_envs), dim3(kWave), 0, st, *m, *s);
@@ -535,6 +226,46 @@ int run_substep(const gsr_model* m, const gsr_state* s, hipStream_t st) {
 }
 
 }  // namespace
+
+namespace gsr {
+
+const void* fused_kernel(int W, int G) {
+  if (W == 4) return G ? fused_kernel_w4_g1() : fused_kernel_w4_g0();
+  return G ? fused_kernel_w2_g1() : fused_kernel_w2_g0();
+}
+
+void launch_fused(int W, int G, dim3 grid, unsigned lds, hipStream_t st, const char* table, uint32_t* queue,
+                  int persistent) {
+  if (W == 4) {
+    if (G) launch_fused_w4_g1(grid, lds, st, table, queue, persistent);
+    else launch_fused_w4_g0(grid, lds, st, table, queue, persistent);
+  } else {
+    if (G) launch_fused_w2_g1(grid, lds, st, table, queue, persistent);
+    else launch_fused_w2_g0(grid, lds, st, table, queue, persistent);
+  }
+}
+
+const void* phase_kernel(int P) {
+  switch (P) {
+    case 0: return phase_kernel_0();
+    case 1: return phase_kernel_1();
+    case 2: return phase_kernel_2();
+    case 3: return phase_kernel_3();
+    default: return phase_kernel_4();
+  }
+}
+
+void launch_phase(int P, dim3 grid, unsigned lds, hipStream_t st, const char* table) {
+  switch (P) {
+    case 0: launch_phase_0(grid, lds, st, table); break;
+    case 1: launch_phase_1(grid, lds, st, table); break;
+    case 2: launch_phase_2(grid, lds, st, table); break;
+    case 3: launch_phase_3(grid, lds, st, table); break;
+    default: launch_phase_4(grid, lds, st, table); break;
+  }
+}
+
+}  // namespace gsr
 
 extern "C" {
 
@@ -688,6 +419,25 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
     ranges[g] = BatchRange{type, G.env_begin, G.n_envs, G.envs_per_tile, tiles, tiles + n_t, 0, 0};
     tiles += n_t;
   }
+  // Groups over one state must agree on the execution variants (the constraint-row and Hessian layouts follow the
+  // solver variant, so two variants on one state would write each other's envs) and must not share envs.
+  {
+    std::vector<int> order(n_groups);
+    for (int g = 0; g < n_groups; g++) order[g] = g;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      const uintptr_t pa = (uintptr_t)groups[a].state.qpos, pb = (uintptr_t)groups[b].state.qpos;
+      return pa != pb ? pa < pb : groups[a].env_begin < groups[b].env_begin;
+    });
+    for (int i = 1; i < n_groups; i++) {
+      const gsr_group& A = groups[order[i - 1]];
+      const gsr_group& B = groups[order[i]];
+      if (A.state.qpos != B.state.qpos) continue;
+      if (A.model.solver_variant != B.model.solver_variant || A.model.collide_variant != B.model.collide_variant ||
+          A.model.dyn_variant != B.model.dyn_variant)
+        return GSR_E_ARG;
+      if (B.env_begin < A.env_begin + A.n_envs) return GSR_E_ARG;
+    }
+  }
   BatchHeader h{n_groups, tiles, 0u, 0u};
   FusedTypes ty;
   std::memset(&ty, 0, sizeof(ty));
@@ -707,10 +457,10 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return GSR_E_LAUNCH;
   if (info->phased) {
     const int lc = phase_lds(out, 1), ls = phase_lds(out, 3);
-    if (lc > 0 && hipFuncSetAttribute((const void*)k_phase<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lc) !=
+    if (lc > 0 && hipFuncSetAttribute(phase_kernel(1), hipFuncAttributeMaxDynamicSharedMemorySize, lc) !=
                       hipSuccess)
       return GSR_E_LAUNCH;
-    if (ls > 0 && hipFuncSetAttribute((const void*)k_phase<3>, hipFuncAttributeMaxDynamicSharedMemorySize, ls) !=
+    if (ls > 0 && hipFuncSetAttribute(phase_kernel(3), hipFuncAttributeMaxDynamicSharedMemorySize, ls) !=
                       hipSuccess)
       return GSR_E_LAUNCH;
     out.n_groups = n_groups;
@@ -726,8 +476,7 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   }
   const int W = info->waves_per_simd == 4 ? 4 : 2;
   const int G = info->gjk_inline ? 1 : 0;
-  const void* fn = W == 4 ? (G ? (const void*)k_fused<4, 1> : (const void*)k_fused<4, 0>)
-                          : (G ? (const void*)k_fused<2, 1> : (const void*)k_fused<2, 0>);
+  const void* fn = fused_kernel(W, G);
   if (lds > 0 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     return GSR_E_LAUNCH;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWave, lds) != hipSuccess) return GSR_E_LAUNCH;
@@ -743,43 +492,47 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   return GSR_OK;
 }
 
-int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream) {
-  if (!dev_table || !info || n_substeps < 1 || info->grid < 1 || info->n_tiles < 1 || info->n_types < 1 ||
-      info->n_types > GSR_MAX_TYPES)
-    return GSR_E_ARG;
+static bool fused_info_ok(const void* dev_table, const gsr_batch_info* info) {
+  return dev_table && info && info->grid >= 1 && info->n_tiles >= 1 && info->n_types >= 1 &&
+         info->n_types <= GSR_MAX_TYPES;
+}
+
+int gsr_fused_kernel_count(const gsr_batch_info* info) {
+  if (!info) return GSR_E_ARG;
+  if (info->phased) return 5;
+  return info->gjk_inline ? 1 : 1 + info->n_types;
+}
+
+int gsr_launch_fused_kernel(const void* dev_table, const gsr_batch_info* info, int32_t k, hipStream_t stream) {
+  if (!fused_info_ok(dev_table, info)) return GSR_E_ARG;
+  const int n = gsr_fused_kernel_count(info);
+  if (k < 0 || k >= n) return GSR_E_ARG;
   const char* t = static_cast<const char*>(dev_table);
-  uint32_t* queue = reinterpret_cast<uint32_t*>(const_cast<char*>(t) + 2 * sizeof(int32_t));
   if (info->phased) {
-    const dim3 grid(info->n_tiles), block(kWave);
-    const int lc = phase_lds(*info, 1), ls = phase_lds(*info, 3);
-    for (int i = 0; i < n_substeps; i++) {
-      hipLaunchKernelGGL(k_phase<0>, grid, block, 0, stream, t);
-      hipLaunchKernelGGL(k_phase<1>, grid, block, lc, stream, t);
-      hipLaunchKernelGGL(k_phase<2>, grid, block, 0, stream, t);
-      hipLaunchKernelGGL(k_phase<3>, grid, block, ls, stream, t);
-      hipLaunchKernelGGL(k_phase<4>, grid, block, 0, stream, t);
-    }
-    return launch_status();
-  }
-  for (int i = 0; i < n_substeps; i++) {
-    const dim3 grid(info->grid), block(kWave);
-    const int pers = (int)info->persistent;
-    if (info->waves_per_simd == 4) {
-      if (info->gjk_inline) hipLaunchKernelGGL((k_fused<4, 1>), grid, block, info->lds_bytes, stream, t, queue, pers);
-      else hipLaunchKernelGGL((k_fused<4, 0>), grid, block, info->lds_bytes, stream, t, queue, pers);
-    } else {
-      if (info->gjk_inline) hipLaunchKernelGGL((k_fused<2, 1>), grid, block, info->lds_bytes, stream, t, queue, pers);
-      else hipLaunchKernelGGL((k_fused<2, 0>), grid, block, info->lds_bytes, stream, t, queue, pers);
-    }
-    if (info->gjk_inline) continue;
-    for (int k = 0; k < info->n_types; k++) {
-      const gsr_model& m = info->models[k];
-      const gsr_state& st = info->states[k];
-      const size_t lds = m.solver_variant == 1 ? wave_lds_bytes(m.n_dofs, m.n_entities, m.wave_h_floats) : 0;
-      hipLaunchKernelGGL(k_gjk_resume, dim3(st.n_envs), dim3(kWave), lds, stream, m, st);
-    }
+    const int lds = k == 1 ? phase_lds(*info, 1) : k == 3 ? phase_lds(*info, 3) : 0;
+    launch_phase(k, dim3(info->n_tiles), lds, stream, t);
+  } else if (k == 0) {
+    uint32_t* queue = reinterpret_cast<uint32_t*>(const_cast<char*>(t) + 2 * sizeof(int32_t));
+    launch_fused(info->waves_per_simd == 4 ? 4 : 2, info->gjk_inline ? 1 : 0, dim3(info->grid), info->lds_bytes,
+                 stream, t, queue, (int)info->persistent);
+  } else {
+    const gsr_model& m = info->models[k - 1];
+    const gsr_state& st = info->states[k - 1];
+    const size_t lds = m.solver_variant == 1 ? wave_lds_bytes(m.n_dofs, m.n_entities, m.wave_h_floats) : 0;
+    launch_gjk_resume(m, st, (unsigned)lds, stream);
   }
   return launch_status();
+}
+
+int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream) {
+  if (!fused_info_ok(dev_table, info) || n_substeps < 1) return GSR_E_ARG;
+  const int n = gsr_fused_kernel_count(info);
+  for (int i = 0; i < n_substeps; i++)
+    for (int k = 0; k < n; k++) {
+      const int rc = gsr_launch_fused_kernel(dev_table, info, k, stream);
+      if (rc) return rc;
+    }
+  return GSR_OK;
 }
 
 int gsr_graph_capture(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream,

@@ -290,6 +290,21 @@ class RigidSolver:
         if self._substep % RATE_CHECK_ERRNO == 0:
             self.check_errno()
 
+    def set_variants(self, solver_variant: Optional[int] = None, collide_variant: Optional[int] = None,
+                     dyn_variant: Optional[int] = None):
+        """Switch the execution variants of this solver's kernels (between substeps: every variant reads and writes
+        the same persistent state; only the per-substep constraint rows and Hessian take the solver variant's
+        layout, and every later launch and readback then uses it)."""
+        d = self.dm.desc
+        if solver_variant is not None:
+            if int(solver_variant) == 1 and self.model.n_entities > WAVE_MAX_ENTITIES:
+                raise ValueError(f"wavefront solver supports at most {WAVE_MAX_ENTITIES} entities")
+            d.solver_variant = int(solver_variant)
+        if collide_variant is not None:
+            d.collide_variant = int(collide_variant)
+        if dyn_variant is not None:
+            d.dyn_variant = int(dyn_variant)
+
     def launch_kernel(self, k: int):
         _lib.check(self._lib.gsr_launch_kernel(self.dm.desc, self.st.desc, k, self._stream()), "gsr_launch_kernel")
 

@@ -213,11 +213,19 @@ typedef struct gsr_batch_info {
 int64_t gsr_batch_table_bytes(int32_t n_groups);
 /* Validate groups, assign tiles and copy the table to `dev_table` (synchronously); fills `info` (whose `persistent`
  * and `waves_per_simd` are read as inputs), which the launch functions need. Groups are stepped in the given order,
- * so list the most expensive first. No launch that uses `dev_table` may be in flight. */
+ * so list the most expensive first. Groups over one state (same `state.qpos`) must use the same solver, collide
+ * and dyn variants and disjoint env ranges (GSR_E_ARG otherwise). No launch that uses `dev_table` may be in
+ * flight. */
 int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table, int64_t table_bytes,
                      gsr_batch_info* info);
 /* n_substeps fused launches of the uploaded batch on `stream`. */
 int gsr_step_fused(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream);
+/* Kernels one fused substep launches, in order: the phased form's five stage kernels, or the single fused kernel
+ * followed (GJK/EPA deferred) by one resume kernel per scene type. No counterpart in the reference: used to time
+ * each kernel of the fused substep on the caller's stream. */
+int gsr_fused_kernel_count(const gsr_batch_info* info);
+/* Launch kernel k (0 <= k < gsr_fused_kernel_count) of one fused substep; gsr_step_fused launches exactly these. */
+int gsr_launch_fused_kernel(const void* dev_table, const gsr_batch_info* info, int32_t k, hipStream_t stream);
 /* Capture n_substeps fused launches into a hipGraph (stream must not be the legacy default stream) and return the
  * instantiated executable; launch it with gsr_graph_launch, free it with gsr_graph_destroy. */
 int gsr_graph_capture(const void* dev_table, const gsr_batch_info* info, int32_t n_substeps, hipStream_t stream,
