@@ -87,6 +87,7 @@ class Part:
         self.snap = None
         self.gen_state = None
         self.torch = torch
+        self.n_reset = 0  # envs restarted after an errno bit the reference raises on
 
     @property
     def n(self):
@@ -127,14 +128,19 @@ class Runner:
             self.controls()
             self.plain_step()
             self.step += 1
+            if self.step % 10 == 0:
+                self.check()
             if self.step % 50 == 0:
                 log(f"bench: advanced to step {self.step}")
         self.torch.cuda.synchronize()
         self.check()
 
     def check(self):
+        """The errno poll of Simulator.step (every 10 steps), with the episode-termination rule instead of an
+        exception: a diverged env (NaN, overflow) restarts from its initial state and is counted."""
+        from genesis_amd.workloads import reset_failed_envs
         for p in self.parts:
-            p.S.check_errno()
+            p.n_reset += reset_failed_envs(p.w, p.M, p.S)
 
     def snapshot(self):
         self.torch.cuda.synchronize()
@@ -159,6 +165,7 @@ class Runner:
             one_step(None)
             self.step += 1
         self.restore()
+        resets0 = sum(p.n_reset for p in self.parts)
         t0 = time.perf_counter()
         for i in range(steps):
             self.controls()
@@ -168,7 +175,8 @@ class Runner:
                 self.check()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        log(f"bench: {label}: {steps} steps in {dt:.3f} s")
+        self.last_resets = sum(p.n_reset for p in self.parts) - resets0
+        log(f"bench: {label}: {steps} steps in {dt:.3f} s, {self.last_resets} env(s) reset")
         return dt
 
 
@@ -236,19 +244,21 @@ def cpu_baseline(parts, first_step, seconds=4.0):
     trajectory as the GPU run; a mixed batch's rate is the split-weighted combination (time per mixed step =
     sum over scene types of envs / rate)."""
     T = cpu_threads()
+    n_all, n_one = 8 * T, 8  # eight envs per thread, so thread start-up does not dominate a cheap scene's step
     per, t_all, t_one = {}, 0.0, 0.0
     for p in parts:
-        r_all, n_all = oracle_rate(p, first_step, seconds, T, T)
-        r_one, n_one = oracle_rate(p, first_step, seconds / 2, 1, max(1, T // 4))
+        r_all, _ = oracle_rate(p, first_step, seconds, T, n_all)
+        r_one, _ = oracle_rate(p, first_step, seconds / 2, 1, n_one)
         per[p.name] = {"all_cores": round(r_all, 1), "one_thread": round(r_one, 1)}
         t_all += p.n / r_all
         t_one += p.n / r_one
     n = sum(p.n for p in parts)
     return {"value": round(n / t_all, 1), "unit": "env-steps/s", "cores": T, "kind": "port",
             "value_one_thread": round(n / t_one, 1), "per_scene": per, "host": host_cpu(),
-            "sample": f"CPU oracle (FP32), per scene type {T} envs on {T} threads and {max(1, T // 4)} envs on 1 "
-                      f"thread, steps {first_step}.. of each scene's seeded trajectory for >= {seconds:g} s "
-                      f"(1 thread: >= {seconds / 2:g} s); mixed rate = split-weighted"}
+            "sample": f"CPU oracle (FP32), per scene type {n_all} envs on {T} threads and {n_one} envs on 1 "
+                      f"thread, steps {first_step}.. of each scene's seeded trajectory (untimed up to step "
+                      f"{first_step}), timed >= {seconds:g} s (1 thread: >= {seconds / 2:g} s); a mixed batch's "
+                      f"rate is the split-weighted combination"}
 
 
 def plain_kernel_events(runner, K):
@@ -290,6 +300,7 @@ def bench_mixed(args):
     W = args.warmup
     log(f"bench: mixed {[(p.name, p.n) for p in parts]}, window steps {first}..{first + K - 1}")
     runner.advance(first)
+    advance_resets = {p.name: p.n_reset for p in parts}
     runner.snapshot()
 
     # the fused plan: tiles and variants per scene type and the launch form, tuned on the snapshot state
@@ -315,7 +326,7 @@ def bench_mixed(args):
         one, kms = plain_kernel_events(runner, K)
         t = runner.timed(one, W, K, "plain (fused plan's variants)")
         res["plain"] = {"value": round(n_total * K / t, 1), "ms_per_step": round(1e3 * t / K, 4),
-                        "kernel_ms": kms(),
+                        "kernel_ms": kms(), "envs_reset": runner.last_resets,
                         "variants": {p.name: ["lane", "wave"][variants[id(p.S)][0]] for p in parts}}
         final_plain = [(p.S.st.t["qpos"].clone(), p.S.st.t["vel"].clone()) for p in parts]
     if "default" in forms:
@@ -324,7 +335,7 @@ def bench_mixed(args):
         one, kms = plain_kernel_events(runner, K)
         t = runner.timed(one, W, K, "plain (default variants)")
         res["plain_default_variants"] = {"value": round(n_total * K / t, 1), "ms_per_step": round(1e3 * t / K, 4),
-                                         "kernel_ms": kms(),
+                                         "kernel_ms": kms(), "envs_reset": runner.last_resets,
                                          "variants": {p.name: ["lane", "wave"][p.default_variants[0]] for p in parts}}
     fb = FB.FusedBatch(plan, **launch)
     knames = fb.kernel_names()
@@ -339,6 +350,7 @@ def bench_mixed(args):
             ev[i][1].record()
 
     t_fused = runner.timed(fused_step, W, K, "fused graph")
+    fused_resets = runner.last_resets
     substep_ms = sum(a.elapsed_time(b) for a, b in ev) / K
     bitwise = None
     if "plain" in forms:
@@ -385,6 +397,7 @@ def bench_mixed(args):
         "speedup_vs_plain_default_variants": (round(value / res["plain_default_variants"]["value"], 3)
                                               if "plain_default_variants" in res else None),
         "fused_bitwise_equals_plain": bitwise,
+        "envs_reset": {"advance": advance_resets, "fused": fused_resets},
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "fused_substep_gpu_ms": round(substep_ms, 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
@@ -432,6 +445,7 @@ def bench_single(args):
                    "n_dofs": p.M.n_dofs, "n_geoms": p.M.n_geoms,
                    "solver": ["lane-per-env", "wavefront-per-env"][p.S.dm.desc.solver_variant]},
         "kernel_ms": k_ms,
+        "envs_reset": {"advance": p.n_reset - runner.last_resets, "timed": runner.last_resets},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["bytes"] if tr else None, "traffic_source": tr["source"] if tr else None,
@@ -461,8 +475,7 @@ def bench_serve(args):
     for g in tuned:
         for p in pools.values():
             if p.S is g.solver:
-                p.envs_per_tile = g.envs_per_tile
-                p.S.set_variants(g.solver_variant, g.collide_variant)
+                p.set_execution(g.envs_per_tile, g.solver_variant, g.collide_variant)
                 p.ms_per_env = time_group(g, steps=3, warmup=1, **launch) / n_tune
     log("serve: ms per env per tick " + ", ".join(f"{p.scene} {p.ms_per_env * 1e3:.3f} us" for p in pools.values()))
     budget = int(args.serve_budget_gb * 2**30)

@@ -19,10 +19,12 @@ EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_group_siz
            "gsr_step", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno",
            "gsr_get_contacts", "gsr_links_contact_force", "gsr_get_equalities", "gsr_run_stage", "gsr_launch_kernel", "gsr_batch_table_bytes",
            "gsr_batch_upload", "gsr_step_fused", "gsr_fused_kernel_count", "gsr_launch_fused_kernel",
-           "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy"]
+           "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy",
+           "gsr_serve_host_alloc", "gsr_serve_host_free", "gsr_serve_state_bytes", "gsr_serve_table_bytes",
+           "gsr_serve_clock_khz", "gsr_serve_init", "gsr_serve_tick", "gsr_serve_graph_capture"]
 
 _CTYPE = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "float": ctypes.c_float,
-          "int64_t": ctypes.c_int64}
+          "int64_t": ctypes.c_int64, "uint64_t": ctypes.c_uint64, "double": ctypes.c_double}
 
 
 def _defines():
@@ -78,6 +80,25 @@ class GsrBatchInfo(ctypes.Structure):
     _fields_ = _parse_struct("gsr_batch_info")
 
 
+class GsrServePool(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_serve_pool")
+
+
+class GsrServeRequest(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_serve_request")
+
+
+class GsrServeDone(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_serve_done")
+
+
+_STRUCTS.update(gsr_serve_request=GsrServeRequest, gsr_serve_done=GsrServeDone)
+
+
+class GsrServeHost(ctypes.Structure):
+    _fields_ = _parse_struct("gsr_serve_host")
+
+
 MODEL_POINTERS = [n for n, t in GsrModel._fields_ if t is ctypes.c_void_p]
 STATE_POINTERS = [n for n, t in GsrState._fields_ if t is ctypes.c_void_p]
 
@@ -123,6 +144,17 @@ def load(path: str = LIB_PATH):
                                     P(ctypes.c_void_p)]
     L.gsr_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.gsr_graph_destroy.argtypes = [ctypes.c_void_p]
+    L.gsr_serve_host_alloc.argtypes = [P(P(GsrServeHost))]
+    L.gsr_serve_host_free.argtypes = [P(GsrServeHost)]
+    L.gsr_serve_state_bytes.restype = ctypes.c_int64
+    L.gsr_serve_state_bytes.argtypes = [ctypes.c_int32]
+    L.gsr_serve_table_bytes.restype = ctypes.c_int64
+    L.gsr_serve_table_bytes.argtypes = [ctypes.c_int32]
+    L.gsr_serve_init.argtypes = [P(GsrServePool), ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                 ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, P(GsrBatchInfo), ctypes.c_void_p]
+    L.gsr_serve_tick.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(GsrBatchInfo), P(GsrServeHost), ctypes.c_void_p]
+    L.gsr_serve_graph_capture.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(GsrBatchInfo), P(GsrServeHost),
+                                          ctypes.c_void_p, P(ctypes.c_void_p)]
     if (L.gsr_model_size() != ctypes.sizeof(GsrModel) or L.gsr_state_size() != ctypes.sizeof(GsrState)
             or L.gsr_group_size() != ctypes.sizeof(GsrGroup)
             or L.gsr_batch_info_size() != ctypes.sizeof(GsrBatchInfo)):

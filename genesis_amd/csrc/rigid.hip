@@ -386,6 +386,8 @@ static int phase_lds(const gsr_batch_info& b, int phase) {
   return (int)lds;
 }
 
+static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, size_t lds, gsr_batch_info* info);
+
 int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table, int64_t table_bytes,
                      gsr_batch_info* info) {
   if (!groups || !dev_table || !info || n_groups < 1 || n_groups > GSR_MAX_GROUPS) return GSR_E_ARG;
@@ -452,6 +454,12 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   if (hipMemcpy(static_cast<char*>(dev_table) + kTypesOffset, &ty, sizeof(ty), hipMemcpyHostToDevice) != hipSuccess)
     return GSR_E_LAUNCH;
   if (hipMemcpy(dev_table, &h, 2 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) return GSR_E_LAUNCH;
+  return finish_launch_info(out, n_groups, tiles, lds, info);
+}
+
+// Launch form of a batch whose types are in `out`: kernel attributes (dynamic LDS), grid and the form's flags, read
+// from `info` and written back to it.
+static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, size_t lds, gsr_batch_info* info) {
   int dev = 0, n_cu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return GSR_E_LAUNCH;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return GSR_E_LAUNCH;
@@ -564,6 +572,116 @@ int gsr_graph_launch(void* graph_exec, hipStream_t stream) {
 int gsr_graph_destroy(void* graph_exec) {
   if (!graph_exec) return GSR_E_ARG;
   return hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)) == hipSuccess ? GSR_OK : GSR_E_LAUNCH;
+}
+
+// ---- serving form -------------------------------------------------------------------------------------------------
+int gsr_serve_host_alloc(gsr_serve_host** out) {
+  if (!out) return GSR_E_ARG;
+  *out = nullptr;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, sizeof(gsr_serve_host), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess || !p)
+    return GSR_E_LAUNCH;
+  std::memset(p, 0, sizeof(gsr_serve_host));
+  *out = static_cast<gsr_serve_host*>(p);
+  return GSR_OK;
+}
+
+int gsr_serve_host_free(gsr_serve_host* host) {
+  if (!host) return GSR_E_ARG;
+  return hipHostFree(host) == hipSuccess ? GSR_OK : GSR_E_LAUNCH;
+}
+
+int64_t gsr_serve_state_bytes(int32_t n_tiles) {
+  return n_tiles < 1 ? GSR_E_ARG : (int64_t)serve_state_bytes(n_tiles);
+}
+
+int64_t gsr_serve_table_bytes(int32_t n_tiles) {
+  return n_tiles < 1 ? GSR_E_ARG : (int64_t)serve_table_bytes(n_tiles);
+}
+
+int gsr_serve_clock_khz(void) {
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return GSR_E_LAUNCH;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return GSR_E_LAUNCH;
+  return khz;
+}
+
+int gsr_serve_init(const gsr_serve_pool* pools, int32_t n_pools, double mem_budget, double tick_budget,
+                   void* dev_state, int64_t state_bytes, void* dev_table, int64_t table_bytes, gsr_batch_info* info,
+                   hipStream_t stream) {
+  if (!pools || !dev_state || !dev_table || !info || n_pools < 1 || n_pools > GSR_MAX_TYPES) return GSR_E_ARG;
+  gsr_batch_info out;
+  std::memset(&out, 0, sizeof(out));
+  int tiles = 0;
+  size_t lds = 0;
+  for (int p = 0; p < n_pools; p++) {
+    const gsr_serve_pool& P = pools[p];
+    const int rc = check(&P.model, &P.state);
+    if (rc) return rc;
+    if (P.envs_per_tile < 1 || P.envs_per_tile > kWave || P.n_tiles < 1 ||
+        (int64_t)P.envs_per_tile * P.n_tiles > P.state.n_envs || !P.init_qpos)
+      return GSR_E_ARG;
+    if (P.ctrl_kind < 0 || P.ctrl_kind > 2) return GSR_E_ARG;
+    if (P.ctrl_kind != 0 && (P.ctrl_dof_begin < 0 || P.ctrl_dof_end > P.model.n_dofs ||
+                             P.ctrl_dof_begin >= P.ctrl_dof_end || (P.ctrl_kind == 1 && P.ctrl_period < 1) ||
+                             (P.ctrl_kind == 1 && (!P.ctrl_lo || !P.ctrl_hi)) || (P.ctrl_kind == 2 && !P.ctrl_base)))
+      return GSR_E_ARG;
+    for (int q = 0; q < p; q++)
+      if (pools[q].state.qpos == P.state.qpos) return GSR_E_ARG;  // one state per pool
+    out.models[p] = P.model;
+    out.states[p] = P.state;
+    if (P.model.solver_variant == 1)
+      lds = std::max(lds, wave_lds_bytes(P.model.n_dofs, P.model.n_entities, P.model.wave_h_floats));
+    if (P.model.collide_variant == 1) lds = std::max(lds, wave_collide_lds_bytes(P.model.n_geoms));
+    tiles += P.n_tiles;
+  }
+  out.n_types = n_pools;
+  if (state_bytes < (int64_t)serve_state_bytes(tiles) || table_bytes < (int64_t)serve_table_bytes(tiles))
+    return GSR_E_ARG;
+  const int rc = serve_init(pools, n_pools, mem_budget, tick_budget, dev_state, state_bytes, dev_table, table_bytes,
+                            stream);
+  if (rc) return rc;
+  // every tile may be active: the launches cover all of them (idle ones return at once)
+  return finish_launch_info(out, tiles, tiles, lds, info);
+}
+
+int gsr_serve_tick(void* dev_state, void* dev_table, const gsr_batch_info* info, gsr_serve_host* host,
+                   hipStream_t stream) {
+  if (!dev_state || !host || !fused_info_ok(dev_table, info)) return GSR_E_ARG;
+  gsr_serve_host* hd = nullptr;
+  if (hipHostGetDevicePointer((void**)&hd, host, 0) != hipSuccess || !hd) return GSR_E_ARG;
+  const int n_tiles = info->n_tiles;
+  launch_serve(0, dev_state, dev_table, hd, n_tiles, stream);
+  launch_serve(1, dev_state, dev_table, hd, n_tiles, stream);
+  launch_serve(2, dev_state, dev_table, hd, n_tiles, stream);
+  const int n = gsr_fused_kernel_count(info);
+  for (int k = 0; k < n; k++) {
+    const int rc = gsr_launch_fused_kernel(dev_table, info, k, stream);
+    if (rc) return rc;
+  }
+  launch_serve(3, dev_state, dev_table, hd, n_tiles, stream);
+  return launch_status();
+}
+
+int gsr_serve_graph_capture(void* dev_state, void* dev_table, const gsr_batch_info* info, gsr_serve_host* host,
+                            hipStream_t stream, void** graph_exec) {
+  if (!graph_exec || !stream) return GSR_E_ARG;
+  *graph_exec = nullptr;
+  if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return GSR_E_LAUNCH;
+  int rc = gsr_serve_tick(dev_state, dev_table, info, host, stream);
+  hipGraph_t graph = nullptr;
+  hipError_t err = hipStreamEndCapture(stream, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (err != hipSuccess || !graph) return GSR_E_LAUNCH;
+  hipGraphExec_t exec = nullptr;
+  err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (err != hipSuccess) return GSR_E_LAUNCH;
+  *graph_exec = exec;
+  return GSR_OK;
 }
 
 }  // extern "C"

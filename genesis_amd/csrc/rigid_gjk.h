@@ -112,6 +112,9 @@ __device__ inline void gjk_support_one(const gsr_model& m, int g, const GPose& P
           (db.z < 0.f ? -1.f : 1.f) * data[2] * 0.5f};
     vid = (vl.x > 0.f ? 1 : 0) + (vl.y > 0.f ? 2 : 0) + (vl.z > 0.f ? 4 : 0) + m.geom_vert_start[g];
     v = xform(vl, P.p, P.q);
+  } else if (m.geom_type[g] == GEOM_MESH) {
+    // the support field, not an exhaustive vertex search (that one is the MuJoCo-compatibility path)
+    v = dev_support_mesh(m, g, P, d, vid);
   } else {
     v = dev_support(m, g, P, d);
   }

@@ -284,5 +284,13 @@ void launch_fused(int W, int G, dim3 grid, unsigned lds, hipStream_t st, const c
 const void* phase_kernel(int P);
 void launch_phase(int P, dim3 grid, unsigned lds, hipStream_t st, const char* table);
 void launch_gjk_resume(const gsr_model& m, const gsr_state& s, unsigned lds, hipStream_t st);
+// serving form (rigid_serve.hip): scheduler state and tile table sizes, set-up, and the tick's own kernels
+// (stage 0 schedule, 1 init, 2 control, 3 post)
+size_t serve_state_bytes(int n_tiles);
+size_t serve_table_bytes(int n_tiles);
+int serve_init(const gsr_serve_pool* pools, int n_pools, double mem_budget, double tick_budget, void* dev_state,
+               int64_t state_bytes, void* dev_table, int64_t table_bytes, hipStream_t stream);
+void launch_serve(int stage, void* dev_state, void* dev_table, gsr_serve_host* host_dev, int n_tiles,
+                  hipStream_t stream);
 
 }  // namespace gsr

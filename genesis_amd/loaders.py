@@ -8,9 +8,10 @@ Semantics follow the reference's loaders where they are pure bookkeeping:
   kp = -gear*biasprm[1], kv = -gear*biasprm[2] and forcerange (genesis/utils/mjcf.py:295-370), JOINT equalities.
 
 The reference routes both formats through MuJoCo's compiler (rigid_entity.py:366-418), which is not available here,
-so these loaders restate the documented file semantics directly. Collision meshes are not convexified here: a mesh
-collision geom is replaced by a capsule fitted to its vertices, and a URDF cylinder by a capsule of the same radius
-and length (the scene descriptions of the benchmark workloads specify capsule/sphere collision geometry).
+so these loaders restate the documented file semantics directly. An MJCF collision mesh becomes a MESH geom holding
+the vertices of its convex hull in the mesh file's frame (the reference convexifies collision meshes with a hull,
+genesis/utils/mesh.py:299-475; its optional decimation above 500 faces is not applied); a URDF cylinder becomes a
+capsule of the same radius and length (the quadruped workload specifies capsule/sphere collision geometry).
 """
 from __future__ import annotations
 
@@ -23,7 +24,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from .model import (
-    EQ_JOINT, GEOM_BOX, GEOM_CAPSULE, GEOM_SPHERE, JOINT_FIXED, JOINT_FREE, JOINT_PRISMATIC, JOINT_REVOLUTE,
+    EQ_JOINT, GEOM_BOX, GEOM_CAPSULE, GEOM_MESH, GEOM_SPHERE, JOINT_FIXED, JOINT_FREE, JOINT_PRISMATIC, JOINT_REVOLUTE,
     JOINT_SPHERICAL, EntitySpec, EqualitySpec, GeomSpec, JointSpec, LinkSpec, R_to_quat, quat_mul, quat_to_R,
     rpy_to_quat,
 )
@@ -252,7 +253,8 @@ def _mj_quat(attrs):
     return np.array([1.0, 0, 0, 0])
 
 
-def parse_mjcf(path: str, mesh_to_capsule: bool = True) -> EntitySpec:
+def parse_mjcf(path: str, mesh_to_capsule: bool = False) -> EntitySpec:
+    """mesh_to_capsule: replace each collision mesh by a capsule fitted to its vertices instead of its convex hull."""
     root = ET.parse(path).getroot()
     base_dir = os.path.dirname(path)
     comp = root.find("compiler")
@@ -348,6 +350,10 @@ def parse_mjcf(path: str, mesh_to_capsule: bool = True) -> EntitySpec:
                 c, q, r, length = fit_capsule(v)
                 Rg = quat_to_R(gq)
                 geoms.append(GeomSpec(GEOM_CAPSULE, gpos + Rg @ c, quat_mul(gq, q), (r, length), friction=fric))
+            elif gtype == "mesh":
+                from .meshes import convex_hull_vertices
+                hull = convex_hull_vertices(read_mesh_vertices(meshes[a["mesh"]]))
+                geoms.append(GeomSpec(GEOM_MESH, gpos, gq, (), friction=fric, verts=hull))
         my_idx = len(links)
         links.append(LinkSpec(el.get("name"), parent_idx, pos, quat, jspecs, geoms, ipos, iquat, I, mass))
         for j in jspecs:
@@ -435,7 +441,7 @@ def entity_to_json(ent: EntitySpec) -> dict:
                                  sol_params=arr(j.sol_params), init_qpos=arr(j.init_qpos)) for j in l.joints],
                     geoms=[dict(type=g.type, pos=arr(g.pos), quat=arr(g.quat), data=list(map(float, g.data)),
                                 friction=g.friction, sol_params=arr(g.sol_params), contype=g.contype,
-                                conaffinity=g.conaffinity) for g in l.geoms]) for l in ent.links],
+                                conaffinity=g.conaffinity, verts=arr(g.verts)) for g in l.geoms]) for l in ent.links],
         equalities=[dict(type=e.type, obj1=e.obj1, obj2=e.obj2, data=arr(e.data), sol_params=arr(e.sol_params))
                     for e in ent.equalities],
     )
@@ -451,7 +457,7 @@ def entity_from_json(d: dict) -> EntitySpec:
                         j["stiffness"], j["frictionloss"], j["kp"], j["kv"], a(j["force_range"]), a(j["sol_params"]),
                         a(j["init_qpos"])) for j in l["joints"]]
         gs = [GeomSpec(g["type"], a(g["pos"]), a(g["quat"]), tuple(g["data"]), g["friction"], a(g["sol_params"]),
-                       g["contype"], g["conaffinity"]) for g in l["geoms"]]
+                       g["contype"], g["conaffinity"], a(g.get("verts"))) for g in l["geoms"]]
         links.append(LinkSpec(l["name"], l["parent"], a(l["pos"]), a(l["quat"]), js, gs, a(l["inertial_pos"]),
                               a(l["inertial_quat"]), a(l["inertial_i"]), l["inertial_mass"]))
     eqs = [EqualitySpec(e["type"], e["obj1"], e["obj2"], a(e["data"]), a(e["sol_params"])) for e in d["equalities"]]

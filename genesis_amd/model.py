@@ -125,6 +125,7 @@ class GeomSpec:
     sol_params: Optional[np.ndarray] = None
     contype: int = 1
     conaffinity: int = 1
+    verts: Optional[np.ndarray] = None  # mesh: vertices of the convex collision hull, geom frame
 
 
 @dataclass
@@ -243,6 +244,9 @@ def geom_mass_properties(g: GeomSpec, rho: float):
         Izz = 0.5 * mc * r * r + 0.4 * ms * r * r
         Ixx = mc * (3 * r * r + L * L) / 12.0 + ms * (0.4 * r * r + L * L / 4.0 + 3.0 * L * r / 8.0)
         return mc + ms, np.zeros(3), np.diag([Ixx, Ixx, Izz])
+    if t == GEOM_MESH:
+        from .meshes import hull_mass_properties
+        return hull_mass_properties(g.verts, rho)
     raise ValueError(f"no mass model for geom type {t}")
 
 
@@ -379,6 +383,11 @@ def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel
                 elif gs_.type == GEOM_CAPSULE:
                     verts = capsule_vertices(gs_.data[0], gs_.data[1])
                     data = [gs_.data[0], gs_.data[1]]
+                elif gs_.type == GEOM_MESH:
+                    if gs_.verts is None or len(gs_.verts) < 4:
+                        raise ValueError("a mesh geom needs the vertices of its convex hull")
+                    verts = np.asarray(gs_.verts, np.float64)
+                    data = []
                 else:
                     raise ValueError(f"unsupported geom type {gs_.type}")
                 fric = ent.friction if ent.friction is not None else (gs_.friction if gs_.friction is not None else 1.0)
@@ -510,6 +519,20 @@ def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel
         lo, hi = v.min(0), v.max(0)
         for c in range(8):
             M.geom_init_aabb[i_g, c] = [hi[0] if c & 4 else lo[0], hi[1] if c & 2 else lo[1], hi[2] if c & 1 else lo[2]]
+    # mesh support fields (support_field_decomp.py:24-94), from the float32 vertices: only mesh geoms look them up
+    from .meshes import SUPPORT_RES, support_field
+    cell_start, sv, sid, cells = np.zeros(G, i32), [], [], 0
+    for i_g in range(G):
+        if geoms[i_g]["type"] != GEOM_MESH:
+            continue
+        pos, vid = support_field(M.verts[vs[i_g]:ve[i_g]])
+        cell_start[i_g] = cells
+        sv.append(pos)
+        sid.append(vid)
+        cells += SUPPORT_RES * SUPPORT_RES
+    M.support_cell_start = cell_start
+    M.support_v = (np.concatenate(sv) if sv else np.zeros((1, 3))).astype(f32)
+    M.support_vid = (np.concatenate(sid) if sid else np.zeros(1)).astype(i32)
     # equalities
     M.eq_type = np.array([e["type"] for e in eqs], i32)
     M.eq_obj1 = np.array([e["obj1"] for e in eqs], i32)

@@ -38,7 +38,8 @@ MODEL_TABLES_I = [
     "link_parent", "link_root", "link_q_start", "link_q_end", "link_dof_start", "link_dof_end", "link_joint_start",
     "link_joint_end", "link_is_fixed", "joint_type", "joint_q_start", "joint_dof_start", "joint_dof_end", "dof_entity",
     "ent_link_start", "ent_link_end", "ent_dof_start", "ent_dof_end", "ent_mass_offset", "geom_type", "geom_link",
-    "geom_vert_start", "geom_vert_end", "eq_type", "eq_obj1", "eq_obj2", "pair_idx",
+    "geom_vert_start", "geom_vert_end", "eq_type", "eq_obj1", "eq_obj2", "pair_idx", "support_cell_start",
+    "support_vid",
 ]
 MODEL_TABLES_F = [
     "link_pos", "link_quat", "link_inertial_pos", "link_inertial_quat", "link_inertial_i", "link_inertial_mass",
@@ -46,6 +47,7 @@ MODEL_TABLES_F = [
     "dof_force_range", "dof_kp", "dof_kv", "dof_damping", "dof_armature", "dof_stiffness", "dof_frictionloss",
     "dof_invweight", "qpos0", "ent_gravity_compensation", "geom_pos", "geom_quat", "geom_data", "geom_friction",
     "geom_sol_params", "geom_center", "geom_init_aabb", "verts", "eq_data", "eq_sol_params", "mass_parent_mask",
+    "support_v",
 ]
 
 INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", "n_broad", "broad_pairs",
@@ -260,6 +262,7 @@ class RigidSolver:
         self.st = DeviceState(self.dm, self._B)
         self._errno_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._substep = 0
+        self._mass_valid = False  # mass_blk holds the last substep's mass matrix (none before the first substep)
         self.forward_kinematics()
 
     # -- raw launches -------------------------------------------------------------------------------------------
@@ -286,6 +289,7 @@ class RigidSolver:
 
     def substep(self):
         _lib.check(self._lib.gsr_step(self.dm.desc, self.st.desc, 1, self._stream()), "gsr_step")
+        self._mass_valid = True
         self._substep += 1
         if self._substep % RATE_CHECK_ERRNO == 0:
             self.check_errno()
@@ -452,21 +456,51 @@ class RigidSolver:
     # -- checkpoint (scene.py:1283-1344) -------------------------------------------------------------------------
     def checkpoint_arrays(self):
         """Host copies of every state array that outlives a substep (the rest is rewritten by each substep before
-        it is read), keyed "RigidSolver.<field>", like the reference's dump_ckpt_to_numpy."""
-        return {f"RigidSolver.{k}": v.detach().cpu().numpy() for k, v in self.st.t.items()
-                if k not in CHECKPOINT_SKIP}
+        it is read), keyed "RigidSolver.<field>", and of every model parameter table ("RigidSolver.model.<table>",
+        including per-env tables and the parameters setters changed, plus meaninertia), like the reference's
+        dump_ckpt_to_numpy, which saves every solver field (base_solver.py:63-88)."""
+        out = {f"RigidSolver.{k}": v.detach().cpu().numpy() for k, v in self.st.t.items()
+               if k not in CHECKPOINT_SKIP}
+        for k in MODEL_TABLES_F:
+            out[f"RigidSolver.model.{k}"] = self.dm.tensors[k].detach().cpu().numpy()
+        out["RigidSolver.model.meaninertia"] = np.float32(self.model.meaninertia)
+        return out
 
     def load_checkpoint_arrays(self, arrays):
-        t = self.st.t
-        for k, v in t.items():
+        def fits(a, v):
+            return tuple(a.shape) == tuple(v.shape) and np.dtype(a.dtype) == np.dtype(str(v.dtype).replace("torch.", ""))
+
+        pending = []
+        for k, v in self.st.t.items():
             if k in CHECKPOINT_SKIP:
                 continue
             a = arrays.get(f"RigidSolver.{k}")
             if a is None:
                 raise KeyError(f"checkpoint has no array for state field {k}")
-            if tuple(a.shape) != tuple(v.shape) or np.dtype(a.dtype) != np.dtype(str(v.dtype).replace("torch.", "")):
+            if not fits(a, v):
                 raise ValueError(f"checkpoint array {k}: {a.dtype}{tuple(a.shape)} does not fit {v.dtype}{tuple(v.shape)}")
+            pending.append((v, a))
+        for k in MODEL_TABLES_F:
+            a = arrays.get(f"RigidSolver.model.{k}")
+            if a is None:
+                raise KeyError(f"checkpoint has no model table {k}")
+            if not fits(a, self.dm.tensors[k]):
+                raise ValueError(f"checkpoint model table {k} does not fit this scene")
+        for v, a in pending:
             v.copy_(torch.from_numpy(np.ascontiguousarray(a)).to(v.device))
+        # model tables: the device tables in place (every descriptor keeps its pointers) and the host model arrays
+        # (shared tables; a per-env table has no host copy)
+        for k in MODEL_TABLES_F:
+            a = np.ascontiguousarray(arrays[f"RigidSolver.model.{k}"])
+            self.dm.tensors[k].copy_(torch.from_numpy(a).to(self.device))
+            host = getattr(self.model, k)
+            # a per-env table has no host copy; an empty table is uploaded as one padding element
+            if not (self.dm.batch_dofs and k in BATCHED_DOF_TABLES) and host.size:
+                host[...] = a.reshape(host.shape).astype(host.dtype)
+        mi = arrays.get("RigidSolver.model.meaninertia")
+        if mi is not None:
+            self.model.meaninertia = np.float32(mi)
+            self.dm.desc.meaninertia = float(mi)
 
     # -- model parameters (rsd:1777-2010 setters, 2334-2452 getters) ---------------------------------------------
     # One value per dof / link / geom shared by every env: the reference's default batch_dofs_info =
@@ -617,12 +651,15 @@ class RigidSolver:
         return self._model_read("geom_friction", geoms_idx, self.model.n_geoms)
 
     def get_mass_mat(self, dofs_idx=None, envs_idx=None, decompose=False):
-        """Joint-space mass matrix (B, n, n) at the current state (rsd:2436-2449), as the step's dynamics builds it:
-        entity blocks with armature and the implicit damping terms (dof damping, and kv for position / velocity
-        controlled dofs, times dt). decompose: (L, D_inv) with unit-diagonal L and M = L^T diag(1 / D_inv) L. The
-        dynamics kernel is run on the cached kinematics; its other outputs are recomputed from the same inputs by
-        the next step, so the call changes no result."""
-        self.launch_kernel(0)
+        """Joint-space mass matrix (B, n, n) the solver holds (rsd:2436-2449): the one the last substep's dynamics
+        built, at the pose before that substep's integration, with armature and the implicit damping terms (dof
+        damping, and kv for position / velocity controlled dofs, times dt). Before the first substep the dynamics
+        kernel builds it at the current pose; its other outputs are recomputed from the same inputs by the next
+        step, so the call changes no result. decompose: (L, D_inv) with unit-diagonal L and
+        M = L^T diag(1 / D_inv) L."""
+        if not self._mass_valid:
+            self.launch_kernel(0)
+            self._mass_valid = True
         Mo, B = self.model, self._B
         src = self.st.t["mass_L_blk" if decompose else "mass_blk"]
         out = torch.zeros((B, Mo.n_dofs, Mo.n_dofs), dtype=torch.float32, device=self.device)

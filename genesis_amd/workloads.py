@@ -267,3 +267,38 @@ def init_state(w: Workload, M, S, env_begin=0, n_envs=None):
         S.set_qpos(torch.as_tensor(q, dtype=torch.float32), envs_idx=envs)
     else:
         S.set_qpos(torch.as_tensor(np.tile(M.qpos0, (n, 1)), dtype=torch.float32), envs_idx=envs)
+
+
+# errno bits the reference raises on (rigid_solver_decomp.py:941-956): broad-phase or contact overflow, NaN
+FATAL_ERRNO = 0xF
+
+
+def reset_envs(w: Workload, M, S, envs):
+    """Env i of the listed envs (solver indices) gets the fresh state init_state gives env i at the start of the
+    workload: its own initial pose, zero velocities, force control, cleared caches and errno."""
+    import torch
+    envs = torch.as_tensor(envs, device=S.device, dtype=torch.long).reshape(-1)
+    if envs.numel() == 0:
+        return
+    S.st.init_state(envs)
+    idx = envs.cpu().numpy()
+    if w.init_qpos is not None:
+        q = np.asarray(w.init_qpos(int(idx.max()) + 1, M.n_qs))[idx]
+    elif w.name == "quadruped":
+        q = np.tile(np.concatenate([GO2_INIT_BASE, GO2_STANDING]), (len(idx), 1))
+    else:
+        q = np.tile(M.qpos0, (len(idx), 1))
+    S.set_qpos(torch.as_tensor(q, dtype=torch.float32), envs_idx=envs)
+
+
+def reset_failed_envs(w: Workload, M, S) -> int:
+    """The episode-termination rule of the benchmark loops: envs whose last steps set an errno bit the reference
+    raises on (a diverged env: NaN accelerations or state, or a capacity overflow) restart from their initial state,
+    as a reinforcement-learning loop resets a terminated env, instead of stopping every other env. Returns how many
+    envs were reset (one host synchronisation)."""
+    import torch
+    bad = torch.nonzero((S.st.t["errno_env"][0] & FATAL_ERRNO) != 0).flatten()
+    n = int(bad.numel())
+    if n:
+        reset_envs(w, M, S, bad)
+    return n

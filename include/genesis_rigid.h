@@ -84,6 +84,10 @@ typedef struct gsr_model {
   const int32_t *geom_type, *geom_link, *geom_vert_start, *geom_vert_end;
   const float *geom_pos, *geom_quat, *geom_data, *geom_friction, *geom_sol_params, *geom_center, *geom_init_aabb;
   const float *verts;
+  /* mesh support field: per mesh geom, 180 x 180 direction cells (theta-major) from support_cell_start[g], each the
+     support vertex's position (3 floats) and its index within the geom */
+  const int32_t *support_cell_start, *support_vid;
+  const float *support_v;
   /* equalities */
   const int32_t *eq_type, *eq_obj1, *eq_obj2;
   const float *eq_data, *eq_sol_params;
@@ -232,6 +236,76 @@ int gsr_graph_capture(const void* dev_table, const gsr_batch_info* info, int32_t
                       void** graph_exec);
 int gsr_graph_launch(void* graph_exec, hipStream_t stream);
 int gsr_graph_destroy(void* graph_exec);
+
+/* ---- Serving form: device-side request scheduler (no counterpart in the reference) ------------------------------
+ * Resident pools of env slots (one per scene type, cut into tiles of envs_per_tile envs) serve rollout requests
+ * (scene type, n_envs, horizon, seed). A tick (gsr_serve_tick, or its hipGraph from gsr_serve_graph_capture) admits
+ * arrived requests in arrival order while request slots, free tiles and the memory and tick budgets allow, resets
+ * the admitted envs, applies each active request's controls, runs one fused substep over the active tiles, advances
+ * every active request by one step and retires those at their horizon. Host and device exchange arrivals,
+ * completions and per-tick GPU timestamps through a gsr_serve_host block in coherent pinned host memory, with no
+ * host synchronisation: the host appends arrivals (entry first, then arrivals_head) and reads completions and
+ * ticks_done; the device reads arrivals and writes the rest. A request's results do not depend on its slots, the
+ * tile size or the other requests (same per-env device code; controls from a hash of request-local indices). */
+#define GSR_SERVE_MAX_REQUESTS 1024 /* requests admitted at once */
+#define GSR_SERVE_RING 4096         /* entries of the arrival, completion and tick-timestamp rings */
+
+typedef struct gsr_serve_pool {
+  gsr_model model;          /* by value, with the execution variants the pool runs */
+  gsr_state state;          /* by value: the pool's envs are envs [0, envs_per_tile * n_tiles) of this state */
+  int32_t envs_per_tile, n_tiles;
+  int32_t ctrl_kind;        /* 0 none; 1 position targets U[ctrl_lo, ctrl_hi] per controlled dof, redrawn when the
+                               request's step is a multiple of ctrl_period; 2 position targets ctrl_base +
+                               U[-ctrl_amp, ctrl_amp] every step */
+  int32_t ctrl_period, ctrl_dof_begin, ctrl_dof_end;
+  float ctrl_amp, pad;
+  const float *ctrl_lo, *ctrl_hi, *ctrl_base; /* device, one per controlled dof */
+  const float *init_qpos;   /* device, env-minor [n_qs][envs_per_tile * n_tiles]: initial qpos of request-local env i */
+  double bytes_per_env;     /* memory-budget bytes one admitted env holds */
+  double cost_per_env;      /* tick-budget units (e.g. predicted ms) one admitted env adds to a tick */
+} gsr_serve_pool;
+
+typedef struct gsr_serve_request {
+  int32_t rid, pool, n_envs, horizon;
+  uint32_t seed, pad0, pad1, pad2;
+} gsr_serve_request;
+
+typedef struct gsr_serve_done {
+  int32_t rid, tick, failed, slot; /* failed: an env of the request hit an errno bit the reference raises on */
+} gsr_serve_done;
+
+typedef struct gsr_serve_host {
+  int32_t arrivals_head; /* host: requests appended so far */
+  int32_t arrivals_tail; /* device: requests admitted so far */
+  int32_t done_count;    /* device: completion records written so far */
+  int32_t ticks_done;    /* device: ticks finished */
+  int64_t env_steps;     /* device: env-steps stepped so far */
+  gsr_serve_request arrivals[GSR_SERVE_RING];
+  gsr_serve_done done[GSR_SERVE_RING];
+  uint64_t tick_t0[GSR_SERVE_RING], tick_t1[GSR_SERVE_RING]; /* GPU wall clock at tick start / end */
+  int32_t tick_envs[GSR_SERVE_RING];                                   /* envs stepped by each tick */
+} gsr_serve_host;
+
+/* Coherent pinned host memory for one gsr_serve_host (zero-filled); free with gsr_serve_host_free. */
+int gsr_serve_host_alloc(gsr_serve_host** out);
+int gsr_serve_host_free(gsr_serve_host* host);
+/* Device bytes of the scheduler state and of the tile table for n_tiles tiles over all pools. */
+int64_t gsr_serve_state_bytes(int32_t n_tiles);
+int64_t gsr_serve_table_bytes(int32_t n_tiles);
+/* GPU wall-clock rate of the tick timestamps, in kHz. */
+int gsr_serve_clock_khz(void);
+/* Set up the scheduler over n_pools pools (<= GSR_MAX_TYPES, heaviest first): memory and tick budgets, all tiles
+ * free. `info` is read for the launch form (persistent, waves_per_simd, gjk_inline, phased) and filled like
+ * gsr_batch_upload fills it. Synchronises the stream. */
+int gsr_serve_init(const gsr_serve_pool* pools, int32_t n_pools, double mem_budget, double tick_budget,
+                   void* dev_state, int64_t state_bytes, void* dev_table, int64_t table_bytes, gsr_batch_info* info,
+                   hipStream_t stream);
+/* One tick on `stream` (no host synchronisation). */
+int gsr_serve_tick(void* dev_state, void* dev_table, const gsr_batch_info* info, gsr_serve_host* host,
+                   hipStream_t stream);
+/* Capture one tick into a hipGraph (launch with gsr_graph_launch, free with gsr_graph_destroy). */
+int gsr_serve_graph_capture(void* dev_state, void* dev_table, const gsr_batch_info* info, gsr_serve_host* host,
+                            hipStream_t stream, void** graph_exec);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,9 @@ struct Model {
   std::vector<T> eq_data, eq_sol_params;  // (NEQ,11), (NEQ,7)
   std::vector<T> mass_parent_mask;        // (D,D)
   std::vector<int32_t> pair_idx;          // (G,G)
+  // mesh support field (support_field_decomp.py:24-94): 180 x 180 direction cells per geom from its first cell
+  std::vector<int32_t> support_cell_start, support_vid;
+  std::vector<V3<T>> support_v;
 
   void load(const RawModel& r) {
     L = r.scalar_i("n_links"); J = r.scalar_i("n_joints"); D = r.scalar_i("n_dofs"); Q = r.scalar_i("n_qs");
@@ -132,6 +135,7 @@ struct Model {
     cp("eq_type", eq_type); cp("eq_obj1", eq_obj1); cp("eq_obj2", eq_obj2);
     cf("eq_data", eq_data); cf("eq_sol_params", eq_sol_params);
     cf("mass_parent_mask", mass_parent_mask); cp("pair_idx", pair_idx);
+    cp("support_cell_start", support_cell_start); cp("support_vid", support_vid); cv3("support_v", support_v);
   }
 };
 
@@ -700,6 +704,49 @@ V3<T> support_capsule(const Model<T>& m, const V3<T>& gpos, const Q4<T>& gquat, 
   V3<T> ep = gpos + (hl * side) * axis;
   return ep + d * r;
 }
+// Mesh support from the precomputed direction grid (support_field_decomp.py:145-186, _func_support_mesh): the
+// mesh-frame direction's spherical angles select a cell; of the four cells at the floor / ceil corners (theta
+// wrapped, phi kept off the poles) the candidate vertex with the largest projection wins, the first on ties.
+const int SUPPORT_RES = 180;
+template <class T>
+V3<T> support_mesh_local(const Model<T>& m, V3<T> dm, int g, int& vid) {
+  const T pi = T(M_PI);
+  const T theta = tatan2(dm.y, dm.x);
+  // a unit direction's z can round past +-1; clamped so the pole cells are looked up instead of a NaN angle
+  const T phi = tacos(std::min(std::max(dm.z, T(-1)), T(1)));
+  const T ii = (theta + pi) / pi / T(2) * T(SUPPORT_RES);
+  const T jj = phi / pi * T(SUPPORT_RES);
+  T dot_max = T(-1e20);
+  V3<T> v(0, 0, 0);
+  vid = 0;
+  for (int i4 = 0; i4 < 4; i4++) {
+    int i = (i4 % 2) ? int(std::ceil(ii)) % SUPPORT_RES : int(std::floor(ii)) % SUPPORT_RES;
+    int j;
+    if (i4 / 2 > 0) {
+      j = std::min(std::max(int(std::ceil(jj)), 0), SUPPORT_RES - 1);
+      if (j == SUPPORT_RES - 1) j = SUPPORT_RES - 2;
+    } else {
+      j = std::min(std::max(int(std::floor(jj)), 0), SUPPORT_RES - 1);
+      if (j == 0) j = 1;
+    }
+    const int idx = m.support_cell_start[g] + i * SUPPORT_RES + j;
+    const V3<T> pos = m.support_v[idx];
+    const T dp = dot(pos, dm);
+    if (dp > dot_max) {
+      v = pos;
+      dot_max = dp;
+      vid = m.support_vid[idx];
+    }
+  }
+  return v;
+}
+// world-frame mesh support (support_field_decomp.py:125-141, _func_support_world)
+template <class T>
+V3<T> support_mesh(const Model<T>& m, const V3<T>& gpos, const Q4<T>& gquat, V3<T> d, int g, int& vid) {
+  const V3<T> dm = transform_by_quat(d, inv_quat(gquat));
+  return transform_by_trans_quat(support_mesh_local(m, dm, g, vid), gpos, gquat);
+}
+
 // mpr support_driver (mpr_decomp.py:171-197) over the (possibly perturbed) pose held in the env
 template <class T>
 V3<T> support_driver(const Model<T>& m, const Env<T>& e, V3<T> d, int g) {
@@ -707,7 +754,10 @@ V3<T> support_driver(const Model<T>& m, const Env<T>& e, V3<T> d, int g) {
   if (t == GEOM_SPHERE) return support_sphere(m, e.g_pos[g], d, g);
   if (t == GEOM_CAPSULE) return support_capsule(m, e.g_pos[g], e.g_quat[g], d, g);
   if (t == GEOM_BOX) return support_box(m, e.g_pos[g], e.g_quat[g], d, g);
-  // mesh support fields are not part of the compiled workloads
+  if (t == GEOM_MESH) {
+    int vid;
+    return support_mesh(m, e.g_pos[g], e.g_quat[g], d, g, vid);
+  }
   return e.g_pos[g];
 }
 
@@ -999,6 +1049,9 @@ void gjk_support_one(const Model<T>& m, const Env<T>& e, V3<T> d, int g, V3<T>& 
              (db.z < 0 ? T(-1) : T(1)) * data[2] * T(0.5));
     vid = (vl.x > 0 ? 1 : 0) + (vl.y > 0 ? 2 : 0) + (vl.z > 0 ? 4 : 0) + m.geom_vert_start[g];
     v = transform_by_trans_quat(vl, e.g_pos[g], e.g_quat[g]);
+  } else if (t == GEOM_MESH) {
+    // the support field, not an exhaustive vertex search (that one is the MuJoCo-compatibility path)
+    v = support_mesh(m, e.g_pos[g], e.g_quat[g], d, g, vid);
   } else {
     v = support_driver(m, e, d, g);
   }
@@ -2315,3 +2368,18 @@ extern "C" int orc_broadphase_from_aabbs(void* hp, int b, const double* aabb_min
   return run(h->mf, h->ef[b]);
 }
 
+
+// Mesh-frame support-field lookup of geom g for direction d (test hook for the grid lookup):
+// writes the support vertex and returns its index within the geom.
+extern "C" int orc_support_mesh(void* hp, int g, const double* d, double* v_out) {
+  OrcHandle* h = static_cast<OrcHandle*>(hp);
+  auto run = [&](auto& m) {
+    using TT = typename std::decay<decltype(m.dt)>::type;
+    int vid = 0;
+    V3<TT> v = support_mesh_local(m, V3<TT>(TT(d[0]), TT(d[1]), TT(d[2])), g, vid);
+    v_out[0] = double(v.x); v_out[1] = double(v.y); v_out[2] = double(v.z);
+    return vid;
+  };
+  if (h->precision == 64) return run(h->md);
+  return run(h->mf);
+}

@@ -36,6 +36,8 @@ def lib():
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_broadphase_from_aabbs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_broadphase_from_aabbs.restype = ctypes.c_int
+        L.orc_support_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_support_mesh.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -45,6 +47,7 @@ INT_TABLES = [
     "link_joint_end", "link_is_fixed", "link_entity", "joint_type", "joint_q_start", "joint_dof_start", "joint_dof_end",
     "dof_entity", "ent_link_start", "ent_link_end", "ent_dof_start", "ent_dof_end", "geom_type", "geom_link",
     "geom_is_fixed", "geom_vert_start", "geom_vert_end", "eq_type", "eq_obj1", "eq_obj2", "pair_idx",
+    "support_cell_start", "support_vid",
 ]
 FLOAT_TABLES = [
     "link_pos", "link_quat", "link_inertial_pos", "link_inertial_quat", "link_inertial_i", "link_inertial_mass",
@@ -52,7 +55,7 @@ FLOAT_TABLES = [
     "dof_force_range", "dof_kp", "dof_kv", "dof_damping", "dof_armature", "dof_stiffness", "dof_frictionloss",
     "dof_invweight", "qpos0", "ent_gravity_compensation", "geom_pos", "geom_quat", "geom_center", "geom_data",
     "geom_friction", "geom_sol_params", "geom_init_aabb", "verts", "eq_data", "eq_sol_params", "mass_parent_mask",
-    "gravity",
+    "gravity", "support_v",
 ]
 
 # state field -> (kind, per-env shape builder)
@@ -180,6 +183,13 @@ class Oracle:
     def run(self, n_steps=1, stage=0, envs=None, n_threads=1):
         b0, b1 = (0, self.B) if envs is None else envs
         lib().orc_run(self.h, b0, b1, n_steps, stage, n_threads)
+
+    def support_mesh(self, g, d):
+        """Support-field lookup of mesh geom g for a mesh-frame direction d: (vertex, its index in the geom)."""
+        dv = np.ascontiguousarray(d, np.float64)
+        out = np.zeros(3, np.float64)
+        vid = lib().orc_support_mesh(self.h, g, dv.ctypes.data, out.ctypes.data)
+        return out, vid
 
     def broadphase_from_aabbs(self, b, aabb_min, aabb_max):
         amin = np.ascontiguousarray(aabb_min, np.float64).ravel()

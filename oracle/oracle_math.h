@@ -13,6 +13,7 @@ namespace orc {
 template <class T> inline T tsin(T x) { return T(std::sin(double(x))); }
 template <class T> inline T tcos(T x) { return T(std::cos(double(x))); }
 template <class T> inline T tatan2(T y, T x) { return T(std::atan2(double(y), double(x))); }
+template <class T> inline T tacos(T x) { return T(std::acos(double(x))); }
 template <class T> inline T tpow(T x, T y) { return T(std::pow(double(x), double(y))); }
 
 template <class T>

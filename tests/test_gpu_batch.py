@@ -52,26 +52,40 @@ def _overlap_pile(parts, seed=11):
         S.set_qpos(torch.as_tensor(q.reshape(B, -1), dtype=torch.float32))
 
 
-@pytest.mark.parametrize("tiles,launch,deep", [
-    ((64, 64, 1), dict(persistent=True, waves_per_simd=4), False),
-    ((32, 64, 3), dict(persistent=False, waves_per_simd=4), False),
-    ((64, 16, 4), dict(persistent=True, waves_per_simd=2), False),
-    ((64, 64, 1), dict(persistent=False, waves_per_simd=2, gjk_inline=False), True),
-    ((32, 64, 2), dict(persistent=True, waves_per_simd=4, gjk_inline=True), True),
-    ((64, 64, 1), dict(phased=True), False),
-    ((32, 16, 2), dict(phased=True), True),
-])
-def test_fused_equals_plain_bitwise(tiles, launch, deep):
-    """deep: overlapping boxes in the pile part, so the GJK/EPA pass runs (deferred to the resume launch, or inside
-    the fused kernel) while the plain step runs it in its own pass."""
+# (solver, collide, dyn) variants of every group: the all-lane path of the fused tile runner, each mixed branch, and
+# the all-wavefront path; each with the single fused kernel (GJK/EPA inside or deferred to the resume launch) and the
+# phased launches, with and without deep penetrations (the GJK/EPA pass)
+VARIANTS = [(0, 0, 0), (0, 1, 0), (1, 0, 1), (0, 0, 1), (1, 1, 0), (1, 1, 1)]
+LAUNCHES = [dict(persistent=True, waves_per_simd=4, gjk_inline=False),
+            dict(persistent=False, waves_per_simd=2, gjk_inline=True),
+            dict(phased=True)]
+
+
+@pytest.mark.parametrize("variants", VARIANTS)
+@pytest.mark.parametrize("li", range(len(LAUNCHES)))
+def test_fused_equals_plain_bitwise(variants, li):
+    """Every group and the plain solvers run the same (solver, collide, dyn) variants; the pile part starts from
+    overlapping poses in every other case (deep: the GJK/EPA pass runs, deferred to the resume launch, inside the
+    fused kernel or in its phase, while the plain step runs it in its own pass)."""
     from genesis_amd.batch import FusedBatch, Group
+    launch = LAUNCHES[li]
+    deep = (VARIANTS.index(variants) + li) % 2 == 1
+    sv, cv, dv = variants
+    tiles = (64, 64, 1) if sv == 0 else (4, 2, 1)
+    if cv == 0 and sv == 0:
+        tiles = (32, 64, 3)
     A, B = _parts(COUNTS), _parts(COUNTS)
+    for parts in (A, B):
+        for _, _, S, _ in parts:
+            S.set_variants(sv, cv, dv)
     if deep:
         _overlap_pile(A)
         _overlap_pile(B)
-    # plain variants: lane solver for arm/quadruped, wavefront solver for the pile
     fb = FusedBatch([Group(S, envs_per_tile=t) for (_, _, S, _), t in zip(B, tiles)][::-1], **launch)
-    for step in range(12):
+    for (_, _, SA, _), (_, _, SB, _) in zip(A, B):
+        assert tuple(SA.dm.desc.__getattribute__(k) for k in ("solver_variant", "collide_variant", "dyn_variant")) == \
+            tuple(SB.dm.desc.__getattribute__(k) for k in ("solver_variant", "collide_variant", "dyn_variant"))
+    for step in range(10):
         for parts in (A, B):
             for _, _, S, c in parts:
                 c(step)
@@ -79,13 +93,40 @@ def test_fused_equals_plain_bitwise(tiles, launch, deep):
             S.substep()
         fb.step()
         torch.cuda.synchronize()
-        _assert_same(_state(A), _state(B), f"step {step} tiles {tiles} {launch}")
+        _assert_same(_state(A), _state(B), f"step {step} variants {variants} {launch}")
     if deep:
         for parts in (A, B):
             e = parts[2][2].st.t["errno_env"][0]
             assert bool(((e & 0x100) != 0).all()), "the GJK/EPA pass did not run in every pile env"
     for _, _, S, _ in B:
         S.check_errno()
+
+
+def test_batch_upload_rejects_conflicting_groups():
+    """Groups over one state must agree on the variants (the row layout follows the solver variant) and must not
+    share envs."""
+    import ctypes
+    from genesis_amd import _lib
+    from genesis_amd.batch import FusedBatch, Group
+    P = _parts([("quadruped", 40)])
+    S = P[0][2]
+    with pytest.raises(ValueError):
+        FusedBatch([Group(S, 0, 20, 4, 1, 1), Group(S, 20, 20, 4, 0, 1)])
+    with pytest.raises(RuntimeError):
+        FusedBatch([Group(S, 0, 25, 4), Group(S, 20, 20, 4)])
+    # the C-ABI itself refuses a variant mix on one state (the Python wrapper is not the only caller)
+    lib = _lib.load()
+    arr = (_lib.GsrGroup * 2)()
+    for i, (b, n, v) in enumerate(((0, 20, 1), (20, 20, 0))):
+        m = _lib.GsrModel.from_buffer_copy(S.dm.desc)
+        m.solver_variant = v
+        arr[i].model, arr[i].state = m, S.st.desc
+        arr[i].env_begin, arr[i].n_envs, arr[i].envs_per_tile = b, n, 4
+    nbytes = lib.gsr_batch_table_bytes(2)
+    table = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+    info = _lib.GsrBatchInfo()
+    torch.cuda.synchronize()
+    assert lib.gsr_batch_upload(arr, 2, ctypes.c_void_p(table.data_ptr()), nbytes, ctypes.byref(info)) == -1
 
 
 def test_fused_group_subrange_leaves_other_envs_alone():

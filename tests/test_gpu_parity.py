@@ -330,3 +330,60 @@ def test_fp64_drift_over_1000_steps(case, B, steps):
     print(f"{case}: max |qpos - FP64| every 100 steps, FP32 oracle: {['%.2e' % d for d in drift_o32]}")
     assert np.isfinite(drift_gpu).all()
     assert max(drift_gpu) <= 10 * max(drift_o32) + 1e-3
+
+
+@pytest.mark.parametrize("collide_variant", [0, 1])
+def test_mesh_arm_contacts_match_oracle(collide_variant):
+    """The Panda arm's collision meshes (convex hulls, support-field lookups in MPR and GJK/EPA) against the plane:
+    random joint angles within the limits put links into the floor; from the same state the HIP collision stage
+    gives the oracle's contacts in the same order, positions, normals and depths within 1e-4, over resynced steps
+    of the full step (variant 1 solver, both collision variants)."""
+    _need_gpu()
+    from genesis_amd.model import GEOM_MESH
+    from genesis_amd.rigid_solver import RigidSolver
+    ents, opt = CASES["arm"]()
+    M = compile_model(ents, opt)
+    assert (M.geom_type == GEOM_MESH).sum() >= 10
+    B = 64
+    rng = np.random.default_rng(17)
+    S = RigidSolver(M, B, "cuda", solver_variant=1, collide_variant=collide_variant)
+    S.set_qpos(torch.as_tensor(_random_qpos(M, B, rng), dtype=torch.float32))
+    O = _oracle(M, B)
+    n_mesh = 0
+    worst = 0.0
+    for step in range(8):
+        torch.cuda.synchronize()
+        keep = {k: v.clone() for k, v in S.st.t.items()}
+        gpu_to_oracle(S, O)
+        S.run_stage(3)
+        O.run(stage=3)
+        torch.cuda.synchronize()
+        t = S.st.t
+        ncg = t["n_contacts"][0].cpu().numpy()
+        geoms = t["contact_geoms"].T.cpu().numpy().reshape(B, -1, 2)
+        pos = t["contact_pos"].T.cpu().numpy().reshape(B, -1, 3)
+        nrm = t["contact_normal"].T.cpu().numpy().reshape(B, -1, 3)
+        pen = t["contact_pen"].T.cpu().numpy()
+        oc = O.get("contacts")
+        for b in range(B):
+            n = int(ncg[b])
+            assert n == len(oc[b]), f"step {step} env {b}: {n} contacts vs the oracle's {len(oc[b])}"
+            np.testing.assert_array_equal(geoms[b, :n], oc[b][:, 0:2].astype(np.int32))
+            np.testing.assert_allclose(pos[b, :n], oc[b][:, 4:7], atol=1e-4)
+            np.testing.assert_allclose(nrm[b, :n], oc[b][:, 7:10], atol=1e-4)
+            np.testing.assert_allclose(pen[b, :n], oc[b][:, 10], atol=1e-4)
+            n_mesh += int(sum(M.geom_type[g] == GEOM_MESH for g in geoms[b, :n, 1]))
+        # then the whole step from the same state: qpos / qvel within the tolerance
+        for k, v in keep.items():
+            t[k].copy_(v)
+        gpu_to_oracle(S, O)
+        S.substep()
+        O.run(1)
+        torch.cuda.synchronize()
+        for name in ("qpos", "vel"):
+            g_, o_ = S.st.t[name].T.cpu().numpy(), O.get(name)
+            err = np.abs(g_ - o_) - (TOL + TOL * np.abs(o_))
+            worst = max(worst, float(err.max()))
+            assert (err <= 0).all(), f"step {step} {name}: max excess {err.max():.3g}"
+    print(f"mesh arm (collide variant {collide_variant}): {n_mesh} mesh contacts, worst margin {worst:.3g}")
+    assert n_mesh > 0

@@ -119,6 +119,19 @@ def test_mass_mat_of_identical_robots_and_its_factor():
     gpu_to_oracle(S, O)
     O.run(stage=2)
     np.testing.assert_allclose(S.get_mass_mat().cpu().numpy(), O.get("mass_mat")[0], atol=1e-5, rtol=1e-5)
+    # after a step it is the matrix that step's dynamics built, at the pose before the step's integration
+    a1.control_dofs_position(torch.full((a1.n_dofs,), 0.4))
+    for _ in range(3):
+        scene.step()
+    torch.cuda.synchronize()
+    gpu_to_oracle(S, O)
+    O.run(stage=2)
+    scene.step()
+    m_step = S.get_mass_mat().cpu().numpy()
+    np.testing.assert_allclose(m_step, O.get("mass_mat")[0], atol=1e-5, rtol=1e-5)
+    gpu_to_oracle(S, O)  # M at the post-step pose is another matrix
+    O.run(stage=2)
+    assert np.abs(O.get("mass_mat")[0] - m_step).max() > 1e-6
     # reading it between steps changes no result
     other = gs.Scene(sim_options=gs.options.SimOptions(dt=0.01, substeps=1))
     other.add_entity(gs.morphs.Plane())

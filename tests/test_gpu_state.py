@@ -60,6 +60,92 @@ def test_checkpoint_resume_is_bitwise(tmp_path, variant):
         assert torch.equal(ca[k], cb[k]), k
 
 
+def test_checkpoint_carries_changed_parameters(tmp_path):
+    """Parameters changed through the setters (dof gains, armature with the invweights and meaninertia it
+    recomputes, link mass, geom friction, per-env mass shifts) travel in the checkpoint: a freshly built scene that
+    loads it resumes bitwise like the uninterrupted run, and reports the changed parameters."""
+    a, arm_a = _box_scene(6, 1, arm=True)
+    S = a.rigid_solver
+    dofs = np.arange(arm_a.dof_start, arm_a.dof_start + 7)
+    S.set_dofs_kp(np.linspace(200.0, 800.0, 7), dofs_idx=dofs)
+    S.set_dofs_armature(np.full(7, 0.2), dofs_idx=dofs)
+    S.set_links_inertial_mass([3.0], links_idx=[1])
+    S.set_geoms_friction([0.4], geoms_idx=[2])
+    S.set_links_mass_shift([0.3, -0.2], links_idx=[2, 3], envs_idx=[1])
+    arm_a.control_dofs_position(torch.full((6, 7), 0.3), dofs_idx_local=np.arange(7))
+    for _ in range(20):
+        a.step()
+    path = tmp_path / "ckpt_params.npz"
+    a.save_checkpoint(path)
+    for _ in range(20):
+        a.step()
+    qa, va = _qv(a)
+    b, arm_b = _box_scene(6, 1, arm=True)
+    Sb = b.rigid_solver
+    assert not torch.equal(Sb.get_dofs_kp(dofs), S.get_dofs_kp(dofs))
+    b.load_checkpoint(path)
+    for g in ("get_dofs_kp", "get_dofs_armature", "get_dofs_invweight"):
+        assert torch.equal(getattr(Sb, g)(dofs), getattr(S, g)(dofs)), g
+    assert float(Sb.dm.desc.meaninertia) == float(S.dm.desc.meaninertia)
+    assert float(Sb.model.link_inertial_mass[1]) == 3.0 and float(Sb.model.geom_friction[2]) == np.float32(0.4)
+    for _ in range(20):
+        b.step()
+    qb, vb = _qv(b)
+    assert torch.equal(qa, qb) and torch.equal(va, vb)
+
+
+def test_set_state_on_some_envs_steps_like_an_oracle_built_from_the_snapshot():
+    """kernel_set_state (rigid_solver_decomp.py:7212-7258) and set_state (:1470-1512) on envs 1 and 3: their qpos,
+    velocities and shifts come from the snapshot, their controls fall back to zero force (ctrl_pos / ctrl_vel are
+    left as they were), the contact and warm-start caches are cleared; envs 0 and 2 keep their state. An oracle
+    whose restored envs are built from the snapshot by those rules (not copied from the GPU after set_state) and
+    whose other envs are copied from the GPU before set_state steps like the HIP solver."""
+    from oracle.oracle import Oracle
+    from tests.sync import gpu_fields
+    scene, arm = _box_scene(4, 1, arm=True)
+    S = scene.rigid_solver
+    arm.control_dofs_position(torch.full((4, 7), -0.2), dofs_idx_local=np.arange(7))
+    for _ in range(15):
+        scene.step()
+    st = S.get_state()
+    arm.control_dofs_position(torch.full((4, 7), 0.4), dofs_idx_local=np.arange(7))
+    S.set_links_mass_shift([0.2], links_idx=[1], envs_idx=[3])
+    for _ in range(10):
+        scene.step()
+    torch.cuda.synchronize()
+    pre = gpu_fields(S)
+    restored = [1, 3]
+    S.set_state(st, envs_idx=restored)
+    O = Oracle(S.model, 4, 32)
+    for k, v in pre.items():
+        O.set(k, v)
+    O.set("errno", np.zeros(4, np.int32))
+    D = S.model.n_dofs
+    for b in restored:
+        O.set("qpos", st.qpos[b:b + 1].cpu().numpy().astype(np.float64), [b])
+        O.set("vel", st.dofs_vel[b:b + 1].cpu().numpy().astype(np.float64), [b])
+        O.set("mass_shift", st.mass_shift[b:b + 1].cpu().numpy().astype(np.float64), [b])
+        O.set("i_pos_shift", st.i_pos_shift[b:b + 1].cpu().numpy().reshape(1, -1).astype(np.float64), [b])
+        O.set("friction_ratio", st.friction_ratio[b:b + 1].cpu().numpy().astype(np.float64), [b])
+        O.set("ctrl_mode", np.zeros((1, D), np.int32), [b])
+        O.set("ctrl_force", np.zeros((1, D)), [b])
+        O.set("cache_normal", np.zeros((1, 3 * max(S.model.n_pairs, 1))), [b])
+        O.set("qacc_ws", np.zeros((1, D)), [b])
+        O.set("is_warmstart", np.zeros(1, np.int32), [b])
+        O.set("first_time", np.ones(1, np.int32), [b])
+    assert float(S.get_links_mass_shift([1], envs_idx=[3])[0, 0]) == 0.0  # the snapshot's shift came back
+    for step in range(5):
+        scene.step()
+        O.run(1)
+        torch.cuda.synchronize()
+        for name in ("qpos", "vel"):
+            g_, o_ = S.st.t[name].T.cpu().numpy(), O.get(name)
+            np.testing.assert_allclose(g_, o_, rtol=1e-4, atol=1e-4, err_msg=f"step {step} {name}")
+        if step == 0:
+            q = S.st.t["qpos"].T.cpu().numpy()
+            assert np.abs(q[restored] - st.qpos[restored].cpu().numpy()).max() < 0.05  # restored, then one step
+
+
 def test_checkpoint_rejects_another_scene(tmp_path):
     a, _ = _box_scene(4, None)
     path = tmp_path / "ckpt.npz"

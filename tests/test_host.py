@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     src = open(os.path.join(ROOT, "include", "genesis_rigid.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\bint\s+(gsr_\w+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(?:int|int64_t)\s+(gsr_\w+)\s*\(", src)))
 
 
 @pytest.fixture(scope="module")
@@ -147,3 +147,4 @@ def test_meaninertia_divides_inside_entity_loop():
     from genesis_amd.model import _host_fk_crba
     *_, Mm = _host_fk_crba(M, M.qpos0.astype(np.float64))
     np.testing.assert_allclose(M.meaninertia, np.trace(Mm) / 6.0, rtol=1e-5)
+

@@ -279,3 +279,62 @@ def test_multi_contact_dedup_keeps_one_point_contact(Oracle):
     assert len(c) == 1
     np.testing.assert_allclose(c[0, 10], 5e-4, rtol=1e-3)
     np.testing.assert_allclose(c[0, 4:6], pos[:2], atol=1e-6)
+
+
+def _mesh_model(verts, pos=(0.0, 0.0, 1.0), gravity=(0.0, 0.0, -9.81)):
+    from genesis_amd.model import GEOM_MESH, JOINT_FREE
+    from genesis_amd.meshes import convex_hull_vertices
+    hull = convex_hull_vertices(verts)
+    pos = np.asarray(pos, float)
+    body = EntitySpec([LinkSpec("m", -1, pos, np.array([1.0, 0, 0, 0]),
+                                [JointSpec("f", JOINT_FREE, init_qpos=np.concatenate([pos, [1, 0, 0, 0]]))],
+                                [GeomSpec(GEOM_MESH, verts=hull)])], rho=500.0)
+    return compile_model([W.plane_entity(), body], CompileOptions(dt=0.01, gravity=gravity)), hull
+
+
+def test_mesh_support_field_known_answers(Oracle):
+    """support_field_decomp.py:24-94, 145-186 on a random convex hull. Along a grid direction (theta_i, phi_j) the
+    lookup's four candidate cells include that direction's own cell, whose precomputed vertex is the exact
+    maximiser, so the lookup attains the brute-force maximum projection; along any direction it returns a hull
+    vertex, its index matches, and it falls short of the maximum by at most the grid's angular resolution."""
+    from genesis_amd.meshes import SUPPORT_RES
+    rng = np.random.default_rng(3)
+    M, hull = _mesh_model(rng.normal(size=(300, 3)) * [0.1, 0.2, 0.05])
+    g = 1
+    v32 = M.verts[M.geom_vert_start[g]:M.geom_vert_end[g]].astype(np.float64)
+    o = Oracle(M, 1, 64)
+    R = np.linalg.norm(v32, axis=1).max()
+    for i in range(0, SUPPORT_RES, 7):
+        for j in range(3, SUPPORT_RES - 3, 11):
+            th, ph = 2 * np.pi * i / SUPPORT_RES - np.pi, np.pi * j / SUPPORT_RES
+            d = np.array([np.sin(ph) * np.cos(th), np.sin(ph) * np.sin(th), np.cos(ph)])
+            v, vid = o.support_mesh(g, d)
+            np.testing.assert_array_equal(v, v32[vid])
+            assert v @ d >= (v32 @ d).max() - 1e-12, (i, j)
+    step = 2 * np.pi / SUPPORT_RES
+    for d in rng.normal(size=(400, 3)):
+        d /= np.linalg.norm(d)
+        v, vid = o.support_mesh(g, d)
+        np.testing.assert_array_equal(v, v32[vid])
+        assert v @ d >= (v32 @ d).max() - 2 * R * step
+
+
+def test_mesh_cube_on_plane_known_answer(Oracle):
+    """A cube given as a mesh (its 8 corners, through the support field instead of the box closed form) sunk 2 mm
+    into the plane: the deepest-vertex contact and the perturbed detections give the 4 bottom corners, each at
+    depth 2 mm, normal -z, contact points midway (z = -1 mm); a missing de-duplication would keep more."""
+    import itertools
+    s, pen = 0.1, 2e-3
+    corners = np.array(list(itertools.product([-s, s], repeat=3)))
+    M, _ = _mesh_model(corners, pos=(0.0, 0.0, s - pen), gravity=(0.0, 0.0, 0.0))
+    o = Oracle(M, 1, 64)
+    o.run(1, stage=3)
+    c = o.get("contacts")[0]
+    assert len(c) == 4
+    np.testing.assert_allclose(c[0, 10], pen, atol=1e-8)  # float32 vertices
+    # perturbed detections measure along the first-order corrected normal: pen (1 + O(mc_perturbation^2))
+    np.testing.assert_allclose(c[1:, 10], pen, rtol=2e-4)
+    np.testing.assert_allclose(c[:, 6], -pen / 2, atol=1e-6)
+    np.testing.assert_allclose(c[:, 7:10], np.tile([0, 0, -1.0], (4, 1)), atol=2e-4)
+    xy = sorted(map(tuple, np.round(c[:, 4:6] / s).astype(int).tolist()))
+    assert xy == [(-1, -1), (-1, 1), (1, -1), (1, 1)]

@@ -1,20 +1,7 @@
-"""Host logic of the serving loop (no GPU): slot allocation and the synthetic arrival stream."""
+"""Host logic of the serving loop (no GPU): the synthetic arrival stream and the request checks."""
 import numpy as np
 
-from genesis_amd.serving import SlotAllocator, poisson_requests
-
-
-def test_slot_allocator_first_fit_and_coalescing():
-    a = SlotAllocator(100)
-    assert a.alloc(30) == 0 and a.alloc(30) == 30 and a.alloc(30) == 60
-    assert a.alloc(20) is None and a.n_free == 10
-    a.release(30, 30)
-    assert a.alloc(40) is None  # 30 + 10 free but not contiguous
-    a.release(0, 30)
-    assert a.free == [(0, 60), (90, 10)]
-    a.release(60, 30)
-    assert a.free == [(0, 100)]
-    assert a.alloc(100) == 0 and a.n_free == 0
+from genesis_amd.serving import poisson_requests
 
 
 def test_poisson_requests_match_the_stated_distribution():
