@@ -293,19 +293,26 @@ def test_wave_collide_equals_lane_collide(case):
             S.launch_kernel(3)
 
 
-@pytest.mark.parametrize("case,B,steps", [("arm", 8, 1000), ("quadruped", 8, 1000)])
+@pytest.mark.parametrize("case,B,steps", [("arm", 8, 1000), ("quadruped", 8, 1000), ("pile20", 8, 500)])
 def test_fp64_drift_over_1000_steps(case, B, steps):
     """Free runs (no resync) of the HIP step, the FP32 oracle and the FP64 oracle from one state under the same controls:
     the HIP step's drift from FP64 must be of the same order as the FP32 oracle's own drift (both are FP32 roundings of
-    the same algorithm). The drifts are printed: they are the stated FP64 drift of both implementations."""
+    the same algorithm). The drifts are printed: they are the stated FP64 drift of both implementations. The 20-box
+    pile runs its 500-step horizon from its seeded poses (falling, landing, settling)."""
     _need_gpu()
     from tests.sync import gpu_to_oracle as g2o
-    ents, opt = CASES[case]()
+    if case == "pile20":
+        w = W.pile_workload(n_boxes=20, n_envs=B)
+        ents, opt = w.entities, w.options
+    else:
+        ents, opt = CASES[case]()
     M = compile_model(ents, opt)
     S = _solver(M, B)
     if case == "quadruped":
         S.set_qpos(torch.as_tensor(np.tile(np.concatenate([W.GO2_INIT_BASE, W.GO2_STANDING]), (B, 1)),
                                    dtype=torch.float32))
+    if case == "pile20":
+        S.set_qpos(torch.as_tensor(w.init_qpos(B, M.n_qs), dtype=torch.float32))
     O32, O64 = _oracle(M, B, 32), _oracle(M, B, 64)
     torch.cuda.synchronize()
     g2o(S, O32)

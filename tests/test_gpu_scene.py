@@ -119,7 +119,30 @@ def test_mass_mat_of_identical_robots_and_its_factor():
     gpu_to_oracle(S, O)
     O.run(stage=2)
     np.testing.assert_allclose(S.get_mass_mat().cpu().numpy(), O.get("mass_mat")[0], atol=1e-5, rtol=1e-5)
-    # after a step it is the matrix that step's dynamics built, at the pose before the step's integration
+    # reading it between steps changes no result
+    other = gs.Scene(sim_options=gs.options.SimOptions(dt=0.01, substeps=1))
+    other.add_entity(gs.morphs.Plane())
+    other.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 0, 0)))
+    other.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 2, 0)))
+    other.build()
+    for _ in range(10):
+        S.get_mass_mat()
+        scene.step()
+        other.step()
+    assert torch.equal(S.st.t["qpos"], other.rigid_solver.st.t["qpos"])
+
+
+def test_mass_mat_after_a_step_is_the_steps_own():
+    """rigid_solver_decomp.py:2436-2449 returns the mass_mat field the last substep's dynamics filled: after a step,
+    the matrix at the pose before that step's integration, not at the pose the step produced."""
+    from oracle.oracle import Oracle
+    from tests.sync import gpu_to_oracle
+    scene = gs.Scene(sim_options=gs.options.SimOptions(dt=0.01, substeps=1))
+    scene.add_entity(gs.morphs.Plane())
+    a1 = scene.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 0, 0)))
+    scene.build()
+    S = scene.rigid_solver
+    O = Oracle(S.model, 1, 64)
     a1.control_dofs_position(torch.full((a1.n_dofs,), 0.4))
     for _ in range(3):
         scene.step()
@@ -132,17 +155,6 @@ def test_mass_mat_of_identical_robots_and_its_factor():
     gpu_to_oracle(S, O)  # M at the post-step pose is another matrix
     O.run(stage=2)
     assert np.abs(O.get("mass_mat")[0] - m_step).max() > 1e-6
-    # reading it between steps changes no result
-    other = gs.Scene(sim_options=gs.options.SimOptions(dt=0.01, substeps=1))
-    other.add_entity(gs.morphs.Plane())
-    other.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 0, 0)))
-    other.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml", pos=(0, 2, 0)))
-    other.build()
-    for _ in range(10):
-        S.get_mass_mat()
-        scene.step()
-        other.step()
-    assert torch.equal(S.st.t["qpos"], other.rigid_solver.st.t["qpos"])
 
 
 def _xyz_to_quat(xyz):
