@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libgenesis_rigid.so")
 
 EXPORTS = ["gsr_abi_version", "gsr_model_size", "gsr_state_size", "gsr_group_size", "gsr_batch_info_size",
            "gsr_gjk_workspace_words",
-           "gsr_step", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno",
+           "gsr_step", "gsr_noalloc_scratch_bytes", "gsr_step_noalloc", "gsr_forward_kinematics", "gsr_reset_envs", "gsr_get_errno",
            "gsr_get_contacts", "gsr_links_contact_force", "gsr_get_equalities", "gsr_run_stage", "gsr_launch_kernel", "gsr_batch_table_bytes",
            "gsr_batch_upload", "gsr_step_fused", "gsr_fused_kernel_count", "gsr_launch_fused_kernel",
            "gsr_graph_capture", "gsr_graph_launch", "gsr_graph_destroy",
@@ -124,6 +124,9 @@ def load(path: str = LIB_PATH):
     L.gsr_state_size.restype = ctypes.c_int
     P = ctypes.POINTER
     L.gsr_step.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
+    L.gsr_noalloc_scratch_bytes.restype = ctypes.c_int64
+    L.gsr_noalloc_scratch_bytes.argtypes = [P(GsrModel), ctypes.c_int32]
+    L.gsr_step_noalloc.argtypes = [P(GsrModel), P(GsrState), ctypes.c_int32, ctypes.c_void_p]
     L.gsr_forward_kinematics.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
     L.gsr_reset_envs.argtypes = [P(GsrModel), P(GsrState), ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p]
     L.gsr_get_errno.argtypes = [P(GsrState), ctypes.c_void_p, ctypes.c_void_p]

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "rigid_tiles.h"
@@ -267,6 +268,59 @@ void launch_phase(int P, dim3 grid, unsigned lds, hipStream_t st, const char* ta
 
 }  // namespace gsr
 
+// ---- the allocation-free step (gsr_step_noalloc) -------------------------------------------------------------
+// Per-substep scratch of gsr_step_noalloc: constraint rows, Newton solver vectors and Hessian, per-entity row lists,
+// GJK/EPA workspace and deferral list, integration scratch. One copy per GPU, held until the process ends; the host
+// name `scratch` is only a stand-in for it, so host code uses the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[GSR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// device address of the current device's `scratch`, looked up once per device
+int scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return GSR_E_LAUNCH;
+  if (dev < 0 || dev >= kMaxDevices) return GSR_E_SHAPE;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GSR_E_LAUNCH;
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return GSR_OK;
+}
+
+// Carves the scratch arrays of n_envs envs out of `base` (each 256-byte aligned) into `s`; returns the bytes used,
+// or only counts them when base is null. Sizes per env follow the state layout of each array.
+size_t carve_scratch(const gsr_model& m, int n_envs, char* base, gsr_state* s) {
+  const size_t B = (size_t)n_envs;
+  const size_t R = (size_t)std::max(m.max_rows, 1), W = (size_t)m.row_width, D = (size_t)std::max(m.n_dofs, 1);
+  const size_t Q = (size_t)std::max(m.n_qs, 1), E = (size_t)m.n_entities, MB = (size_t)std::max(m.max_broad, 1);
+  size_t off = 0;
+  auto take = [&](auto*& field, size_t per_env) {
+    using T = std::remove_reference_t<decltype(*field)>;
+    const size_t bytes = (std::max(per_env, (size_t)1) * B * sizeof(T) + 255) & ~(size_t)255;
+    if (base) field = reinterpret_cast<T*>(base + off);
+    off += bytes;
+  };
+  take(s->jac_val, R * W); take(s->jac_idx, R * W); take(s->jac_nnz, R);
+  take(s->efc_diag, R); take(s->efc_aref, R); take(s->efc_D, R); take(s->efc_fl, R); take(s->efc_Jaref, R);
+  take(s->efc_jv, R); take(s->efc_quad, 3 * R); take(s->efc_active, R);
+  take(s->qacc, D); take(s->Ma, D); take(s->grad, D); take(s->Mgrad, D); take(s->search, D); take(s->mv, D);
+  take(s->qfrc_constraint, D); take(s->nt_vec, D); take(s->nt_H, D * (D + 1) / 2); take(s->solver_iters, 1);
+  take(s->ent_row_ptr, E + 1); take(s->ent_row_list, 2 * R); take(s->efc_ent, R);
+  take(s->gjk_ws, (size_t)kGjkWsWords); take(s->gjk_defer_n, 1); take(s->gjk_defer, 2 * MB);
+  take(s->qpos_next, Q); take(s->vel_next, D);
+  return off;
+}
+
+}  // namespace
+
 extern "C" {
 
 int gsr_abi_version(void) { return GSR_ABI_VERSION; }
@@ -282,6 +336,31 @@ int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps,
   if (n_substeps < 1) return GSR_E_ARG;
   for (int i = 0; i < n_substeps; i++) {
     rc = run_substep(model, state, stream);
+    if (rc) return rc;
+  }
+  return GSR_OK;
+}
+
+int64_t gsr_noalloc_scratch_bytes(const gsr_model* model, int32_t n_envs) {
+  if (!model || n_envs <= 0) return GSR_E_ARG;
+  gsr_state tmp;
+  std::memset(&tmp, 0, sizeof(tmp));
+  return (int64_t)carve_scratch(*model, n_envs, nullptr, &tmp);
+}
+
+int gsr_step_noalloc(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream) {
+  if (!model || !state) return GSR_E_ARG;
+  if (state->n_envs <= 0 || n_substeps < 1) return GSR_E_ARG;
+  gsr_state s = *state;
+  if (carve_scratch(*model, s.n_envs, nullptr, &s) > (size_t)GSR_NOALLOC_SCRATCH_BYTES) return GSR_E_SHAPE;
+  void* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  carve_scratch(*model, s.n_envs, static_cast<char*>(base), &s);
+  rc = check(model, &s);
+  if (rc) return rc;
+  for (int i = 0; i < n_substeps; i++) {
+    rc = run_substep(model, &s, stream);
     if (rc) return rc;
   }
   return GSR_OK;

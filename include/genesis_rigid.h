@@ -147,6 +147,21 @@ int gsr_batch_info_size(void);
 int gsr_gjk_workspace_words(void);
 
 int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream);
+/* ---- Allocation-free step -------------------------------------------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): gsr_step with the per-substep scratch -- constraint rows
+ * (jac_val, jac_idx, jac_nnz, efc_diag, efc_aref, efc_D, efc_fl, efc_Jaref, efc_jv, efc_quad, efc_active), the
+ * Newton solver's vectors and Hessian (qacc, Ma, grad, Mgrad, search, mv, qfrc_constraint, nt_vec, nt_H,
+ * solver_iters), the per-entity row lists (ent_row_ptr, ent_row_list, efc_ent), the GJK/EPA workspace and deferral
+ * list (gjk_ws, gjk_defer_n, gjk_defer) and the integration scratch (qpos_next, vel_next) -- taken from a static
+ * device array of GSR_NOALLOC_SCRATCH_BYTES per GPU instead of the state (those state pointers are ignored and may
+ * be NULL). The step allocates and frees nothing; results are those of gsr_step. Every call on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (two streams, two host threads) race. Returns GSR_E_SHAPE when
+ * the scratch of n_envs envs exceeds GSR_NOALLOC_SCRATCH_BYTES (gsr_noalloc_scratch_bytes gives the need). */
+#define GSR_NOALLOC_SCRATCH_BYTES (256u << 20)
+int64_t gsr_noalloc_scratch_bytes(const gsr_model* model, int32_t n_envs);
+int gsr_step_noalloc(const gsr_model* model, const gsr_state* state, int32_t n_substeps, hipStream_t stream);
+
 int gsr_forward_kinematics(const gsr_model* model, const gsr_state* state, const int32_t* envs_idx, int32_t n_idx,
                            hipStream_t stream);
 int gsr_reset_envs(const gsr_model* model, const gsr_state* state, const int32_t* envs_idx, int32_t n_idx,

@@ -148,3 +148,28 @@ def test_meaninertia_divides_inside_entity_loop():
     *_, Mm = _host_fk_crba(M, M.qpos0.astype(np.float64))
     np.testing.assert_allclose(M.meaninertia, np.trace(Mm) / 6.0, rtol=1e-5)
 
+
+NOALLOC_SCRATCH = ["jac_val", "jac_idx", "jac_nnz", "efc_diag", "efc_aref", "efc_D", "efc_fl", "efc_Jaref", "efc_jv",
+                   "efc_quad", "efc_active", "qacc", "Ma", "grad", "Mgrad", "search", "mv", "qfrc_constraint",
+                   "nt_vec", "nt_H", "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_ws",
+                   "gjk_defer_n", "gjk_defer", "qpos_next", "vel_next"]
+
+
+def test_noalloc_scratch_size_matches_state_layout(lib):
+    """gsr_noalloc_scratch_bytes = the per-env sizes of the scratch arrays of the state layout, times n_envs, each
+    array rounded up to 256 bytes; the entry point rejects null arguments before any HIP call."""
+    from genesis_amd import workloads as W
+    from genesis_amd.model import compile_model
+    from genesis_amd.rigid_solver import finalize_model, state_sizes
+    assert lib.gsr_step_noalloc(None, None, 1, None) == -1
+    for w in (W.arm_workload(), W.workload_by_name("pile20", 4)):
+        M = compile_model(w.entities, w.options)
+        finalize_model(M)
+        sz = state_sizes(M)
+        d = _lib.GsrModel()
+        d.n_dofs, d.n_qs, d.n_entities, d.max_rows, d.row_width, d.max_broad = (
+            M.n_dofs, M.n_qs, M.n_entities, M.max_rows, M.row_width, M.max_broad)
+        for n in (1, 37, 1000):
+            want = sum(-(-sz[k] * n * 4 // 256) * 256 for k in NOALLOC_SCRATCH)
+            assert lib.gsr_noalloc_scratch_bytes(ctypes.byref(d), n) == want
+        assert lib.gsr_noalloc_scratch_bytes(ctypes.byref(d), 0) == -1
