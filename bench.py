@@ -307,7 +307,7 @@ def bench_mixed(args):
     if args.plan:
         spec = json.loads(args.plan)
         plan = [FB.Group(next(p.S for p in parts if p.name == g["scene"]), 0, None, g["envs_per_tile"],
-                         g["solver"], g["collide"]) for g in spec["groups"]]
+                         g["solver"], g["collide"], g.get("solve_tile")) for g in spec["groups"]]
         launch, report = spec["launch"], []
     else:
         plan, launch, report = FB.autotune([FB.Group(p.S) for p in parts])
@@ -315,7 +315,8 @@ def bench_mixed(args):
     names = {id(p.S): p.name for p in parts}
     variants = {id(g.solver): (g.solver_variant, g.collide_variant) for g in plan}
     plan_json = {"groups": [{"scene": names[id(g.solver)], "solver": g.solver_variant, "collide": g.collide_variant,
-                             "envs_per_tile": g.envs_per_tile} for g in plan], "launch": launch}
+                             "envs_per_tile": g.envs_per_tile, "solve_tile": g.wave_envs_per_tile} for g in plan],
+                 "launch": launch}
     log("bench: plan " + json.dumps(plan_json))
 
     res = {}

@@ -34,13 +34,15 @@ class Group:
     envs_per_tile: int = 64
     solver_variant: Optional[int] = None
     collide_variant: Optional[int] = None
+    # phased launch only: envs per tile of the GJK/EPA pass and the solve (None: envs_per_tile)
+    wave_envs_per_tile: Optional[int] = None
 
     def resolved(self) -> "Group":
         n = self.solver._B - self.env_begin if self.n_envs is None else self.n_envs
         d = self.solver.dm.desc
         v = int(d.solver_variant) if self.solver_variant is None else int(self.solver_variant)
         c = int(d.collide_variant) if self.collide_variant is None else int(self.collide_variant)
-        return Group(self.solver, self.env_begin, n, self.envs_per_tile, v, c)
+        return Group(self.solver, self.env_begin, n, self.envs_per_tile, v, c, self.wave_envs_per_tile)
 
 
 class FusedBatch:
@@ -72,6 +74,7 @@ class FusedBatch:
             arr[i].model = g.solver.dm.desc
             arr[i].state = g.solver.st.desc
             arr[i].env_begin, arr[i].n_envs, arr[i].envs_per_tile = g.env_begin, g.n_envs, g.envs_per_tile
+            arr[i].wave_envs_per_tile = int(g.wave_envs_per_tile or 0)
         self._arr = arr
         self.info = _lib.GsrBatchInfo()
         self.info.persistent = int(bool(persistent))
@@ -142,12 +145,16 @@ class FusedBatch:
             pass
 
 
-def candidates(solver: RigidSolver) -> List[Tuple[int, int, int]]:
-    """(solver_variant, collide_variant, envs_per_tile) triples worth timing for one scene type."""
+def candidates(solver: RigidSolver, phased: bool = False) -> List[Tuple[int, int, int, Optional[int]]]:
+    """(solver_variant, collide_variant, envs_per_tile, wave_envs_per_tile) choices worth timing for one scene type.
+    Under the phased launch the solve (and the GJK/EPA pass) has tiles of its own, so a lane-per-env collision with
+    64 envs a tile can sit beside a wavefront solve with one to four."""
     M = solver.model
-    out = [(0, 0, 64), (0, 0, 32), (0, 1, 8)]
+    out = [(0, 0, 64, None), (0, 0, 32, None), (0, 1, 8, None)]
     if M.n_entities <= WAVE_MAX_ENTITIES:
-        out += [(1, 1, t) for t in (1, 2, 4, 8)] + [(1, 0, 4)]
+        out += [(1, 1, t, None) for t in (1, 2, 4, 8)] + [(1, 0, 4, None)]
+        if phased:
+            out += [(1, 0, 64, w) for w in (1, 2, 4)] + [(1, 1, 4, 1), (1, 1, 8, 2)]
     return out
 
 
@@ -189,25 +196,48 @@ LAUNCH_CANDIDATES = [dict(persistent=p, waves_per_simd=w, gjk_inline=g)
 
 
 def _tune_groups(groups, launch, steps, warmup, verbose, log, report):
-    """Pick (solver variant, collision variant, envs per tile) per group by timing each group alone under `launch`,
-    then order the groups by per-tile cost, descending, so the longest tiles start first."""
-    tuned = []
+    """Pick (solver variant, collision variant, envs per tile, solve tile) per group by timing each group alone under
+    `launch`, then order the groups by per-tile cost, descending, so the longest tiles start first. Under the phased
+    launch the choices are then refined on the whole batch, one group at a time (a group timed alone shows its
+    latency; in the batch its stages overlap the other groups' and what counts is the work it adds)."""
+    phased = bool(launch.get("phased"))
+    tuned, cands = [], []
     for gi, g in enumerate(groups):
         g = g.resolved()
         best = None
-        for v, c, t in candidates(g.solver):
-            ms = time_group(Group(g.solver, g.env_begin, g.n_envs, t, v, c), steps, warmup, **launch)
-            report.append({"group": gi, "solver": v, "collide": c, "envs_per_tile": t, "ms": ms,
-                           "phased": bool(launch.get("phased"))})
+        cs = candidates(g.solver, phased)
+        for v, c, t, w in cs:
+            ms = time_group(Group(g.solver, g.env_begin, g.n_envs, t, v, c, w), steps, warmup, **launch)
+            report.append({"group": gi, "solver": v, "collide": c, "envs_per_tile": t, "solve_tile": w, "ms": ms,
+                           "phased": phased})
             if verbose:
-                log(f"  group {gi} solver {v} collide {c} tile {t} {launch}: {ms:.3f} ms")
+                log(f"  group {gi} solver {v} collide {c} tile {t} solve tile {w} {launch}: {ms:.3f} ms")
             if best is None or ms < best[0]:
-                best = (ms, v, c, t)
-        ms, v, c, t = best
+                best = (ms, v, c, t, w)
+        ms, v, c, t, w = best
         n_tiles = (g.n_envs + t - 1) // t
-        tuned.append((ms / n_tiles, Group(g.solver, g.env_begin, g.n_envs, t, v, c)))
-    tuned.sort(key=lambda x: -x[0])
-    return [g for _, g in tuned]
+        tuned.append((ms / n_tiles, Group(g.solver, g.env_begin, g.n_envs, t, v, c, w)))
+        cands.append(cs)
+    order = sorted(range(len(tuned)), key=lambda i: -tuned[i][0])
+    plan = [tuned[i][1] for i in order]
+    if not phased or len(plan) < 2:
+        return plan
+    best_ms = time_batch(plan, steps, warmup, **launch)
+    for pos, gi in enumerate(order):
+        for v, c, t, w in cands[gi]:
+            g0 = plan[pos]
+            if (v, c, t, w) == (g0.solver_variant, g0.collide_variant, g0.envs_per_tile, g0.wave_envs_per_tile):
+                continue
+            trial = list(plan)
+            trial[pos] = Group(g0.solver, g0.env_begin, g0.n_envs, t, v, c, w)
+            ms = time_batch(trial, steps, warmup, **launch)
+            report.append({"group": gi, "solver": v, "collide": c, "envs_per_tile": t, "solve_tile": w, "ms": ms,
+                           "phased": True, "in_batch": True})
+            if verbose:
+                log(f"  in batch: group {gi} solver {v} collide {c} tile {t} solve tile {w}: {ms:.3f} ms")
+            if ms < best_ms:
+                best_ms, plan = ms, trial
+    return plan
 
 
 def autotune(groups: Sequence[Group], steps: int = 3, warmup: int = 1, verbose: bool = False, log=print):

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -180,7 +181,7 @@ int check(const gsr_model* m, const gsr_state* s) {
   if (m->n_geoms >= (1 << 15)) return GSR_E_SHAPE;
   if (m->solver_variant == 1) {
     if (m->n_entities > kMaxPatternEntities) return GSR_E_SHAPE;
-    if (m->wave_h_floats < 0 || wave_lds_bytes(m->n_dofs, m->n_entities, m->wave_h_floats) > 160 * 1024)
+    if (m->wave_h_floats < 0 || wave_lds_bytes(*m) > 160 * 1024)
       return GSR_E_SHAPE;
   }
   if (m->collide_variant == 1) {
@@ -460,12 +461,13 @@ static int phase_lds(const gsr_batch_info& b, int phase) {
     const gsr_model& m = b.models[k];
     if (phase == 1 && m.collide_variant == 1) lds = std::max(lds, (size_t)wave_collide_lds_bytes(m.n_geoms));
     if (phase == 3 && m.solver_variant == 1)
-      lds = std::max(lds, (size_t)wave_lds_bytes(m.n_dofs, m.n_entities, m.wave_h_floats));
+      lds = std::max(lds, (size_t)wave_lds_bytes(m));
   }
   return (int)lds;
 }
 
-static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, size_t lds, gsr_batch_info* info);
+static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, int wave_tiles, size_t lds,
+                              gsr_batch_info* info);
 
 int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table, int64_t table_bytes,
                      gsr_batch_info* info) {
@@ -474,7 +476,7 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
   std::vector<BatchRange> ranges(n_groups);
   gsr_batch_info out;
   std::memset(&out, 0, sizeof(out));
-  int tiles = 0;
+  int tiles = 0, wtiles = 0;
   size_t lds = 0;
   for (int g = 0; g < n_groups; g++) {
     const gsr_group& G = groups[g];
@@ -482,6 +484,8 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
     if (rc) return rc;
     if (G.n_envs < 1 || G.env_begin < 0 || G.env_begin + G.n_envs > G.state.n_envs) return GSR_E_ARG;
     if (G.envs_per_tile < 1 || G.envs_per_tile > kWave) return GSR_E_ARG;
+    const int Tw = G.wave_envs_per_tile > 0 ? G.wave_envs_per_tile : G.envs_per_tile;
+    if (G.wave_envs_per_tile < 0 || Tw > kWave) return GSR_E_ARG;
     int type = -1;
     for (int k = 0; k < out.n_types; k++)
       if (!std::memcmp(&out.models[k], &G.model, sizeof(gsr_model)) &&
@@ -494,11 +498,13 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
       out.states[type] = G.state;
     }
     if (G.model.solver_variant == 1)
-      lds = std::max(lds, wave_lds_bytes(G.model.n_dofs, G.model.n_entities, G.model.wave_h_floats));
+      lds = std::max(lds, wave_lds_bytes(G.model));
     if (G.model.collide_variant == 1) lds = std::max(lds, wave_collide_lds_bytes(G.model.n_geoms));
-    int n_t = (G.n_envs + G.envs_per_tile - 1) / G.envs_per_tile;
-    ranges[g] = BatchRange{type, G.env_begin, G.n_envs, G.envs_per_tile, tiles, tiles + n_t, 0, 0};
+    const int n_t = (G.n_envs + G.envs_per_tile - 1) / G.envs_per_tile, n_w = (G.n_envs + Tw - 1) / Tw;
+    ranges[g] = BatchRange{type, G.env_begin, G.n_envs, G.envs_per_tile, tiles, tiles + n_t, Tw, wtiles, wtiles + n_w,
+                           0, 0, 0};
     tiles += n_t;
+    wtiles += n_w;
   }
   // Groups over one state must agree on the execution variants (the constraint-row and Hessian layouts follow the
   // solver variant, so two variants on one state would write each other's envs) and must not share envs.
@@ -519,7 +525,7 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
       if (B.env_begin < A.env_begin + A.n_envs) return GSR_E_ARG;
     }
   }
-  BatchHeader h{n_groups, tiles, 0u, 0u};
+  BatchHeader h{n_groups, tiles, 0u, 0u, wtiles, 0, 0, 0};
   FusedTypes ty;
   std::memset(&ty, 0, sizeof(ty));
   for (int k = 0; k < out.n_types; k++) {
@@ -532,13 +538,17 @@ int gsr_batch_upload(const gsr_group* groups, int32_t n_groups, void* dev_table,
     return GSR_E_LAUNCH;
   if (hipMemcpy(static_cast<char*>(dev_table) + kTypesOffset, &ty, sizeof(ty), hipMemcpyHostToDevice) != hipSuccess)
     return GSR_E_LAUNCH;
-  if (hipMemcpy(dev_table, &h, 2 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) return GSR_E_LAUNCH;
-  return finish_launch_info(out, n_groups, tiles, lds, info);
+  if (hipMemcpy(dev_table, &h, 2 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(static_cast<char*>(dev_table) + offsetof(BatchHeader, n_wave_tiles), &h.n_wave_tiles, sizeof(int32_t),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return GSR_E_LAUNCH;
+  return finish_launch_info(out, n_groups, tiles, wtiles, lds, info);
 }
 
 // Launch form of a batch whose types are in `out`: kernel attributes (dynamic LDS), grid and the form's flags, read
 // from `info` and written back to it.
-static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, size_t lds, gsr_batch_info* info) {
+static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, int wave_tiles, size_t lds,
+                              gsr_batch_info* info) {
   int dev = 0, n_cu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return GSR_E_LAUNCH;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return GSR_E_LAUNCH;
@@ -552,6 +562,7 @@ static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, size
       return GSR_E_LAUNCH;
     out.n_groups = n_groups;
     out.n_tiles = tiles;
+    out.n_wave_tiles = wave_tiles;
     out.persistent = 0;
     out.waves_per_simd = 0;
     out.gjk_inline = 0;
@@ -569,6 +580,7 @@ static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, size
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWave, lds) != hipSuccess) return GSR_E_LAUNCH;
   out.n_groups = n_groups;
   out.n_tiles = tiles;
+  out.n_wave_tiles = wave_tiles;
   out.persistent = info->persistent ? 1 : 0;
   out.waves_per_simd = W;
   out.gjk_inline = G;
@@ -581,7 +593,7 @@ static int finish_launch_info(gsr_batch_info& out, int n_groups, int tiles, size
 
 static bool fused_info_ok(const void* dev_table, const gsr_batch_info* info) {
   return dev_table && info && info->grid >= 1 && info->n_tiles >= 1 && info->n_types >= 1 &&
-         info->n_types <= GSR_MAX_TYPES;
+         info->n_types <= GSR_MAX_TYPES && (!info->phased || info->n_wave_tiles >= 1);
 }
 
 int gsr_fused_kernel_count(const gsr_batch_info* info) {
@@ -597,7 +609,7 @@ int gsr_launch_fused_kernel(const void* dev_table, const gsr_batch_info* info, i
   const char* t = static_cast<const char*>(dev_table);
   if (info->phased) {
     const int lds = k == 1 ? phase_lds(*info, 1) : k == 3 ? phase_lds(*info, 3) : 0;
-    launch_phase(k, dim3(info->n_tiles), lds, stream, t);
+    launch_phase(k, dim3(k == 2 || k == 3 ? info->n_wave_tiles : info->n_tiles), lds, stream, t);
   } else if (k == 0) {
     uint32_t* queue = reinterpret_cast<uint32_t*>(const_cast<char*>(t) + 2 * sizeof(int32_t));
     launch_fused(info->waves_per_simd == 4 ? 4 : 2, info->gjk_inline ? 1 : 0, dim3(info->grid), info->lds_bytes,
@@ -605,7 +617,7 @@ int gsr_launch_fused_kernel(const void* dev_table, const gsr_batch_info* info, i
   } else {
     const gsr_model& m = info->models[k - 1];
     const gsr_state& st = info->states[k - 1];
-    const size_t lds = m.solver_variant == 1 ? wave_lds_bytes(m.n_dofs, m.n_entities, m.wave_h_floats) : 0;
+    const size_t lds = m.solver_variant == 1 ? wave_lds_bytes(m) : 0;
     launch_gjk_resume(m, st, (unsigned)lds, stream);
   }
   return launch_status();
@@ -710,7 +722,7 @@ int gsr_serve_init(const gsr_serve_pool* pools, int32_t n_pools, double mem_budg
     out.models[p] = P.model;
     out.states[p] = P.state;
     if (P.model.solver_variant == 1)
-      lds = std::max(lds, wave_lds_bytes(P.model.n_dofs, P.model.n_entities, P.model.wave_h_floats));
+      lds = std::max(lds, wave_lds_bytes(P.model));
     if (P.model.collide_variant == 1) lds = std::max(lds, wave_collide_lds_bytes(P.model.n_geoms));
     tiles += P.n_tiles;
   }
@@ -721,7 +733,7 @@ int gsr_serve_init(const gsr_serve_pool* pools, int32_t n_pools, double mem_budg
                             stream);
   if (rc) return rc;
   // every tile may be active: the launches cover all of them (idle ones return at once)
-  return finish_launch_info(out, tiles, tiles, lds, info);
+  return finish_launch_info(out, tiles, tiles, tiles, lds, info);
 }
 
 int gsr_serve_tick(void* dev_state, void* dev_table, const gsr_batch_info* info, gsr_serve_host* host,

@@ -24,8 +24,7 @@ __global__ void __launch_bounds__(kWave) k_solve_wave(gsr_model m, gsr_state s) 
 
 void launch_solve(const gsr_model& m, const gsr_state& s, hipStream_t st) {
   if (m.solver_variant == 1)
-    hipLaunchKernelGGL(k_solve_wave, dim3(s.n_envs), dim3(kWave), wave_lds_bytes(m.n_dofs, m.n_entities,
-                       m.wave_h_floats), st, m, s);
+    hipLaunchKernelGGL(k_solve_wave, dim3(s.n_envs), dim3(kWave), wave_lds_bytes(m), st, m, s);
   else
     hipLaunchKernelGGL(k_solve, dim3((s.n_envs + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m, s);
 }

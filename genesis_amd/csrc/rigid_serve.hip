@@ -189,13 +189,14 @@ __global__ void __launch_bounds__(kSchedThreads) k_serve_schedule(ServeDev* S, c
     if (a) {
       const int p = pool_of_tile(S, g);
       const int ept = S->pools[p].envs_per_tile;
-      R[idx] = BatchRange{p, (g - S->pool_tile0[p]) * ept, T.n[g], ept, idx, idx + 1, 0, 0};
+      R[idx] = BatchRange{p, (g - S->pool_tile0[p]) * ept, T.n[g], ept, idx, idx + 1, ept, idx, idx + 1, 0, 0, 0};
     }
     base += chunk;
   }
   if (t == 0) {
     H->n_groups = base;
     H->n_tiles = base;
+    H->n_wave_tiles = base;
   }
 }
 
@@ -404,7 +405,7 @@ int serve_init(const gsr_serve_pool* pools, int n_pools, double mem_budget, doub
     ty->s[p] = pools[p].state;
   }
   int rc = GSR_OK;
-  BatchHeader bh{0, 0, 0u, 0u};
+  BatchHeader bh{0, 0, 0u, 0u, 0, 0, 0, 0};
   if ((size_t)state_bytes < serve_state_bytes(tiles) || (size_t)table_bytes < serve_table_bytes(tiles)) {
     rc = GSR_E_ARG;
   } else if (hipMemcpyAsync(dev_state, h, sizeof(ServeDev), hipMemcpyHostToDevice, stream) != hipSuccess ||

@@ -414,18 +414,23 @@ __device__ inline LSPoint dev_ls_point(const Env& e, SolverScalars& sc, float al
   return p;
 }
 
-__device__ inline void dev_update_bracket(const Env& e, SolverScalars& sc, const LSPoint* cand, LSPoint& p, int& flag,
-                                          LSPoint& pn) {
+// moves p to the candidates (c0, c1, c2 in this order) whose slope has its sign and is closer to zero; pn is the
+// next point from p when it moved. Named candidates, not an array, so they stay in registers.
+__device__ inline void dev_update_bracket(const Env& e, SolverScalars& sc, const LSPoint& c0, const LSPoint& c1,
+                                          const LSPoint& c2, LSPoint& p, int& flag, LSPoint& pn) {
   flag = 0;
-  for (int i = 0; i < 3; i++) {
-    if (p.d0 < 0.f && cand[i].d0 < 0.f && p.d0 < cand[i].d0) {
-      p = cand[i];
+  auto step = [&](const LSPoint& c) {
+    if (p.d0 < 0.f && c.d0 < 0.f && p.d0 < c.d0) {
+      p = c;
       flag = 1;
-    } else if (p.d0 > 0.f && cand[i].d0 > 0.f && p.d0 > cand[i].d0) {
-      p = cand[i];
+    } else if (p.d0 > 0.f && c.d0 > 0.f && p.d0 > c.d0) {
+      p = c;
       flag = 2;
     }
-  }
+  };
+  step(c0);
+  step(c1);
+  step(c2);
   pn = p;
   if (flag > 0) pn = dev_ls_point(e, sc, p.alpha - p.d0 / p.d1);
 }
@@ -492,18 +497,23 @@ __device__ inline float dev_linesearch(const Env& e, SolverScalars& sc) {
   LSPoint p1n = dev_ls_point(e, sc, p1.alpha - p1.d0 / p1.d1);
   while (sc.ls_it < m.ls_iterations) {
     LSPoint pm = dev_ls_point(e, sc, (p1.alpha + p2.alpha) * 0.5f);
-    LSPoint cand[3] = {p1n, p2n, pm};
+    const LSPoint c0 = p1n, c1 = p2n, c2 = pm;
     int best = -1;
-    float bcost = 0.f;
-    for (int ii = 0; ii < 3; ii++)
-      if (fabsf(cand[ii].d0) < gtol && (best < 0 || cand[ii].cost < bcost)) {
-        bcost = cand[ii].cost;
-        best = ii;
+    float bcost = 0.f, balpha = 0.f;
+    auto consider = [&](const LSPoint& c, int idx) {
+      if (fabsf(c.d0) < gtol && (best < 0 || c.cost < bcost)) {
+        bcost = c.cost;
+        best = idx;
+        balpha = c.alpha;
       }
-    if (best >= 0) return cand[best].alpha;
+    };
+    consider(c0, 0);
+    consider(c1, 1);
+    consider(c2, 2);
+    if (best >= 0) return balpha;
     int b1, b2;
-    dev_update_bracket(e, sc, cand, p1, b1, p1n);
-    dev_update_bracket(e, sc, cand, p2, b2, p2n);
+    dev_update_bracket(e, sc, c0, c1, c2, p1, b1, p1n);
+    dev_update_bracket(e, sc, c0, c1, c2, p2, b2, p2n);
     if (b1 == 0 && b2 == 0) return pm.alpha;
   }
   if (p1.cost <= p2.cost && p1.cost < p0.cost) return p1.alpha;

@@ -123,6 +123,9 @@ __host__ __device__ inline size_t wave_lds_bytes(int D, int E, int H) {
          sizeof(int) * 3 * (size_t)D + sizeof(int) * (9 * (size_t)E + 6) + sizeof(int) * (6 * (size_t)E + kWavePreSlots + 1) +
          64;
 }
+__host__ __device__ inline size_t wave_lds_bytes(const gsr_model& m) {
+  return wave_lds_bytes(m.n_dofs, m.n_entities, m.wave_h_floats);
+}
 
 __device__ inline WaveLds wave_lds_carve(lds_c* base, int D, int E, int H) {
   WaveLds w;
@@ -169,6 +172,12 @@ struct WaveCtx {
   const Env& e;
   const WaveLds& L;
   int lane, D, E, n_con, ne, nef;
+  // Models whose dofs all belong to one entity (an arm or a legged robot over a plane), at most 64 of them, with H
+  // in LDS: H is one dense packed triangle (dense_h), and the rows staged per pass of the dense sums below (0: the
+  // per-entity row walk) with their LDS staging area
+  bool dense_h;
+  int dense_rows;
+  lds_f* dense_buf;
   // entity-block mass matrix entry (i, j) of entity ie (model numbering: entity and dofs), geometry read from LDS
   __device__ __forceinline__ float mm(int ie, int i, int j) const {
     int st = L.nes[ie], n = L.nee[ie] - st;
@@ -475,6 +484,106 @@ struct HLds {
   __device__ __forceinline__ lds_f* blk(int i, int ei, int ek) const { return row(i, ei) + off(ei, ek) + L.es[ek]; }
 };
 
+// ---- single-entity models: rows staged densely --------------------------------------------------------------------
+// When every dof belongs to one entity, every row touches only that entity and its row list is every row in
+// ascending order. J^T f and the J^T D J part of the Hessian are then summed with one lane per dof (per entry) over
+// rows staged densely in LDS, dense_rows at a time, instead of one lane walking every row through global memory.
+// Each sum adds the general path's terms in the same row order from +0 (an absent dof contributes +-0, which changes
+// no bit of a sum that starts at +0), so both paths give the same bits.
+
+// lane-local: row c into staging slot `lane` (values over the model's dofs, zero elsewhere) with its scale, or a
+// zero row when `use` is false
+__device__ __forceinline__ void dense_stage_row(const WaveCtx& w, int c, bool use, float scale) {
+  const Env& e = w.e;
+  lds_f* row = w.dense_buf + w.lane * w.D;
+  for (int d = 0; d < w.D; d++) row[d] = 0.f;
+  if (use) {
+    const int nnz = e.EI(e.s.jac_nnz, c);
+    if (nnz <= kStage) {
+      RowStage r;
+      stage_row(e, c, nnz, r);
+#pragma unroll
+      for (int k = 0; k < kStage; k++)
+        if (k < nnz) row[r.d[k]] = r.v[k];
+    } else {
+      for (int k = 0; k < nnz; k++) row[e.JI(c, k)] = e.JV(c, k);
+    }
+  }
+  w.dense_buf[w.dense_rows * w.D + w.lane] = use ? scale : 0.f;
+}
+
+// J^T f into L.qfrc (lane d owns dof d)
+__device__ inline void wave_jtf_dense(const WaveCtx& w) {
+  const Env& e = w.e;
+  const int D = w.D, R = w.dense_rows;
+  const lds_f* sc = w.dense_buf + R * D;
+  float acc = 0.f;
+  for (int c0 = 0; c0 < w.n_con; c0 += R) {
+    const int nr = min(R, w.n_con - c0);
+    if (w.lane < nr) {
+      const float f = e.EF(e.s.efc_force, c0 + w.lane);
+      dense_stage_row(w, c0 + w.lane, f != 0.f, f);
+    }
+    __syncthreads();
+    if (w.lane < D) {
+      const lds_f* col = w.dense_buf + w.lane;
+      for (int r = 0; r < nr; r++) acc = acc + col[r * D] * sc[r];
+    }
+    __syncthreads();
+  }
+  if (w.lane < D) w.L.qfrc[w.lane] = acc;
+}
+
+// H = M + J^T D J over the active rows, lanes over the entries of the lower triangle (entry t = i (i + 1) / 2 + j,
+// j <= i, for t = lane, lane + 64, ...); each entry's partial sum waits in the entry itself between passes
+__device__ __forceinline__ void dense_entry(int t, int& i, int& j) {
+  i = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+  while (i * (i + 1) / 2 > t) i--;
+  while ((i + 1) * (i + 2) / 2 <= t) i++;
+  j = t - i * (i + 1) / 2;
+}
+
+template <class HS>
+__device__ inline void wave_hessian_dense(const WaveCtx& w, const HS& h) {
+  const Env& e = w.e;
+  const gsr_state& s = e.s;
+  const WaveLds& L = w.L;
+  const int D = w.D, R = w.dense_rows, ntri = D * (D + 1) / 2;
+  const float eps = e.m.eps;
+  const lds_f* sc = w.dense_buf + R * D;
+  const int v = L.ent_of_dof[0], ne = L.ord[v], n0 = L.nes[ne], f0 = L.es[v];
+  for (int c0 = 0; c0 < w.n_con; c0 += R) {
+    const int nr = min(R, w.n_con - c0);
+    if (w.lane < nr) {
+      const int c = c0 + w.lane;
+      dense_stage_row(w, c, (e.EI(s.efc_active, c) & 1) != 0, e.EF(s.efc_D, c));
+    }
+    __syncthreads();
+    for (int t = w.lane; t < ntri; t += kWave) {
+      int i, j;
+      dense_entry(t, i, j);
+      // pair (row's later dof i, earlier dof j), skipped when |J[c, i]| <= eps, as the row walk skips it
+      const lds_f* pi = w.dense_buf + i;
+      const lds_f* pj = w.dense_buf + j;
+      auto q = h.blk(f0 + i, v, v) + j;
+      float a = c0 == 0 ? 0.f : *q;
+      for (int r = 0; r < nr; r++) {
+        const float vi = pi[r * D];
+        const float vm = fabsf(vi) > eps ? vi : 0.f;
+        a = a + pj[r * D] * vm * sc[r];
+      }
+      *q = a;
+    }
+    __syncthreads();
+  }
+  for (int t = w.lane; t < ntri; t += kWave) {
+    int i, j;
+    dense_entry(t, i, j);
+    auto q = h.blk(f0 + i, v, v) + j;
+    *q = (w.n_con > 0 ? *q : 0.f) + w.mm(ne, n0 + i, n0 + j);
+  }
+}
+
 // M x (entity blocks), one dof per lane, serial inner sum in the reference order
 __device__ inline void wave_mass_mul(const WaveCtx& w, const lds_f* x, lds_f* out) {
   const Env& e = w.e;
@@ -518,9 +627,13 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, fl
   // rows once and accumulates into the entity's own dofs (no other lane writes them)
   const int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
   const int* lst = s.ent_row_list + (size_t)e.b * 2 * e.m.max_rows;
-  for (int d = w.lane; d < w.D; d += kWave) L.qfrc[d] = 0.f;
-  __syncthreads();
-  for (int ie = w.lane; ie < w.E; ie += kWave) {
+  if (w.dense_rows > 0) {
+    wave_jtf_dense(w);
+  } else {
+    for (int d = w.lane; d < w.D; d += kWave) L.qfrc[d] = 0.f;
+    __syncthreads();
+  }
+  for (int ie = w.lane; ie < w.E && w.dense_rows == 0; ie += kWave) {
     const int s0 = L.nes[ie], s1 = L.nee[ie];
     if (s1 <= s0) continue;
     // two rows per iteration: both rows' loads are issued before either is accumulated (in row order)
@@ -535,15 +648,27 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, fl
         RowStage ra, rb;
         stage_row(e, ca, fa != 0.f ? na : 0, ra);
         stage_row(e, cb, fb != 0.f ? nb : 0, rb);
+        // a row's entries are distinct dofs: all are read before any is written back
+        float old_v[kStage];
 #pragma unroll
         for (int k = 0; k < kStage; k++) {
-          int d = ra.d[k];
-          if (k < na && fa != 0.f && d >= s0 && d < s1) L.qfrc[d] = L.qfrc[d] + ra.v[k] * fa;
+          const int d = ra.d[k];
+          old_v[k] = (k < na && fa != 0.f && d >= s0 && d < s1) ? L.qfrc[d] : 0.f;
         }
 #pragma unroll
         for (int k = 0; k < kStage; k++) {
-          int d = rb.d[k];
-          if (k < nb && fb != 0.f && d >= s0 && d < s1) L.qfrc[d] = L.qfrc[d] + rb.v[k] * fb;
+          const int d = ra.d[k];
+          if (k < na && fa != 0.f && d >= s0 && d < s1) L.qfrc[d] = old_v[k] + ra.v[k] * fa;
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; k++) {
+          const int d = rb.d[k];
+          old_v[k] = (k < nb && fb != 0.f && d >= s0 && d < s1) ? L.qfrc[d] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; k++) {
+          const int d = rb.d[k];
+          if (k < nb && fb != 0.f && d >= s0 && d < s1) L.qfrc[d] = old_v[k] + rb.v[k] * fb;
         }
       } else {
         for (int q = 0; q < (two ? 2 : 1); q++) {
@@ -608,6 +733,103 @@ __device__ __forceinline__ LaneGroup lane_group(int n, int lane) {
 __device__ __forceinline__ float group_sum(float v, int G) {
   for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
   return v;
+}
+
+// ---- single-entity models: the factor as one dense packed triangle ------------------------------------------------
+// With every dof in one entity and H in LDS, H is the packed lower triangle at the start of the LDS store, H(i, j) at
+// hl[i (i + 1) / 2 + j] (factor and model numbering agree). The factorisation, the substitutions and the rank-1
+// updates then run with one lane per row and every sum serial over its index, the operation order of
+// func_nt_chol_factor / func_nt_chol_solve / func_nt_hessian_incremental (constraint_solver_decomp.py), instead of
+// the chunked per-entity forms below with their pattern lookups.
+__device__ __forceinline__ lds_f* dense_row(const WaveLds& L, int i) { return L.hl + i * (i + 1) / 2; }
+
+__device__ inline void wave_chol_factor_dense(const WaveCtx& w) {
+  const float EPS = w.e.m.eps;
+  const int D = w.D;
+  for (int i = 0; i < D; i++) {
+    lds_f* ri = dense_row(w.L, i);
+    float tmp = ri[i];
+#pragma unroll 8
+    for (int j = 0; j < i; j++) {
+      const float h = ri[j];
+      tmp -= h * h;
+    }
+    const float hii = sqrtf(fmaxf(tmp, EPS));
+    const float inv = 1.f / hii;
+    // rows j > i, one per lane
+    for (int j = i + 1 + w.lane; j < D; j += kWave) {
+      lds_f* rj = dense_row(w.L, j);
+      float dt = 0.f;
+#pragma unroll 8
+      for (int k = 0; k < i; k++) dt += rj[k] * ri[k];
+      rj[i] = (rj[i] - dt) * inv;
+    }
+    if (w.lane == 0) ri[i] = hii;
+    __syncthreads();
+  }
+}
+
+// x = H^-1 x in place, lane d holding x[d] in a register: forward substitution (row i subtracts L(i, j) y(j) for j
+// ascending as y(j) becomes final: the serial order of func_nt_chol_solve), then backward substitution column by
+// column (row i subtracts L(j, i) x(j) for j descending as x(j) becomes final)
+__device__ inline void wave_chol_solve_dense(const WaveCtx& w, lds_f* x) {
+  const int D = w.D, lane = w.lane;
+  __syncthreads();
+  float a = lane < D ? x[lane] : 0.f;
+  const lds_f* rl = dense_row(w.L, lane < D ? lane : 0);
+  for (int j = 0; j < D; j++) {
+    const float yj = __shfl(a, j, kWave) / dense_row(w.L, j)[j];
+    if (lane == j) a = yj;
+    else if (lane > j && lane < D) a = a - rl[j] * yj;
+  }
+  for (int i = D - 1; i >= 0; i--) {
+    const lds_f* ri = dense_row(w.L, i);
+    const float xi = __shfl(a, i, kWave) / ri[i];
+    if (lane == i) a = xi;
+    else if (lane < i) a = a - ri[lane] * xi;
+  }
+  if (lane < D) x[lane] = a;
+  __syncthreads();
+}
+
+// rank-1 update / downdate of the factor with row c; returns false on rank loss (the update is then incomplete and
+// the caller rebuilds the factor)
+__device__ inline bool wave_rank1_dense(const WaveCtx& w, int c, float sgn) {
+  const Env& e = w.e;
+  const gsr_state& s = e.s;
+  const float eps = e.m.eps;
+  const int D = w.D, lane = w.lane;
+  const float sq = sqrtf(e.EF(s.efc_D, c));
+  lds_f* x = w.L.vec;
+  for (int d = lane; d < D; d += kWave) x[d] = 0.f;
+  __syncthreads();
+  const int nnz = e.EI(s.jac_nnz, c);
+  for (int k = lane; k < nnz; k += kWave) x[e.JI(c, k)] = e.JV(c, k) * sq;
+  __syncthreads();
+  float xi = lane < D ? x[lane] : 0.f;
+  lds_f* rl = dense_row(w.L, lane < D ? lane : 0);
+  bool ok = true;
+  for (int k = 0; k < D; k++) {
+    const float vk = __shfl(xi, k, kWave);
+    if (!(fabsf(vk) > eps)) continue;
+    lds_f* dk = dense_row(w.L, k) + k;
+    const float Lkk = *dk;
+    float tmp = Lkk * Lkk + vk * vk * sgn;
+    if (tmp < eps) {
+      tmp = eps;
+      ok = false;
+    }
+    const float r = sqrtf(tmp), cc = r / Lkk, cinv = 1.f / cc, sv = vk / Lkk;
+    if (lane > k && lane < D) {
+      const float hv = (rl[k] + sv * xi * sgn) * cinv;
+      rl[k] = hv;
+      xi = xi * cc - sv * hv;
+    }
+    __syncthreads();
+    if (lane == 0) *dk = r;
+    __syncthreads();
+  }
+  return ok;
 }
 
 // Left-looking Cholesky over the pattern (func_nt_chol_factor's entries), level by level, lane groups per entity,
@@ -760,6 +982,13 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
   const WaveLds& L = w.L;
   const int* ptr = s.ent_row_ptr + (size_t)e.b * (w.E + 1);
   const int* lst = s.ent_row_list + (size_t)e.b * 2 * m.max_rows;
+  if (w.dense_rows > 0) {
+    wave_hessian_dense(w, h);
+    __syncthreads();
+    if (!HS::kFence && w.dense_h) wave_chol_factor_dense(w);
+    else wave_chol_factor(w, h);
+    return;
+  }
   for (int ei = w.lane; ei < w.E; ei += kWave) {
     const int s0 = L.es[ei], s1 = L.ee[ei];
     if (s1 <= s0) continue;
@@ -781,8 +1010,12 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
     // the entity's rows in ascending order; each unordered dof pair of a row once (k2 <= k1 in the row's model
     // order, as the reference walks it), stored at (max, min) of its factor dofs by the lane owning the max
     const int ne = L.ord[ei];
-    for (int t = ptr[ne]; t < ptr[ne + 1]; t++) {
-      int c = lst[t];
+    const int t_end = ptr[ne + 1];
+    int c_next = ptr[ne] < t_end ? lst[ptr[ne]] : 0;
+    for (int t = ptr[ne]; t < t_end; t++) {
+      // the next row's index is loaded while this row is summed
+      const int c = c_next;
+      if (t + 1 < t_end) c_next = lst[t + 1];
       // an inactive row adds +-0 to entries that start at +0 and so never hold -0: skipping it changes no bit
       if (!(e.EI(s.efc_active, c) & 1)) continue;
       int nnz = e.EI(s.jac_nnz, c);
@@ -790,9 +1023,10 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
       if (nnz <= kStage) {
         RowStage r;
         stage_row(e, c, nnz, r);
-        // the row's other entity (by position); its dofs are columns of this entity's rows only when it comes first
-        int na, nb;
-        row_entities(w, c, na, nb);
+        // the row's other entity (by position); its dofs are columns of this entity's rows only when it comes first.
+        // The pair was stored by the structure pass (one load, beside the row's own, instead of a chain through nnz)
+        const int ent = s.efc_ent[(size_t)e.b * m.max_rows + c];
+        const int na = ent & 0xffff, nb = ent >> 16;
         const int va = L.vof[na], vb = L.vof[nb];
         const int eo = va == ei ? vb : va;
         const int off_own = h.off(ei, ei), off_oth = eo < ei ? h.off(ei, eo) : 0;
@@ -812,14 +1046,20 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
         for (int k1 = 0; k1 < kStage; k1++) {
           const float v1 = r.v[k1];
           if (k1 < nnz && fabsf(v1) > m.eps) {
+            // the pairs (k1, k2 <= k1) are distinct entries: all of them are read before any is written back, so
+            // the reads overlap instead of each waiting for the previous entry's write
+            float old_v[kStage];
 #pragma unroll
             for (int k2 = 0; k2 <= k1; k2++) {
               // entry (max, min) of the two factor dofs, kept by the lane whose entity holds the max
               const bool first = pd[k1] >= pd[k2];
-              if ((own >> (first ? k1 : k2)) & 1u) {
-                auto q = hb + (first ? ro[k1] + co[k2] : ro[k2] + co[k1]);
-                *q = *q + r.v[k2] * v1 * Dc * act;
-              }
+              old_v[k2] = ((own >> (first ? k1 : k2)) & 1u) ? hb[first ? ro[k1] + co[k2] : ro[k2] + co[k1]] : 0.f;
+            }
+#pragma unroll
+            for (int k2 = 0; k2 <= k1; k2++) {
+              const bool first = pd[k1] >= pd[k2];
+              if ((own >> (first ? k1 : k2)) & 1u)
+                hb[first ? ro[k1] + co[k2] : ro[k2] + co[k1]] = old_v[k2] + r.v[k2] * v1 * Dc * act;
             }
           }
         }
@@ -846,7 +1086,8 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
     }
   }
   __syncthreads();
-  wave_chol_factor(w, h);
+  if (!HS::kFence && w.dense_h) wave_chol_factor_dense(w);
+  else wave_chol_factor(w, h);
 }
 
 // rank-1 update / downdate of the factor with row c (func_nt_hessian_incremental); returns false on rank loss
@@ -1091,7 +1332,8 @@ __device__ __forceinline__ double wave_update_gradient(const WaveCtx& w, const H
     g2 += (double)(g * g);
     L.vec[L.pdof[d]] = g;
   }
-  wave_chol_solve(w, h, L.vec);
+  if (!HS::kFence && w.dense_h) wave_chol_solve_dense(w, L.vec);
+  else wave_chol_solve(w, h, L.vec);
   __syncthreads();
   return g2;
 }
@@ -1100,30 +1342,48 @@ struct WLS {
   float alpha, cost, d0, d1;
 };
 
-__device__ inline WLS wave_ls_point(const WaveCtx& w, float qg0, float qg1, float qg2, float alpha, int& ls_it) {
+// The line search's per-row terms of rows lane, lane + 64, ..., up to kLsRows of them per lane, stay in registers
+// across its points; rows beyond are read back from the env's row arrays. Each point sums the same terms in the same
+// order either way.
+constexpr int kLsRows = 1;
+struct LsCache {
+  float ja[kLsRows], jv[kLsRows], q0[kLsRows], q1[kLsRows], q2[kLsRows];
+};
+
+__device__ __forceinline__ void ls_row_terms(const WaveCtx& w, int c, float ja, float jv, float q0, float q1, float q2,
+                                             float alpha, double& t0, double& t1, double& t2) {
+  const Env& e = w.e;
+  const gsr_state& s = e.s;
+  float x = ja + alpha * jv;
+  bool act = true;
+  if (w.ne <= c && c < w.nef) {
+    float f = e.EF(s.efc_fl, c), r = e.EF(s.efc_diag, c), rf = r * f;
+    bool ln = x <= -rf, lp = x >= rf;
+    if (ln || lp) {
+      q0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
+      q1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
+      q2 = 0.f;
+    }
+  } else if (w.nef <= c) {
+    act = x < 0.f;
+  }
+  t0 += (double)(q0 * (float)act);
+  t1 += (double)(q1 * (float)act);
+  t2 += (double)(q2 * (float)act);
+}
+
+__device__ __forceinline__ WLS wave_ls_point(const WaveCtx& w, const LsCache& rc, float qg0, float qg1, float qg2,
+                                             float alpha, int& ls_it) {
   const Env& e = w.e;
   const gsr_state& s = e.s;
   double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-  for (int c = w.lane; c < w.n_con; c += kWave) {
-    float ja = e.EF(s.efc_Jaref, c), jv = e.EF(s.efc_jv, c);
-    float x = ja + alpha * jv;
-    float q0 = e.QD(c, 0), q1 = e.QD(c, 1), q2 = e.QD(c, 2);
-    bool act = true;
-    if (w.ne <= c && c < w.nef) {
-      float f = e.EF(s.efc_fl, c), r = e.EF(s.efc_diag, c), rf = r * f;
-      bool ln = x <= -rf, lp = x >= rf;
-      if (ln || lp) {
-        q0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
-        q1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
-        q2 = 0.f;
-      }
-    } else if (w.nef <= c) {
-      act = x < 0.f;
-    }
-    t0 += (double)(q0 * (float)act);
-    t1 += (double)(q1 * (float)act);
-    t2 += (double)(q2 * (float)act);
+#pragma unroll
+  for (int k = 0; k < kLsRows; k++) {
+    const int c = w.lane + kWave * k;
+    if (c < w.n_con) ls_row_terms(w, c, rc.ja[k], rc.jv[k], rc.q0[k], rc.q1[k], rc.q2[k], alpha, t0, t1, t2);
   }
+  for (int c = w.lane + kWave * kLsRows; c < w.n_con; c += kWave)
+    ls_row_terms(w, c, e.EF(s.efc_Jaref, c), e.EF(s.efc_jv, c), e.QD(c, 0), e.QD(c, 1), e.QD(c, 2), alpha, t0, t1, t2);
   const float T0 = (float)((double)qg0 + wave_sum_d(t0));
   const float T1 = (float)((double)qg1 + wave_sum_d(t1));
   const float T2 = (float)((double)qg2 + wave_sum_d(t2));
@@ -1150,7 +1410,24 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
   if (snorm < m.eps) return 0.f;
   wave_mass_mul(w, L.search, L.mv);
   __syncthreads();
-  for (int c = w.lane; c < w.n_con; c += kWave) {
+  LsCache rc;
+#pragma unroll
+  for (int k = 0; k < kLsRows; k++) {
+    const int c = w.lane + kWave * k;
+    float jv = 0.f, ja = 0.f, Dc = 0.f;
+    if (c < w.n_con) {
+      jv = row_dot_vec(e, c, e.EI(s.jac_nnz, c), jv, L.search);
+      e.EF(s.efc_jv, c) = jv;
+      Dc = e.EF(s.efc_D, c);
+      ja = e.EF(s.efc_Jaref, c);
+    }
+    rc.ja[k] = ja;
+    rc.jv[k] = jv;
+    rc.q0[k] = Dc * (0.5f * ja * ja);
+    rc.q1[k] = Dc * (jv * ja);
+    rc.q2[k] = Dc * (0.5f * jv * jv);
+  }
+  for (int c = w.lane + kWave * kLsRows; c < w.n_con; c += kWave) {
     float jv = 0.f;
     int nnz = e.EI(s.jac_nnz, c);
     jv = row_dot_vec(e, c, nnz, jv, L.search);
@@ -1169,8 +1446,8 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
   float qg0 = gauss, qg1 = (float)wave_sum_d(a1), qg2 = (float)wave_sum_d(a2);
   __syncthreads();
   int ls_it = 0;
-  WLS p0 = wave_ls_point(w, qg0, qg1, qg2, 0.f, ls_it);
-  WLS p1 = wave_ls_point(w, qg0, qg1, qg2, p0.alpha - p0.d0 / p0.d1, ls_it);
+  WLS p0 = wave_ls_point(w, rc, qg0, qg1, qg2, 0.f, ls_it);
+  WLS p1 = wave_ls_point(w, rc, qg0, qg1, qg2, p0.alpha - p0.d0 / p0.d1, ls_it);
   if (p0.cost < p1.cost) p1 = p0;
   if (fabsf(p1.d0) < gtol) return p1.alpha;
   float direction = (p1.d0 < 0.f) ? 1.f : -1.f;
@@ -1179,39 +1456,49 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
   while (p1.d0 * direction <= -gtol && ls_it < m.ls_iterations) {
     p2 = p1;
     p2update = 1;
-    p1 = wave_ls_point(w, qg0, qg1, qg2, p1.alpha - p1.d0 / p1.d1, ls_it);
+    p1 = wave_ls_point(w, rc, qg0, qg1, qg2, p1.alpha - p1.d0 / p1.d1, ls_it);
     if (fabsf(p1.d0) < gtol) return p1.alpha;
   }
   if (ls_it >= m.ls_iterations) return p1.alpha;
   if (!p2update) return p1.alpha;
   WLS p2n = p1;
-  WLS p1n = wave_ls_point(w, qg0, qg1, qg2, p1.alpha - p1.d0 / p1.d1, ls_it);
+  WLS p1n = wave_ls_point(w, rc, qg0, qg1, qg2, p1.alpha - p1.d0 / p1.d1, ls_it);
   while (ls_it < m.ls_iterations) {
-    WLS pm = wave_ls_point(w, qg0, qg1, qg2, (p1.alpha + p2.alpha) * 0.5f, ls_it);
-    WLS cand[3] = {p1n, p2n, pm};
+    WLS pm = wave_ls_point(w, rc, qg0, qg1, qg2, (p1.alpha + p2.alpha) * 0.5f, ls_it);
+    // candidates p1n, p2n, pm in this order (named values, not an indexed array, so they stay in registers)
+    const WLS c0 = p1n, c1 = p2n, c2 = pm;
     int best = -1;
-    float bcost = 0.f;
-    for (int ii = 0; ii < 3; ii++)
-      if (fabsf(cand[ii].d0) < gtol && (best < 0 || cand[ii].cost < bcost)) {
-        bcost = cand[ii].cost;
-        best = ii;
+    float bcost = 0.f, balpha = 0.f;
+    auto consider = [&](const WLS& c, int idx) {
+      if (fabsf(c.d0) < gtol && (best < 0 || c.cost < bcost)) {
+        bcost = c.cost;
+        best = idx;
+        balpha = c.alpha;
       }
-    if (best >= 0) return cand[best].alpha;
-    int flags[2];
-    WLS* ps[2] = {&p1, &p2};
-    WLS* pns[2] = {&p1n, &p2n};
-    for (int q = 0; q < 2; q++) {
-      WLS& p = *ps[q];
+    };
+    consider(c0, 0);
+    consider(c1, 1);
+    consider(c2, 2);
+    if (best >= 0) return balpha;
+    // p1, then p2, moves to the candidates whose slope has its sign and is closer to zero
+    auto narrow = [&](WLS& p) {
       int flag = 0;
-      for (int i = 0; i < 3; i++) {
-        if (p.d0 < 0.f && cand[i].d0 < 0.f && p.d0 < cand[i].d0) { p = cand[i]; flag = 1; }
-        else if (p.d0 > 0.f && cand[i].d0 > 0.f && p.d0 > cand[i].d0) { p = cand[i]; flag = 2; }
-      }
-      *pns[q] = p;
-      if (flag > 0) *pns[q] = wave_ls_point(w, qg0, qg1, qg2, p.alpha - p.d0 / p.d1, ls_it);
-      flags[q] = flag;
-    }
-    if (flags[0] == 0 && flags[1] == 0) return pm.alpha;
+      auto step = [&](const WLS& c) {
+        if (p.d0 < 0.f && c.d0 < 0.f && p.d0 < c.d0) { p = c; flag = 1; }
+        else if (p.d0 > 0.f && c.d0 > 0.f && p.d0 > c.d0) { p = c; flag = 2; }
+      };
+      step(c0);
+      step(c1);
+      step(c2);
+      return flag;
+    };
+    const int f1 = narrow(p1);
+    p1n = p1;
+    if (f1 > 0) p1n = wave_ls_point(w, rc, qg0, qg1, qg2, p1.alpha - p1.d0 / p1.d1, ls_it);
+    const int f2 = narrow(p2);
+    p2n = p2;
+    if (f2 > 0) p2n = wave_ls_point(w, rc, qg0, qg1, qg2, p2.alpha - p2.d0 / p2.d1, ls_it);
+    if (f1 == 0 && f2 == 0) return pm.alpha;
   }
   if (p1.cost <= p2.cost && p1.cost < p0.cost) return p1.alpha;
   if (p2.cost <= p1.cost && p2.cost < p1.cost) return p2.alpha;
@@ -1269,7 +1556,8 @@ __device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, double
             int j = __ffsll((long long)chg) - 1;
             chg &= chg - 1;
             int aj = __shfl(a, j, kWave);
-            if (!wave_rank1(w, h, c0 + j, (aj & 1) ? 1.f : -1.f)) {
+            const float sgn = (aj & 1) ? 1.f : -1.f;
+            if (!((!HS::kFence && w.dense_h) ? wave_rank1_dense(w, c0 + j, sgn) : wave_rank1(w, h, c0 + j, sgn))) {
               wave_hessian_direct(w, h);
               rebuilt = true;
             }
@@ -1300,7 +1588,25 @@ __device__ __forceinline__ void wave_solve(const Env& e, char* lds) {
   const int D = m.n_dofs, E = m.n_entities;
   WaveLds L = wave_lds_carve((lds_c*)lds, D, E, m.wave_h_floats);
   int n_con = e.I(s.n_con, 0), ne = e.I(s.n_con, 1), nef = ne + e.I(s.n_con, 2);
-  WaveCtx w{e, L, lane, D, E, n_con, ne, nef};
+  // the dense row sums when one entity holds every dof (at most 64, with H in LDS and room after it for 8 or more
+  // staged rows)
+  bool dense_h = false;
+  int dense_rows = 0;
+  lds_f* dense_buf = nullptr;
+  {
+    int n_ent = 0;
+    for (int i = 0; i < E; i++) n_ent += m.ent_dof_end[i] > m.ent_dof_start[i] ? 1 : 0;
+    const int tri = D * (D + 1) / 2, tri4 = (tri + 3) & ~3;
+    dense_h = n_ent == 1 && D <= kWave && tri <= m.wave_h_floats;
+    if (dense_h && tri4 < m.wave_h_floats) {
+      const int R = min(kWave, (m.wave_h_floats - tri4) / (D + 1));
+      if (R >= 8) {
+        dense_rows = R;
+        dense_buf = L.hl + tri4;
+      }
+    }
+  }
+  WaveCtx w{e, L, lane, D, E, n_con, ne, nef, dense_h, dense_rows, dense_buf};
   int ws = e.I(s.is_warmstart, 0);
   // force (the smooth force until the epilogue) and acc_smooth are read from the state: nothing writes them here
   for (int d = lane; d < D; d += kWave) L.qacc[d] = (n_con > 0 && ws) ? e.F(s.qacc_ws, d) : e.F(s.acc_smooth, d);

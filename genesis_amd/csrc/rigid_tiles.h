@@ -59,9 +59,15 @@ __device__ __forceinline__ bool gjk_pending(const Env& e) { return e.I(e.s.gjk_d
 struct BatchHeader {
   int32_t n_groups, n_tiles;
   uint32_t counter, done;  // work queue: next tile, finished workgroups (the last one resets both)
+  int32_t n_wave_tiles;    // tiles of the phased launch's wavefront-per-env stages (GJK/EPA pass, solve)
+  int32_t pad0, pad1, pad2;
 };
+// A group's envs cut into tiles twice: envs_per_tile for the single fused kernel and the phased per-env stages, and
+// wave_envs_per_tile for the phased GJK/EPA pass and solve, whose wavefront variants take a tile's envs one after
+// the other (a lane-per-env collision wants 64 envs a tile, a wavefront solve few)
 struct BatchRange {
-  int32_t type, env_begin, n_envs, envs_per_tile, tile_begin, tile_end, pad0, pad1;
+  int32_t type, env_begin, n_envs, envs_per_tile, tile_begin, tile_end;
+  int32_t wave_envs_per_tile, wave_tile_begin, wave_tile_end, pad0, pad1, pad2;
 };
 struct FusedTypes {
   gsr_model m[GSR_MAX_TYPES];
@@ -202,18 +208,19 @@ __global__ void __launch_bounds__(kWave) k_phase(const char* __restrict__ table)
   const BatchHeader* __restrict__ H = reinterpret_cast<const BatchHeader*>(table);
   ConstTypes* types = (ConstTypes*)(table + kTypesOffset);
   const BatchRange* __restrict__ R = reinterpret_cast<const BatchRange*>(table + kRangesOffset);
-  const int n_tiles = H->n_tiles, n_groups = H->n_groups;
+  constexpr bool kWaveTiles = P == 2 || P == 3;
+  const int n_tiles = kWaveTiles ? H->n_wave_tiles : H->n_tiles, n_groups = H->n_groups;
   if ((int)blockIdx.x >= n_tiles) return;
   const int tile = xcd_env(blockIdx.x, n_tiles);
   int lo = 0, hi = n_groups - 1;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (tile < R[mid].tile_end) hi = mid;
+    if (tile < (kWaveTiles ? R[mid].wave_tile_end : R[mid].tile_end)) hi = mid;
     else lo = mid + 1;
   }
   const BatchRange g = R[lo];
-  const int T = g.envs_per_tile;
-  const int first = (tile - g.tile_begin) * T;
+  const int T = kWaveTiles ? g.wave_envs_per_tile : g.envs_per_tile;
+  const int first = (tile - (kWaveTiles ? g.wave_tile_begin : g.tile_begin)) * T;
   const int n_here = min(T, g.n_envs - first);
   const int k = __builtin_amdgcn_readfirstlane(g.type);
   const FusedTypes* ty = (const FusedTypes*)types;

@@ -59,16 +59,28 @@ WAVE_MAX_ENTITIES = 128
 
 
 # Up to this many envs, one wavefront per env still fills the chip (8 waves per CU at 2,048 envs) and cuts the
-# per-env latency that a few envs' step is made of; beyond it, small articulations run one lane per env.
+# per-env latency that a few envs' step is made of; beyond it, the dynamics and collision stages of a small
+# articulation run one lane per env.
 WAVE_MAX_SMALL_ENVS = 2048
 
 
 def default_solver_variant(M: RigidModel, n_envs: int = None) -> int:
-    """1 = one wavefront per env (large or many-entity systems, or few envs), 0 = one lane per env (many envs of a
-    small articulation)."""
-    if M.n_entities > WAVE_MAX_ENTITIES:
-        return 0
+    """1 = one wavefront per env (every model the pattern bitsets cover: the Newton solve of a many-contact env is
+    latency-bound, and the wavefront solver is the faster one at every env count measured), 0 = one lane per env."""
+    return 0 if M.n_entities > WAVE_MAX_ENTITIES else 1
+
+
+def default_stage_variant(M: RigidModel, n_envs: int = None) -> int:
+    """Variant of the dynamics and collision stages: 1 = one wavefront per env (large systems, or few envs), 0 = one
+    lane per env (many envs of a small articulation)."""
     return 1 if (M.n_dofs >= 48 or (n_envs is not None and n_envs <= WAVE_MAX_SMALL_ENVS)) else 0
+
+
+def default_collide_variant(M: RigidModel, n_envs: int = None) -> int:
+    """As default_stage_variant, except that mesh geoms take the wavefront collision (support-field lookups diverge
+    across a lane-per-env wavefront: the Panda's meshes collide 10x slower one lane per env at 5,492 envs)."""
+    from .model import GEOM_MESH
+    return 1 if (np.asarray(M.geom_type) == GEOM_MESH).any() else default_stage_variant(M, n_envs)
 
 
 def state_sizes(M: RigidModel) -> dict:
@@ -153,11 +165,16 @@ class DeviceModel:
         d.iterations, d.ls_iterations, d.n_contacts_per_pair = o.iterations, o.ls_iterations, o.n_contacts_per_pair
         d.enable_collision, d.enable_joint_limit = int(o.enable_collision), int(o.enable_joint_limit)
         d.enable_multi_contact = int(o.enable_multi_contact)
-        d.solver_variant = default_solver_variant(M, n_envs) if solver_variant is None else int(solver_variant)
+        # an explicit solver variant is also the default of the other stages; otherwise each takes its own default
+        explicit = solver_variant is not None
+        d.solver_variant = int(solver_variant) if explicit else default_solver_variant(M, n_envs)
         if d.solver_variant == 1 and M.n_entities > WAVE_MAX_ENTITIES:
             raise ValueError(f"wavefront solver supports at most {WAVE_MAX_ENTITIES} entities")
-        d.collide_variant = d.solver_variant if collide_variant is None else int(collide_variant)
-        d.dyn_variant = d.solver_variant if dyn_variant is None else int(dyn_variant)
+        if collide_variant is None:
+            collide_variant = d.solver_variant if explicit else default_collide_variant(M, n_envs)
+        if dyn_variant is None:
+            dyn_variant = d.solver_variant if explicit else default_stage_variant(M, n_envs)
+        d.collide_variant, d.dyn_variant = int(collide_variant), int(dyn_variant)
         d.batch_dofs = int(self.batch_dofs)
         d.wave_h_floats = wave_h_floats_default(M) if wave_h_floats is None else int(wave_h_floats)
         d.dt, d.tolerance, d.ls_tolerance = o.dt, o.tolerance, o.ls_tolerance

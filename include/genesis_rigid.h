@@ -211,6 +211,9 @@ typedef struct gsr_group {
   int32_t n_envs;          /* envs of the group: [env_begin, env_begin + n_envs) */
   int32_t envs_per_tile;   /* 1..64; the per-lane stages use one lane per env of the tile */
   int32_t tile_begin;      /* written by gsr_batch_upload: first tile index of the group */
+  int32_t wave_envs_per_tile; /* 0..64, phased launch only: envs per tile of the GJK/EPA pass and the solve, whose
+                                 wavefront variants take a tile's envs one after the other (0 = envs_per_tile) */
+  int32_t pad;
 } gsr_group;
 
 typedef struct gsr_batch_info {
@@ -223,7 +226,7 @@ typedef struct gsr_batch_info {
   int32_t gjk_inline;        /* input: 1 = GJK/EPA inside the fused kernel, 0 = deferred to a per-type resume launch */
   int32_t phased;            /* input: 1 = one launch per substep stage over every tile (persistent, waves_per_simd
                                 and gjk_inline are then ignored), 0 = one fused launch per substep */
-  int32_t pad;
+  int32_t n_wave_tiles;      /* phased: tiles of the GJK/EPA pass and the solve (wave_envs_per_tile) */
   gsr_model models[GSR_MAX_TYPES];
   gsr_state states[GSR_MAX_TYPES];
 } gsr_batch_info;

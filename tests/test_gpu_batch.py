@@ -102,6 +102,37 @@ def test_fused_equals_plain_bitwise(variants, li):
         S.check_errno()
 
 
+@pytest.mark.parametrize("variants,tiles,solve_tiles,deep", [
+    ((1, 0, 0), (64, 64, 16), (1, 2, 1), False),
+    ((1, 0, 0), (32, 64, 12), (2, 4, 3), True),
+    ((1, 1, 1), (8, 4, 2), (2, 1, 1), True),
+    ((0, 0, 0), (64, 64, 3), (16, 8, 1), True),
+])
+def test_phased_solve_tiles_equal_plain_bitwise(variants, tiles, solve_tiles, deep):
+    """The phased launch cuts each group twice: envs_per_tile for the per-env stages and solve tiles for the GJK/EPA
+    pass and the Newton solve. Any pair of tilings gives the plain step's bits."""
+    from genesis_amd.batch import FusedBatch, Group
+    sv, cv, dv = variants
+    A, B = _parts(COUNTS), _parts(COUNTS)
+    for parts in (A, B):
+        for _, _, S, _ in parts:
+            S.set_variants(sv, cv, dv)
+        if deep:
+            _overlap_pile(parts)
+    fb = FusedBatch([Group(S, envs_per_tile=t, wave_envs_per_tile=w)
+                     for (_, _, S, _), t, w in zip(B, tiles, solve_tiles)], phased=True)
+    assert fb.info.n_wave_tiles == sum(-(-S._B // w) for (_, _, S, _), w in zip(B, solve_tiles))
+    for step in range(8):
+        for parts in (A, B):
+            for _, _, S, c in parts:
+                c(step)
+        for _, _, S, _ in A:
+            S.substep()
+        fb.step()
+        torch.cuda.synchronize()
+        _assert_same(_state(A), _state(B), f"step {step} variants {variants} tiles {tiles} solve tiles {solve_tiles}")
+
+
 def test_batch_upload_rejects_conflicting_groups():
     """Groups over one state must agree on the variants (the row layout follows the solver variant) and must not
     share envs."""

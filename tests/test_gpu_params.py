@@ -250,3 +250,37 @@ def test_hessian_lds_budget_changes_no_bit():
         out.append((S.st.t["qpos"].clone(), S.st.t["vel"].clone()))
     for q, v in out[1:]:
         assert torch.equal(q, out[0][0]) and torch.equal(v, out[0][1])
+
+
+@pytest.mark.parametrize("name,B,budgets", [
+    # arm: 9 dofs, triangle 45 (48 with padding): 1024 stages 64 rows a pass, 128 stages 8, 118 leaves too little
+    # room (the per-entity row walk)
+    ("arm", 128, (1024, 128, 118)),
+    # quadruped: 18 dofs, triangle 171 (172): 1024 stages 44 rows, 324 stages 8, 305 the row walk
+    ("quadruped", 64, (1024, 324, 305)),
+])
+def test_dense_row_staging_changes_no_bit(name, B, budgets):
+    """Models whose dofs all belong to one entity sum J^T f and J^T D J with lanes over dofs / Hessian entries over
+    rows staged in LDS, a budget-dependent number of rows per pass; the per-entity row walk gives the same bits, over
+    200 controlled steps with many contact rows. (A Hessian kept in global memory takes the chunked factorisation
+    instead of the dense one, so its rounding differs; the parity tests cover it.)"""
+    _need_gpu()
+    w = W.workload_by_name(name, B)
+    M = compile_model(w.entities, w.options)
+    out, max_rows = [], 0
+    for H in budgets:
+        S = RigidSolver(M, B, "cuda", solver_variant=1, wave_h_floats=H)
+        S.set_variants(1, 1, 1)
+        W.init_state(w, M, S)
+        ctrl = W.Control(w, M, S, seed=5)
+        for step in range(200):
+            ctrl(step)
+            S.substep()
+            if H == budgets[0] and step % 10 == 0:
+                max_rows = max(max_rows, int(S.st.t["n_con"][0].max().item()))
+        torch.cuda.synchronize()
+        out.append((S.st.t["qpos"].clone(), S.st.t["vel"].clone()))
+    for H, (q, v) in zip(budgets[1:], out[1:]):
+        assert torch.equal(q, out[0][0]) and torch.equal(v, out[0][1]), f"budget {H}"
+    # the budgets that stage 8 rows a pass make several passes
+    assert max_rows > 8, max_rows
