@@ -6,7 +6,8 @@
 namespace gsr {
 namespace {
 
-__global__ void __launch_bounds__(kWave) k_gjk_resume(gsr_model m, gsr_state s) {
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2, 8)))
+k_gjk_resume(gsr_model m, gsr_state s) {
   extern __shared__ char lds[];
   if ((int)blockIdx.x >= s.n_envs) return;
   const int b = xcd_env(blockIdx.x, s.n_envs);

@@ -636,18 +636,24 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, fl
   for (int ie = w.lane; ie < w.E && w.dense_rows == 0; ie += kWave) {
     const int s0 = L.nes[ie], s1 = L.nee[ie];
     if (s1 <= s0) continue;
-    // two rows per iteration: both rows' loads are issued before either is accumulated (in row order)
+    // two rows per iteration: both rows' loads are issued before either is accumulated (in row order); the next
+    // pair's row indices are loaded meanwhile, and every row slot is loaded at once with the force and entry count
+    // (slots past the entry count are ignored), so a pair costs one round trip to memory
     const int t_end = ptr[ie + 1];
+    const int W = e.m.row_width;
+    int ca_n = ptr[ie] < t_end ? lst[ptr[ie]] : 0, cb_n = ptr[ie] + 1 < t_end ? lst[ptr[ie] + 1] : 0;
     for (int t = ptr[ie]; t < t_end; t += 2) {
       const bool two = t + 1 < t_end;
-      const int ca = lst[t], cb = two ? lst[t + 1] : ca;
+      const int ca = ca_n, cb = two ? cb_n : ca;
+      if (t + 2 < t_end) ca_n = lst[t + 2];
+      if (t + 3 < t_end) cb_n = lst[t + 3];
       // a zero force adds +-0 to sums that start at +0 and so never hold -0: skipping a row changes no bit
       const float fa = e.EF(s.efc_force, ca), fb = two ? e.EF(s.efc_force, cb) : 0.f;
       const int na = e.EI(s.jac_nnz, ca), nb = e.EI(s.jac_nnz, cb);
-      if (na <= kStage && nb <= kStage) {
+      if (W <= kStage) {
         RowStage ra, rb;
-        stage_row(e, ca, fa != 0.f ? na : 0, ra);
-        stage_row(e, cb, fb != 0.f ? nb : 0, rb);
+        stage_row(e, ca, W, ra);
+        stage_row(e, cb, W, rb);
         // a row's entries are distinct dofs: all are read before any is written back
         float old_v[kStage];
 #pragma unroll
@@ -1013,16 +1019,18 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
     const int t_end = ptr[ne + 1];
     int c_next = ptr[ne] < t_end ? lst[ptr[ne]] : 0;
     for (int t = ptr[ne]; t < t_end; t++) {
-      // the next row's index is loaded while this row is summed
+      // the next row's index is loaded while this row is summed; the row's slots are loaded at once with its
+      // activity, entry count and weight (slots past the entry count are ignored)
       const int c = c_next;
       if (t + 1 < t_end) c_next = lst[t + 1];
-      // an inactive row adds +-0 to entries that start at +0 and so never hold -0: skipping it changes no bit
-      if (!(e.EI(s.efc_active, c) & 1)) continue;
-      int nnz = e.EI(s.jac_nnz, c);
-      float Dc = e.EF(s.efc_D, c), act = 1.f;
-      if (nnz <= kStage) {
+      const int active = e.EI(s.efc_active, c) & 1;
+      const int nnz = e.EI(s.jac_nnz, c);
+      const float Dc = e.EF(s.efc_D, c), act = 1.f;
+      if (m.row_width <= kStage) {
         RowStage r;
-        stage_row(e, c, nnz, r);
+        stage_row(e, c, m.row_width, r);
+        // an inactive row adds +-0 to entries that start at +0 and so never hold -0: skipping it changes no bit
+        if (!active) continue;
         // the row's other entity (by position); its dofs are columns of this entity's rows only when it comes first.
         // The pair was stored by the structure pass (one load, beside the row's own, instead of a chain through nnz)
         const int ent = s.efc_ent[(size_t)e.b * m.max_rows + c];
@@ -1065,6 +1073,7 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
         }
         continue;
       }
+      if (!active) continue;
       for (int k1 = 0; k1 < nnz; k1++) {
         float v1 = e.JV(c, k1);
         if (!(fabsf(v1) > m.eps)) continue;

@@ -202,8 +202,9 @@ __device__ __forceinline__ void phase_tile(const gsr_model& m, const gsr_state& 
   }
 }
 
+// two waves per SIMD at least: the collision phases would otherwise take 256 VGPRs plus AGPRs (one wave per SIMD)
 template <int P>
-__global__ void __launch_bounds__(kWave) k_phase(const char* __restrict__ table) {
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2, 8))) k_phase(const char* __restrict__ table) {
   extern __shared__ char lds[];
   const BatchHeader* __restrict__ H = reinterpret_cast<const BatchHeader*>(table);
   ConstTypes* types = (ConstTypes*)(table + kTypesOffset);

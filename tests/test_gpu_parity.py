@@ -257,6 +257,8 @@ def test_step_parity_resynced(case, B, steps, variant, pre):
           f"{B * steps} env-steps, {total_contacts} contacts, {n_conditioned} entries held to the FP32/FP64 spread")
     assert mismatched_contacts == 0
     assert int(S.st.t["errno_env"].max().item()) & 0xF == 0
+    # the relaxed bound is the exception: at most 1 in 10,000 checked entries may need it
+    assert n_conditioned <= 1e-4 * B * steps * (M.n_qs + M.n_dofs), n_conditioned
 
 
 @pytest.mark.parametrize("case", ["arm", "quadruped", "pile8", "pile64"])
