@@ -157,21 +157,29 @@ __global__ void k_links_contact_force(gsr_model m, gsr_state s, const int32_t* e
 // kernel_get_equality_constraints (constraint_solver_decomp.py:2510-2558) for the JOINT equalities this solver
 // builds (one row each, rows 0..n_eq-1 in model order). Every env has the model's n_equalities, so padded and ragged
 // outputs coincide: row b * n_equalities + i holds {type, obj1, obj2} and the row force in column 0 of 6.
+// kernel_get_equality_constraints (constraint_solver_decomp.py:2510-2558), padded: row b * (n_equalities +
+// max_dyn_welds) + i; rows past an env's own count hold type -1 and zero forces
 __global__ void k_get_equalities(gsr_model m, gsr_state s, int32_t* iout, float* fout) {
-  const int B = s.n_envs, E = m.n_equalities;
+  const int B = s.n_envs, E = m.n_equalities, N = E + m.max_dyn_welds;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)B * E) return;
+  if (t >= (int64_t)B * N) return;
   const int i = (int)(t / B), b = (int)(t % B);
   const Env e(m, s, b);
-  const int64_t row = (int64_t)b * E + i;
-  iout[3 * row] = m.eq_type[i];
-  iout[3 * row + 1] = m.eq_obj1[i];
-  iout[3 * row + 2] = m.eq_obj2[i];
-  // equalities of other types are rejected when the model is compiled; their rows would be 3 (CONNECT) or 6 (WELD)
+  const int64_t row = (int64_t)b * N + i;
+  const int n_dyn = m.max_dyn_welds > 0 ? e.I(s.eq_dyn_n, 0) : 0;
+  if (i >= E + n_dyn) {
+    for (int k = 0; k < 3; k++) iout[3 * row + k] = -1;
+    for (int k = 0; k < 6; k++) fout[6 * row + k] = 0.f;
+    return;
+  }
+  auto rows_of = [&](int j) { return j >= E ? 6 : m.eq_type[j] == EQ_CONNECT ? 3 : m.eq_type[j] == EQ_WELD ? 6 : 1; };
+  iout[3 * row] = i < E ? m.eq_type[i] : (int)EQ_WELD;
+  iout[3 * row + 1] = i < E ? m.eq_obj1[i] : e.I(s.eq_dyn_obj, 2 * (i - E));
+  iout[3 * row + 2] = i < E ? m.eq_obj2[i] : e.I(s.eq_dyn_obj, 2 * (i - E) + 1);
   int c = 0;
-  for (int j = 0; j < i; j++) c += m.eq_type[j] == EQ_JOINT;
-  fout[6 * row] = m.eq_type[i] == EQ_JOINT ? e.EF(s.efc_force, c) : 0.f;
-  for (int k = 1; k < 6; k++) fout[6 * row + k] = 0.f;
+  for (int j = 0; j < i; j++) c += rows_of(j);
+  const int nr = rows_of(i);
+  for (int k = 0; k < 6; k++) fout[6 * row + k] = k < nr ? e.EF(s.efc_force, c + k) : 0.f;
 }
 
 int check(const gsr_model* m, const gsr_state* s) {
@@ -416,7 +424,7 @@ int gsr_get_equalities(const gsr_model* model, const gsr_state* state, int32_t* 
                        hipStream_t stream) {
   int rc = check(model, state);
   if (rc) return rc;
-  const int64_t n = (int64_t)state->n_envs * model->n_equalities;
+  const int64_t n = (int64_t)state->n_envs * (model->n_equalities + model->max_dyn_welds);
   if (n == 0) return 0;
   if (!dev_iout || !dev_fout) return GSR_E_ARG;
   hipLaunchKernelGGL(k_get_equalities, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *model, *state,

@@ -25,6 +25,19 @@ struct Env {
   __device__ __forceinline__ int32_t& I(int32_t* a, int i) const { return a[ix(i)]; }
   // per-dof parameter table entry i of this env: shared by every env, or env-minor when the model batches dofs
   __device__ __forceinline__ float DP(const float* a, int i) const { return m.batch_dofs ? a[ix(i)] : a[i]; }
+  // per-link parameter table entry i (link_inertial_mass, link_invweight), per env when the model batches links
+  __device__ __forceinline__ float LP(const float* a, int i) const { return m.batch_links ? a[ix(i)] : a[i]; }
+  // the 7 solver parameters of joint j / equality i: first entry, stride between entries returned in st
+  __device__ __forceinline__ const float* joint_sp(int j, size_t& st) const {
+    st = m.batch_joints ? (size_t)B : 1;
+    return m.joint_sol_params + (m.batch_joints ? ix(7 * j) : (size_t)(7 * j));
+  }
+  __device__ __forceinline__ const float* eq_sp(int i, size_t& st) const {
+    st = m.batch_eqs ? (size_t)B : 1;
+    return m.eq_sol_params + (m.batch_eqs ? ix(7 * i) : (size_t)(7 * i));
+  }
+  // the solver's tolerance scale of this env (rigid_global_info.meaninertia[i_b])
+  __device__ __forceinline__ float meaninertia() const { return m.env_meaninertia ? m.env_meaninertia[b] : m.meaninertia; }
   __device__ __forceinline__ f3 L3(const float* a, int i) const {
     return f3{a[ix(3 * i)], a[ix(3 * i + 1)], a[ix(3 * i + 2)]};
   }

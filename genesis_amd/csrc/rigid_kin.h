@@ -91,7 +91,7 @@ __device__ inline void dev_com_links(const Env& e, int e0 = 0, int es = 1) {
   }
   GSR_FOR_ENT_LINKS(ie, l) {
     // per-env mass and COM shifts (rsd:4812-4817)
-    float mass = m.link_inertial_mass[l] + e.F(s.mass_shift, l);
+    float mass = e.LP(m.link_inertial_mass, l) + e.F(s.mass_shift, l);
     f3 lp = e.L3(s.links_pos, l);
     q4 lq = e.L4(s.links_quat, l);
     f3 ipos_bw = lp + qrot(T3(m.link_inertial_pos, l) + e.L3(s.i_pos_shift, l), lq);
@@ -109,7 +109,7 @@ __device__ inline void dev_com_links(const Env& e, int e0 = 0, int es = 1) {
   }
   GSR_FOR_ENT_LINKS(ie, l) {
     f3 t = e.L3(s.crb_pos, l) - e.L3(s.root_com, l);  // i_pos
-    float mass = m.link_inertial_mass[l] + e.F(s.mass_shift, l);  // rsd:4910
+    float mass = e.LP(m.link_inertial_mass, l) + e.F(s.mass_shift, l);  // rsd:4910
     float x = t.x, y = t.y, z = t.z;
     float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
     float hhT[9] = {yy + zz, -xy, -xz, -xy, xx + zz, -yz, -xz, -yz, xx + yy};
@@ -301,7 +301,7 @@ __device__ inline void dev_smooth_dynamics(const Env& e, int e0 = 0, int es = 1)
     for (int i = st; i < en; i++)
       for (int j = i + 1; j < en; j++) e.Mm(ie, i, j) = e.Mm(ie, j, i);
     // armature, then the implicit damping terms of approximate_implicitfast
-    for (int d = st; d < en; d++) e.Mm(ie, d, d) = e.Mm(ie, d, d) + m.dof_armature[d];
+    for (int d = st; d < en; d++) e.Mm(ie, d, d) = e.Mm(ie, d, d) + e.DP(m.dof_armature, d);
     for (int d = st; d < en; d++) {
       float v = e.Mm(ie, d, d) + e.DP(m.dof_damping, d) * dt;
       int cm = e.I(s.ctrl_mode, d);

@@ -184,9 +184,11 @@ __device__ __forceinline__ void orthogonals(f3 a, f3& b, f3& c) {
   b = normalized(bb);
   c = cross(a, b);
 }
-// constraint impedance and reference acceleration (geom.py:377-398)
-__device__ __forceinline__ void imp_aref(const float* p, float neg_pen, float vel, float pos, float& imp, float& aref) {
-  float timeconst = p[0], dampratio = p[1], dmin = p[2], dmax = p[3], width = p[4], mid = p[5], power = p[6];
+// constraint impedance and reference acceleration (geom.py:377-398); the 7 solver parameters at p[k * st]
+__device__ __forceinline__ void imp_aref(const float* p, size_t st, float neg_pen, float vel, float pos, float& imp,
+                                         float& aref) {
+  float timeconst = p[0], dampratio = p[st], dmin = p[2 * st], dmax = p[3 * st], width = p[4 * st], mid = p[5 * st],
+        power = p[6 * st];
   float imp_x = fabsf(neg_pen) / width;
   float imp_a = (1.f / crpow(mid, power - 1.f)) * crpow(imp_x, power);
   float imp_b = 1.f - (1.f / crpow(1.f - mid, power - 1.f)) * crpow(1.f - imp_x, power);
@@ -197,6 +199,9 @@ __device__ __forceinline__ void imp_aref(const float* p, float neg_pen, float ve
   float bb = 2.f / (dmax * timeconst);
   float k = 1.f / (dmax * dmax * timeconst * timeconst * dampratio * dampratio);
   aref = -bb * vel - k * imp * pos;
+}
+__device__ __forceinline__ void imp_aref(const float* p, float neg_pen, float vel, float pos, float& imp, float& aref) {
+  imp_aref(p, 1, neg_pen, vel, pos, imp, aref);
 }
 
 }  // namespace gsr

@@ -55,20 +55,152 @@ __device__ inline void row_sort(RowBuf& r) {
   }
 }
 
+// Row of component i3 of the velocity difference of two points, p1 fixed to link1 (sign +1) and p2 fixed to link2
+// (sign -1): the dofs of both root chains, link by link from the link up, each link's dofs last first, as
+// func_equality_connect walks them (constraint_solver_decomp.py:586-616); returns J qvel summed in that order.
+__device__ inline float point_diff_row(const Env& e, int l1, f3 p1, int l2, f3 p2, int i3, RowBuf& r) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  r.n = 0;
+  float jac_qvel = 0.f;
+  for (int iab = 0; iab < 2; iab++) {
+    const float sign = iab == 0 ? 1.f : -1.f;
+    int link = iab == 0 ? l1 : l2;
+    const f3 p = iab == 0 ? p1 : p2;
+    while (link > -1) {
+      const int de = m.link_dof_end[link], nd = de - m.link_dof_start[link];
+      const f3 tpos = p - e.L3(s.root_com, link);
+      for (int k = 0; k < nd; k++) {
+        const int d = de - 1 - k;
+        const f3 vel = qrot(e.L3(s.cdof_vel, d) - cross(tpos, e.L3(s.cdof_ang, d)), qid());
+        const float jac = comp(sign * vel, i3);
+        jac_qvel = jac_qvel + jac * e.F(s.vel, d);
+        row_add(r, d, jac);
+      }
+      link = m.link_parent[link];
+    }
+  }
+  row_sort(r);
+  return jac_qvel;
+}
+
+// stores row c with its impedance terms: imp_aref(sol_params, -pen, J qvel, pos), diag = invweight (1 - imp) / imp
+__device__ inline void eq_row_finish(const Env& e, int c, const RowBuf& r, const float* sp, size_t sps, float pen,
+                                     float jac_qvel, float pos, float invw) {
+  const gsr_state& s = e.s;
+  row_store(e, c, r);
+  float imp, aref;
+  imp_aref(sp, sps, -pen, jac_qvel, pos, imp, aref);
+  const float diag = fmaxf(invw * (1.f - imp) / imp, e.m.eps);
+  e.EF(s.efc_diag, c) = diag;
+  e.EF(s.efc_aref, c) = aref;
+  e.EF(s.efc_D, c) = 1.f / diag;
+}
+
+// CONNECT (func_equality_connect, constraint_solver_decomp.py:528-641): anchor data[0:3] on link1 and data[3:6] on
+// link2 coincide; 3 rows from row c, returns the next row (solver parameters sp[k * sps])
+__device__ inline int dev_connect_rows(const Env& e, int c, int l1, int l2, const float* data, const float* sp,
+                                       size_t sps) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const f3 ga1 = xform(mk3(data[0], data[1], data[2]), e.L3(s.links_pos, l1), e.L4(s.links_quat, l1));
+  const f3 ga2 = xform(mk3(data[3], data[4], data[5]), e.L3(s.links_pos, l2), e.L4(s.links_quat, l2));
+  const float invw = e.LP(m.link_invweight, 2 * l1) + e.LP(m.link_invweight, 2 * l2);
+  for (int i3 = 0; i3 < 3; i3++) {
+    RowBuf r;
+    const float jq = point_diff_row(e, l1, ga1, l2, ga2, i3, r);
+    const f3 pd = ga1 - ga2;
+    eq_row_finish(e, c + i3, r, sp, sps, norm(pd), jq, comp(pd, i3), invw);
+  }
+  return c + 3;
+}
+
+// q * (0, v): ti_quat_mul_axis (genesis/utils/geom.py:195-204)
+__device__ __forceinline__ q4 quat_mul_axis(q4 q, f3 v) {
+  return q4{-q.x * v.x - q.y * v.y - q.z * v.z, +q.w * v.x + q.y * v.z - q.z * v.y,
+            +q.w * v.y + q.z * v.x - q.x * v.z, +q.w * v.z + q.x * v.y - q.y * v.x};
+}
+
+// angular Jacobian difference of the two chains at dof d (a dof of either chain): link1's chain adds cdof_ang, link2's
+// subtracts it, summed from zero in that order (a dof on both chains cancels)
+__device__ inline f3 weld_ang(const Env& e, int l1, int l2, int d) {
+  const gsr_model& m = e.m;
+  f3 a = zero3();
+  for (int iab = 0; iab < 2; iab++)
+    for (int link = iab == 0 ? l1 : l2; link > -1; link = m.link_parent[link])
+      if (d >= m.link_dof_start[link] && d < m.link_dof_end[link]) {
+        a = a + (iab == 0 ? 1.f : -1.f) * e.L3(e.s.cdof_ang, d);
+        break;
+      }
+  return a;
+}
+
+// WELD (func_equality_weld, constraint_solver_decomp.py:824-1015): data[0:3] anchor on link2, data[3:6] anchor on
+// link1, data[6:10] pose of link2 relative to link1, data[10] torque scale; 3 position rows then 3 orientation rows
+// from row c, returns the next row (solver parameters sp[k * sps])
+__device__ inline int dev_weld_rows(const Env& e, int c, int l1, int l2, const float* data, const float* sp,
+                                    size_t sps) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  const q4 q1 = e.L4(s.links_quat, l1), q2 = e.L4(s.links_quat, l2);
+  const q4 rel = q4{data[6], data[7], data[8], data[9]};
+  const float ts = data[10];
+  const f3 ga1 = xform(mk3(data[3], data[4], data[5]), e.L3(s.links_pos, l1), q1);
+  const f3 ga2 = xform(mk3(data[0], data[1], data[2]), e.L3(s.links_pos, l2), q2);
+  const f3 pe = ga1 - ga2;
+  const q4 q = qmul(q1, rel);
+  const q4 iq2 = qinv(q2);
+  const q4 eq = qmul(iq2, q);
+  const f3 re = mk3(eq.x, eq.y, eq.z) * ts;
+  // norm of the 6-vector (position error, orientation error)
+  const float pos_imp = sqrtf(pe.x * pe.x + pe.y * pe.y + pe.z * pe.z + re.x * re.x + re.y * re.y + re.z * re.z);
+  const float invw0 = e.LP(m.link_invweight, 2 * l1) + e.LP(m.link_invweight, 2 * l2);
+  const float invw1 = e.LP(m.link_invweight, 2 * l1 + 1) + e.LP(m.link_invweight, 2 * l2 + 1);
+  RowBuf r;
+  for (int i = 0; i < 3; i++) {
+    const float jq = point_diff_row(e, l1, ga1, l2, ga2, i, r);
+    eq_row_finish(e, c + i, r, sp, sps, pos_imp, jq, comp(pe, i), invw0);
+  }
+  // orientation rows over the same sorted dofs (the dofs of both chains, which r holds now): the angular Jacobian
+  // difference of the two chains rotated by inv(q2) ... q, one row at a time (the per-dof quaternion products are
+  // recomputed per row: the same values)
+  for (int i = 0; i < 3; i++) {
+    float jq = 0.f;
+    for (int k = 0; k < r.n; k++) {
+      const int d = r.idx[k];
+      const q4 q3 = qmul(quat_mul_axis(iq2, weld_ang(e, l1, l2, d)), q);
+      const float jac = 0.5f * (i == 0 ? q3.x : i == 1 ? q3.y : q3.z) * ts;
+      r.val[k] = jac;
+      jq = jq + jac * e.F(s.vel, d);
+    }
+    eq_row_finish(e, c + 3 + i, r, sp, sps, pos_imp, jq, comp(re, i), invw1);
+  }
+  return c + 6;
+}
+
+// equality rows (add_equality_constraints, constraint_solver_decomp.py:778-822): the model's equalities in order,
+// then the env's own welds (their solver parameters as kernel_add_weld_constraint sets them, :2475-2477)
 __device__ inline void dev_equality_rows(const Env& e) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
-  int n_con = 0, n_eq = 0;
+  int n_con = 0;
   for (int ie = 0; ie < m.n_equalities; ie++) {
-    if (m.eq_type[ie] != EQ_JOINT) continue;
-    const float* sp = m.eq_sol_params + 7 * ie;
+    size_t sps;
+    const float* sp = e.eq_sp(ie, sps);
+    const float* data = m.eq_data + 11 * ie;
+    if (m.eq_type[ie] == EQ_CONNECT) {
+      n_con = dev_connect_rows(e, n_con, m.eq_obj1[ie], m.eq_obj2[ie], data, sp, sps);
+      continue;
+    }
+    if (m.eq_type[ie] == EQ_WELD) {
+      n_con = dev_weld_rows(e, n_con, m.eq_obj1[ie], m.eq_obj2[ie], data, sp, sps);
+      continue;
+    }
     int j1 = m.eq_obj1[ie], j2 = m.eq_obj2[ie];
     int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2], d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
     int c = n_con++;
-    n_eq++;
     float pos1 = e.F(s.qpos, q1), pos2 = e.F(s.qpos, q2);
     float diff = pos2 - m.qpos0[q2], pos = pos1 - m.qpos0[q1], deriv = 0.f;
-    const float* data = m.eq_data + 11 * ie;
     for (int i5 = 0; i5 < 5; i5++) {
       float dp = crpow(diff, (float)i5);
       pos = pos - dp * data[i5];
@@ -84,16 +216,23 @@ __device__ inline void dev_equality_rows(const Env& e) {
     r.n = 2;
     row_store(e, c, r);
     float jac_qvel = 1.f * e.F(s.vel, d1) + -deriv * e.F(s.vel, d2);
-    float invw = m.dof_invweight[d1] + m.dof_invweight[d2];
+    float invw = e.DP(m.dof_invweight, d1) + e.DP(m.dof_invweight, d2);
     float imp, aref;
-    imp_aref(sp, -fabsf(pos), jac_qvel, pos, imp, aref);
+    imp_aref(sp, sps, -fabsf(pos), jac_qvel, pos, imp, aref);
     float diag = fmaxf(invw * (1.f - imp) / imp, m.eps);
     e.EF(s.efc_diag, c) = diag;
     e.EF(s.efc_aref, c) = aref;
     e.EF(s.efc_D, c) = 1.f / diag;
   }
+  const int n_dyn = m.max_dyn_welds > 0 ? e.I(s.eq_dyn_n, 0) : 0;
+  for (int k = 0; k < n_dyn; k++) {
+    float data[11];
+    for (int j = 0; j < 11; j++) data[j] = e.F(s.eq_dyn_data, 11 * k + j);
+    const float sp[7] = {2.f * m.dt, 1.f, 0.9f, 0.95f, 0.001f, 0.5f, 2.f};
+    n_con = dev_weld_rows(e, n_con, e.I(s.eq_dyn_obj, 2 * k), e.I(s.eq_dyn_obj, 2 * k + 1), data, sp, 1);
+  }
   e.I(s.n_con, 0) = n_con;
-  e.I(s.n_con, 1) = n_eq;
+  e.I(s.n_con, 1) = n_con;
 }
 
 // frictionloss rows (add_frictionloss_constraints), appended from row n_con; returns the new row count
@@ -106,8 +245,9 @@ __device__ inline int dev_frictionloss_rows(const Env& e, int n_con, int& n_fl) 
       for (int d = m.joint_dof_start[j]; d < m.joint_dof_end[j]; d++) {
         if (e.DP(m.dof_frictionloss, d) > m.eps) {
           float imp, aref;
-          imp_aref(m.joint_sol_params + 7 * j, 0.f, 1.f * e.F(s.vel, d), 0.f, imp, aref);
-          float diag = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
+          size_t sps;
+          imp_aref(e.joint_sp(j, sps), sps, 0.f, 1.f * e.F(s.vel, d), 0.f, imp, aref);
+          float diag = fmaxf(e.DP(m.dof_invweight, d) * (1.f - imp) / imp, m.eps);
           int c = n_con++;
           n_fl++;
           e.EF(s.efc_diag, c) = diag;
@@ -134,8 +274,8 @@ __device__ inline void dev_contact_rows(const Env& e, int ic, int c0) {
   for (int k = 0; k < 7; k++) sp[k] = 0.5f * (m.geom_sol_params[7 * ga + k] + m.geom_sol_params[7 * gb + k]);
   f3 d1, d2;
   orthogonals(cn, d1, d2);
-  float invw = m.link_invweight[2 * la];
-  if (lb > -1) invw = invw + m.link_invweight[2 * lb];
+  float invw = e.LP(m.link_invweight, 2 * la);
+  if (lb > -1) invw = invw + e.LP(m.link_invweight, 2 * lb);
   for (int i = 0; i < 4; i++) {
     f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
     f3 n = dd * fr - cn;
@@ -187,8 +327,9 @@ __device__ inline int dev_limit_rows(const Env& e, int n_con) {
       if (pd < 0.f) {
         float jac = (dmin < dmax) ? 1.f : -1.f;
         float imp, aref;
-        imp_aref(m.joint_sol_params + 7 * j, pd, jac * e.F(s.vel, d), pd, imp, aref);
-        float diag = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
+        size_t sps;
+        imp_aref(e.joint_sp(j, sps), sps, pd, jac * e.F(s.vel, d), pd, imp, aref);
+        float diag = fmaxf(e.DP(m.dof_invweight, d) * (1.f - imp) / imp, m.eps);
         int c = n_con++;
         e.EF(s.efc_diag, c) = diag;
         e.EF(s.efc_aref, c) = aref;
@@ -445,7 +586,7 @@ __device__ inline float dev_linesearch(const Env& e, SolverScalars& sc) {
     snorm += v * v;
   }
   snorm = sqrtf(snorm);
-  float scale = m.meaninertia * (float)(D > 1 ? D : 1);
+  float scale = e.meaninertia() * (float)(D > 1 ? D : 1);
   float gtol = m.tolerance * m.ls_tolerance * snorm * scale;
   sc.ls_it = 0;
   if (snorm < m.eps) return 0.f;
@@ -567,7 +708,7 @@ __device__ inline void dev_solve(const Env& e) {
         dev_update_constraint(e, sc);
         dev_hessian_incremental(e);
         dev_update_gradient(e);
-        float tol_scaled = (m.meaninertia * (float)(D > 1 ? D : 1)) * m.tolerance;
+        float tol_scaled = (e.meaninertia() * (float)(D > 1 ? D : 1)) * m.tolerance;
         float improvement = sc.prev_cost - sc.cost;
         float gn = 0.f;
         for (int d = 0; d < D; d++) {

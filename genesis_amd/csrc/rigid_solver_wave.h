@@ -1414,7 +1414,7 @@ __device__ inline float wave_linesearch(const WaveCtx& w, float gauss) {
   double sn = 0.0;
   for (int d = w.lane; d < w.D; d += kWave) sn += (double)(L.search[d] * L.search[d]);
   float snorm = sqrtf((float)wave_sum_d(sn));
-  float scale = m.meaninertia * (float)(w.D > 1 ? w.D : 1);
+  float scale = e.meaninertia() * (float)(w.D > 1 ? w.D : 1);
   float gtol = m.tolerance * m.ls_tolerance * snorm * scale;
   if (snorm < m.eps) return 0.f;
   wave_mass_mul(w, L.search, L.mv);
@@ -1574,7 +1574,7 @@ __device__ __forceinline__ int wave_newton(const WaveCtx& w, const HS& h, double
         }
       }
       double g2 = wave_update_gradient(w, h);
-      float tol_scaled = (m.meaninertia * (float)(D > 1 ? D : 1)) * m.tolerance;
+      float tol_scaled = (w.e.meaninertia() * (float)(D > 1 ? D : 1)) * m.tolerance;
       double improvement = prev_cost - cost;
       float gn = sqrtf((float)wave_sum_d(g2));
       if (gn < tol_scaled || improvement < tol_scaled) {

@@ -24,7 +24,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from .model import (
-    EQ_JOINT, GEOM_BOX, GEOM_CAPSULE, GEOM_MESH, GEOM_SPHERE, JOINT_FIXED, JOINT_FREE, JOINT_PRISMATIC, JOINT_REVOLUTE,
+    EQ_CONNECT, EQ_JOINT, EQ_WELD, GEOM_BOX, GEOM_CAPSULE, GEOM_MESH, GEOM_SPHERE, JOINT_FIXED, JOINT_FREE, JOINT_PRISMATIC, JOINT_REVOLUTE,
     JOINT_SPHERICAL, EntitySpec, EqualitySpec, GeomSpec, JointSpec, LinkSpec, R_to_quat, quat_mul, quat_to_R,
     rpy_to_quat,
 )
@@ -364,6 +364,12 @@ def parse_mjcf(path: str, mesh_to_capsule: bool = False) -> EntitySpec:
     wb = root.find("worldbody")
     for b in wb.findall("body"):
         add_body(b, -1, "main")
+    eq = root.find("equality")
+    if eq is not None and all(l.name != "world" for l in links) and any(
+            e.tag in ("connect", "weld") and (e.get("body2") or "world") == "world" for e in eq):
+        # a CONNECT / WELD to the world: a fixed, massless link at the origin stands for the world frame
+        links.append(LinkSpec("world", -1, np.zeros(3), np.array([1.0, 0.0, 0.0, 0.0]),
+                              [JointSpec("world_joint", JOINT_FIXED)], [], inertial_mass=0.0))
     # re-order by depth like the URDF path
     depth = []
     for i, l in enumerate(links):
@@ -412,15 +418,38 @@ def parse_mjcf(path: str, mesh_to_capsule: bool = False) -> EntitySpec:
     eqs = []
     eq = root.find("equality")
     if eq is not None:
-        for je in eq.findall("joint"):
-            solref = _floats(je.get("solref"), 2, [0.02, 1.0])
-            solimp = _floats(je.get("solimp"), None, [0.9, 0.95, 0.001, 0.5, 2.0])
+        def sol_params(e):
+            solref = _floats(e.get("solref"), 2, [0.02, 1.0])
+            solimp = _floats(e.get("solimp"), None, [0.9, 0.95, 0.001, 0.5, 2.0])
             solimp = np.concatenate([solimp, [0.9, 0.95, 0.001, 0.5, 2.0][len(solimp):]])
-            poly = _floats(je.get("polycoef"), None, [0.0, 1.0, 0.0, 0.0, 0.0])
-            data = np.zeros(11)
-            data[:len(poly)] = poly
-            eqs.append(EqualitySpec(EQ_JOINT, je.get("joint1"), je.get("joint2"), data=data,
-                                    sol_params=np.concatenate([solref, solimp])))
+            return np.concatenate([solref, solimp])
+
+        # in document order, like MuJoCo's equality list
+        for e in eq:
+            if e.tag == "joint":
+                poly = _floats(e.get("polycoef"), None, [0.0, 1.0, 0.0, 0.0, 0.0])
+                data = np.zeros(11)
+                data[:len(poly)] = poly
+                eqs.append(EqualitySpec(EQ_JOINT, e.get("joint1"), e.get("joint2"), data=data,
+                                        sol_params=sol_params(e)))
+            elif e.tag == "connect":
+                # anchor in body1's frame; its body2-frame position comes from the build pose
+                anchor = _floats(e.get("anchor"), 3, [0.0, 0.0, 0.0])
+                eqs.append(EqualitySpec(EQ_CONNECT, e.get("body1"), e.get("body2") or "world", data=anchor,
+                                        sol_params=sol_params(e)))
+            elif e.tag == "weld":
+                # anchor: the weld point on body2. relpose: body2's pose in body1's frame; its all-zero quaternion
+                # (the default) means the build pose's, otherwise body1's anchor is the point relpose puts there
+                rel = _floats(e.get("relpose"), 7, [0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+                anchor = _floats(e.get("anchor"), 3, [0.0, 0.0, 0.0])
+                ts = float(e.get("torquescale", 1.0))
+                b1, b2 = e.get("body1"), e.get("body2") or "world"
+                if np.any(rel[3:] != 0):
+                    q = rel[3:] / np.linalg.norm(rel[3:])
+                    data = np.concatenate([anchor, rel[:3] + quat_to_R(q) @ anchor, q, [ts]])
+                    eqs.append(EqualitySpec(EQ_WELD, b1, b2, data=data, sol_params=sol_params(e), explicit=True))
+                else:
+                    eqs.append(EqualitySpec(EQ_WELD, b1, b2, sol_params=sol_params(e), anchor2=anchor, torquescale=ts))
     return EntitySpec(new_links, eqs, name=os.path.splitext(os.path.basename(path))[0], local_collision_mask=True)
 
 
@@ -442,7 +471,8 @@ def entity_to_json(ent: EntitySpec) -> dict:
                     geoms=[dict(type=g.type, pos=arr(g.pos), quat=arr(g.quat), data=list(map(float, g.data)),
                                 friction=g.friction, sol_params=arr(g.sol_params), contype=g.contype,
                                 conaffinity=g.conaffinity, verts=arr(g.verts)) for g in l.geoms]) for l in ent.links],
-        equalities=[dict(type=e.type, obj1=e.obj1, obj2=e.obj2, data=arr(e.data), sol_params=arr(e.sol_params))
+        equalities=[dict(type=e.type, obj1=e.obj1, obj2=e.obj2, data=arr(e.data), sol_params=arr(e.sol_params),
+                         explicit=e.explicit, anchor2=arr(e.anchor2), torquescale=e.torquescale)
                     for e in ent.equalities],
     )
 
@@ -460,5 +490,6 @@ def entity_from_json(d: dict) -> EntitySpec:
                        g["contype"], g["conaffinity"], a(g.get("verts"))) for g in l["geoms"]]
         links.append(LinkSpec(l["name"], l["parent"], a(l["pos"]), a(l["quat"]), js, gs, a(l["inertial_pos"]),
                               a(l["inertial_quat"]), a(l["inertial_i"]), l["inertial_mass"]))
-    eqs = [EqualitySpec(e["type"], e["obj1"], e["obj2"], a(e["data"]), a(e["sol_params"])) for e in d["equalities"]]
+    eqs = [EqualitySpec(e["type"], e["obj1"], e["obj2"], a(e["data"]), a(e["sol_params"]), e.get("explicit", False),
+                        a(e.get("anchor2")), e.get("torquescale", 1.0)) for e in d["equalities"]]
     return EntitySpec(links, eqs, d["rho"], d["friction"], d["name"], d["local_collision_mask"])

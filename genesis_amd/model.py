@@ -162,11 +162,20 @@ class LinkSpec:
 
 @dataclass
 class EqualitySpec:
+    """JOINT: obj1 / obj2 joint names, data = the coupling polynomial's 5 coefficients. CONNECT: link names, data[0:3]
+    = the anchor in obj1's frame (its obj2-frame position is taken from the build pose). WELD: link names; the
+    anchors and relative pose come from the build pose unless explicit=True, when data holds all 11 values
+    (anchor on obj2, anchor on obj1, relative quaternion, torque scale). A build-pose weld is welded at obj1's origin
+    (add_weld_constraint's choice), or at anchor2 on obj2 when given (MJCF's weld anchor), with torque scale
+    torquescale."""
     type: int
     obj1: str
     obj2: str
     data: np.ndarray = field(default_factory=lambda: np.array([0.0, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0]))
     sol_params: Optional[np.ndarray] = None
+    explicit: bool = False
+    anchor2: Optional[np.ndarray] = None
+    torquescale: float = 1.0
 
 
 @dataclass
@@ -293,7 +302,9 @@ class CompileOptions:
     enable_multi_contact: bool = True
     max_dynamic_constraints: int = 8
     n_contacts_per_pair: int = 5
-    batch_dofs_info: bool = False  # per-env kp, kv, force range, stiffness, damping, frictionloss, limits
+    batch_dofs_info: bool = False  # per-env kp, kv, force range, stiffness, damping, frictionloss, limits, armature
+    batch_links_info: bool = False  # per-env link masses and link inverse weights
+    batch_joints_info: bool = False  # per-env joint solver parameters
 
 
 def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel:
@@ -422,16 +433,33 @@ def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel
                               inertial_pos=ipos, inertial_quat=iquat, inertial_i=iI, inertial_mass=imass,
                               is_fixed=is_fixed, entity=i_e, name=ls.name))
         for eq in ent.equalities:
-            if eq.type != EQ_JOINT:
-                raise NotImplementedError("only JOINT equalities are built (CONNECT and WELD rows are not)")
             sp = eq.sol_params if eq.sol_params is not None else default_solver_params()
-            eqs.append(dict(type=eq.type, obj1=name_to_joint[eq.obj1], obj2=name_to_joint[eq.obj2],
-                            data=np.pad(np.asarray(eq.data, np.float64), (0, 11 - len(eq.data))),
-                            sol_params=sanitize_sol_params(sp, min_tc, default_tc)[0]))
+            sp = sanitize_sol_params(sp, min_tc, default_tc)[0]
+            if eq.type == EQ_JOINT:
+                eqs.append(dict(type=eq.type, obj1=name_to_joint[eq.obj1], obj2=name_to_joint[eq.obj2],
+                                data=np.pad(np.asarray(eq.data, np.float64), (0, 11 - len(eq.data))), sol_params=sp))
+            elif eq.type in (EQ_CONNECT, EQ_WELD):
+                # link names resolve once every entity is built: this entity's links first, then any entity's
+                eqs.append(dict(type=eq.type, obj1=eq.obj1, obj2=eq.obj2, link_start=link_start, spec=eq,
+                                data=None, sol_params=sp))
+            else:
+                raise ValueError(f"unknown equality type {eq.type}")
         ent_rows.append(dict(link_start=link_start, link_end=len(links), dof_start=dof_start_e, dof_end=n_d,
                              geom_start=geom_start_e, geom_end=len(geoms)))
 
     L, J, D, Q, G, E = len(links), len(joints), n_d, n_q, len(geoms), len(entities)
+    for e in eqs:
+        if e["type"] == EQ_JOINT:
+            continue
+        local = {links[i]["name"]: i for i in range(e["link_start"], L)}
+        glob = {}
+        for i in range(L):
+            glob.setdefault(links[i]["name"], i)
+        for k in ("obj1", "obj2"):
+            name = e[k]
+            if name not in local and name not in glob:
+                raise KeyError(f"equality link {name!r} not found")
+            e[k] = local.get(name, glob.get(name))
     M.n_links, M.n_joints, M.n_dofs, M.n_qs, M.n_geoms, M.n_entities = L, J, D, Q, G, E
     M.n_equalities = len(eqs)
     f32, i32 = np.float32, np.int32
@@ -537,8 +565,8 @@ def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel
     M.eq_type = np.array([e["type"] for e in eqs], i32)
     M.eq_obj1 = np.array([e["obj1"] for e in eqs], i32)
     M.eq_obj2 = np.array([e["obj2"] for e in eqs], i32)
-    M.eq_data = np.array([e["data"] for e in eqs], f32).reshape(len(eqs), 11)
     M.eq_sol_params = np.array([e["sol_params"] for e in eqs], f32).reshape(len(eqs), 7)
+    M.max_dyn_welds = int(opt.max_dynamic_constraints)
 
     # mass parent mask (rigid_solver_decomp.py:512-523)
     mask = np.zeros((D, D), f32)
@@ -595,10 +623,45 @@ def compile_model(entities: List[EntitySpec], opt: CompileOptions) -> RigidModel
         for b in range(a + 1, G):
             if pair[a, b] >= 0:
                 width = max(width, len(chain_dofs[M.geom_link[a]] | chain_dofs[M.geom_link[b]]))
+    # equality rows span the union of their two links' chains; welds registered after the build may join any two
+    for e in eqs:
+        if e["type"] != EQ_JOINT:
+            width = max(width, len(chain_dofs[e["obj1"]] | chain_dofs[e["obj2"]]))
+    if M.max_dyn_welds > 0:
+        for a in range(L):
+            for b in range(a + 1, L):
+                width = max(width, len(chain_dofs[a] | chain_dofs[b]))
     M.row_width = width
     M.max_entity_dofs = int(max([e["dof_end"] - e["dof_start"] for e in ent_rows] + [1]))
 
     M.gravity = np.array(opt.gravity, f32)
+    # CONNECT / WELD data from the build pose when the spec gives none (MJCF's compiled anchors, and the weld's
+    # relative pose as kernel_add_weld_constraint takes it, constraint_solver_decomp.py:2452-2473)
+    lpos = lquat = None
+    for e in eqs:
+        if e["type"] == EQ_JOINT:
+            continue
+        if e["spec"].data is not None and len(e["spec"].data) == 11 and e["type"] == EQ_WELD and e["spec"].explicit:
+            e["data"] = np.asarray(e["spec"].data, np.float64)
+            continue
+        if lpos is None:
+            lpos, lquat = _host_fk_crba(M, M.qpos0.astype(np.float64))[:2]
+        l1, l2 = e["obj1"], e["obj2"]
+        inv = lambda p, t, q: transform_by_quat(p - t, np.array([q[0], -q[1], -q[2], -q[3]]))  # noqa: E731
+        d = np.zeros(11)
+        if e["type"] == EQ_CONNECT:
+            a1 = np.asarray(e["spec"].data[:3], np.float64)
+            d[0:3] = a1
+            d[3:6] = inv(lpos[l1] + transform_by_quat(a1, lquat[l1]), lpos[l2], lquat[l2])
+        else:
+            a2 = e["spec"].anchor2
+            shared = lpos[l1] if a2 is None else lpos[l2] + transform_by_quat(np.asarray(a2, np.float64), lquat[l2])
+            d[3:6] = inv(shared, lpos[l1], lquat[l1])
+            d[0:3] = inv(shared, lpos[l2], lquat[l2]) if a2 is None else a2
+            d[6:10] = quat_mul(np.array([lquat[l1][0], -lquat[l1][1], -lquat[l1][2], -lquat[l1][3]]), lquat[l2])
+            d[10] = e["spec"].torquescale
+        e["data"] = d
+    M.eq_data = np.array([e["data"] for e in eqs], f32).reshape(len(eqs), 11)
     _init_invweight_and_meaninertia(M)
     return M
 

@@ -52,7 +52,8 @@ MODEL_TABLES_F = [
 
 INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", "n_broad", "broad_pairs",
              "n_contacts", "contact_geoms", "active_buffer", "n_con", "jac_idx", "jac_nnz", "efc_active",
-             "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_defer_n", "gjk_defer"}
+             "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_defer_n", "gjk_defer", "eq_dyn_n",
+             "eq_dyn_obj"}
 
 # Entity counts above this cannot use the wavefront solver's pattern bitsets.
 WAVE_MAX_ENTITIES = 128
@@ -93,6 +94,7 @@ def state_sizes(M: RigidModel) -> dict:
         ctrl_force=D, dofs_pos=D, links_pos=3 * L, links_quat=4 * L, geoms_pos=3 * G, geoms_quat=4 * G,
         aabb_min=3 * G, aabb_max=3 * G, cache_normal=3 * P, sort_value=2 * G, sort_ig=2 * G, first_time=1,
         errno_env=1, mass_shift=L, i_pos_shift=3 * L, friction_ratio=G,
+        eq_dyn_n=1, eq_dyn_obj=2 * M.max_dyn_welds, eq_dyn_data=11 * M.max_dyn_welds,
         cdof_ang=3 * D, cdof_vel=3 * D, cdofd_ang=3 * D, cdofd_vel=3 * D, root_com=3 * L, cd_vel=3 * L,
         cd_ang=3 * L, cinr_inertial=9 * L, cinr_pos=3 * L, cinr_mass=L, links_iquat=4 * L,
         mass_blk=M.mass_blk_size, mass_L_blk=M.mass_blk_size, mass_D_inv=D, qf_smooth=D, force=D,
@@ -131,9 +133,13 @@ def wave_h_floats_default(M: RigidModel) -> int:
     diag = int((n * (n + 1) // 2).sum())
     return int(min(WAVE_H_FLOATS, max(1024, -(-4 * diag // 256) * 256)))
 
-# per-dof tables that batch_dofs_info makes per-env (env-minor, like the state)
+# per-dof / per-link / per-joint tables that batch_dofs_info / batch_links_info / batch_joints_info make per-env
+# (env-minor, like the state; options/solvers.py:373-375); equality solver parameters are always per env
 BATCHED_DOF_TABLES = ("dof_kp", "dof_kv", "dof_force_range", "dof_stiffness", "dof_damping", "dof_frictionloss",
-                      "dof_limit")
+                      "dof_limit", "dof_armature", "dof_invweight")
+BATCHED_LINK_TABLES = ("link_inertial_mass", "link_invweight")
+BATCHED_JOINT_TABLES = ("joint_sol_params",)
+BATCHED_EQ_TABLES = ("eq_sol_params",)
 
 
 class DeviceModel:
@@ -150,17 +156,24 @@ class DeviceModel:
             self.tensors[k] = torch.from_numpy(a if a.size else np.zeros(1, np.int32)).to(self.device)
         o = M.options
         self.batch_dofs = bool(getattr(o, "batch_dofs_info", False))
+        self.batch_links = bool(getattr(o, "batch_links_info", False))
+        self.batch_joints = bool(getattr(o, "batch_joints_info", False))
         self.B = max(1, int(n_envs or 0))
         for k in MODEL_TABLES_F:
             a = np.ascontiguousarray(np.asarray(getattr(M, k)), dtype=np.float32).ravel()
-            if self.batch_dofs and k in BATCHED_DOF_TABLES:
+            if self.batched(k):
                 a = np.ascontiguousarray(np.repeat(a[:, None], self.B, axis=1)).ravel()  # (n, B) env-minor
             self.tensors[k] = torch.from_numpy(a if a.size else np.zeros(1, np.float32)).to(self.device)
+        # per-env armature or link masses give each env its own mean inertia (rigid_global_info.meaninertia[i_b])
+        self.env_meaninertia = None
+        if self.batch_dofs or self.batch_links:
+            self.env_meaninertia = torch.full((self.B,), float(M.meaninertia), dtype=torch.float32, device=self.device)
         d = _lib.GsrModel()
         d.n_links, d.n_joints, d.n_dofs, d.n_qs = M.n_links, M.n_joints, M.n_dofs, M.n_qs
         d.n_geoms, d.n_entities, d.n_equalities, d.n_verts, d.n_pairs = (M.n_geoms, M.n_entities, M.n_equalities,
                                                                          M.n_verts, M.n_pairs)
         d.max_contacts, d.max_broad, d.max_rows, d.row_width = M.max_contacts, M.max_broad, M.max_rows, M.row_width
+        d.max_dyn_welds = M.max_dyn_welds
         d.mass_blk_size, d.max_entity_dofs = M.mass_blk_size, M.max_entity_dofs
         d.iterations, d.ls_iterations, d.n_contacts_per_pair = o.iterations, o.ls_iterations, o.n_contacts_per_pair
         d.enable_collision, d.enable_joint_limit = int(o.enable_collision), int(o.enable_joint_limit)
@@ -175,7 +188,8 @@ class DeviceModel:
         if dyn_variant is None:
             dyn_variant = d.solver_variant if explicit else default_stage_variant(M, n_envs)
         d.collide_variant, d.dyn_variant = int(collide_variant), int(dyn_variant)
-        d.batch_dofs = int(self.batch_dofs)
+        d.batch_dofs, d.batch_links, d.batch_joints = int(self.batch_dofs), int(self.batch_links), int(self.batch_joints)
+        d.batch_eqs = 1
         d.wave_h_floats = wave_h_floats_default(M) if wave_h_floats is None else int(wave_h_floats)
         d.dt, d.tolerance, d.ls_tolerance = o.dt, o.tolerance, o.ls_tolerance
         d.eps = float(np.finfo(np.float32).eps)
@@ -184,8 +198,17 @@ class DeviceModel:
         for i in range(3):
             d.gravity[i] = float(M.gravity[i])
         for k in _lib.MODEL_POINTERS:
-            setattr(d, k, self.tensors[k].data_ptr())
+            if k == "env_meaninertia":
+                d.env_meaninertia = None if self.env_meaninertia is None else self.env_meaninertia.data_ptr()
+            else:
+                setattr(d, k, self.tensors[k].data_ptr())
         self.desc = d
+
+    def batched(self, name) -> bool:
+        """Whether the device table `name` holds one value per env (env-minor)."""
+        return (name in BATCHED_EQ_TABLES or (self.batch_dofs and name in BATCHED_DOF_TABLES)
+                or (self.batch_links and name in BATCHED_LINK_TABLES)
+                or (self.batch_joints and name in BATCHED_JOINT_TABLES))
 
 
 class DeviceState:
@@ -221,6 +244,7 @@ class DeviceState:
         self.t["mass_shift"][:, sl] = 0
         self.t["i_pos_shift"][:, sl] = 0
         self.t["friction_ratio"][:, sl] = 1
+        self.t["eq_dyn_n"][:, sl] = 0
 
     def memory_bytes(self):
         return sum(t.numel() * t.element_size() for t in self.t.values())
@@ -481,6 +505,8 @@ class RigidSolver:
         for k in MODEL_TABLES_F:
             out[f"RigidSolver.model.{k}"] = self.dm.tensors[k].detach().cpu().numpy()
         out["RigidSolver.model.meaninertia"] = np.float32(self.model.meaninertia)
+        if self.dm.env_meaninertia is not None:
+            out["RigidSolver.model.env_meaninertia"] = self.dm.env_meaninertia.detach().cpu().numpy()
         return out
 
     def load_checkpoint_arrays(self, arrays):
@@ -512,17 +538,24 @@ class RigidSolver:
             self.dm.tensors[k].copy_(torch.from_numpy(a).to(self.device))
             host = getattr(self.model, k)
             # a per-env table has no host copy; an empty table is uploaded as one padding element
-            if not (self.dm.batch_dofs and k in BATCHED_DOF_TABLES) and host.size:
+            if not self.dm.batched(k) and host.size:
                 host[...] = a.reshape(host.shape).astype(host.dtype)
         mi = arrays.get("RigidSolver.model.meaninertia")
         if mi is not None:
             self.model.meaninertia = np.float32(mi)
             self.dm.desc.meaninertia = float(mi)
+        emi = arrays.get("RigidSolver.model.env_meaninertia")
+        if emi is not None and self.dm.env_meaninertia is not None and emi.shape == tuple(self.dm.env_meaninertia.shape):
+            self.dm.env_meaninertia.copy_(torch.from_numpy(np.ascontiguousarray(emi, np.float32)).to(self.device))
 
-    # -- model parameters (rsd:1777-2010 setters, 2334-2452 getters) ---------------------------------------------
-    # One value per dof / link / geom shared by every env: the reference's default batch_dofs_info =
-    # batch_links_info = False (options/solvers.py:373-375). The host model arrays and their device tables are updated
-    # together (in place, so every descriptor holding the table pointers sees the new values at the next launch).
+    # -- model parameters (rsd:1777-2010 setters, 1880-1925 / 2200-2225 sol params, 2334-2452 getters) -----------
+    # A table holds one value per dof / link / joint shared by every env (the reference's default batch_dofs_info =
+    # batch_links_info = batch_joints_info = False, options/solvers.py:373-375) or, when its kind is batched, one
+    # value per env (env-minor device table, no host copy); equality solver parameters are always per env. Shared
+    # tables keep the host model array and the device table equal; both are updated in place, so every descriptor
+    # holding the table pointers sees the new values at the next launch.
+    _KIND_N = {"dofs": "n_dofs", "links": "n_links", "joints": "n_joints", "geoms": "n_geoms", "eqs": "n_equalities"}
+
     def _model_write(self, name, values, idx, n, width=0):
         arr = getattr(self.model, name)
         i = self._idx(idx, n, device="cpu").numpy()
@@ -545,39 +578,52 @@ class RigidSolver:
     @staticmethod
     def _no_envs(envs_idx, what):
         if envs_idx is not None:
-            raise GenesisException(f"`envs_idx` cannot be specified for non-batched {what} info "
-                                   f"(batched per-env {what} parameters are not built).")
+            raise GenesisException(f"`envs_idx` cannot be specified for non-batched {what} info.")
 
-    def _set_dofs(self, name, values, dofs_idx, envs_idx, width=0):
-        if self.dm.batch_dofs and name in BATCHED_DOF_TABLES:
-            # one value per (env, dof): the table is (n * width, B) env-minor; values (n_envs, n[, width]) or (n[, w])
-            n, D = len(self._idx(dofs_idx, self.model.n_dofs)), self.model.n_dofs
+    def _set_table(self, name, values, idx, envs_idx, kind, width=0):
+        """Write `values` for entries idx (all if None) of table `name`: (n_envs, n[, width]) or (n[, width]) values
+        broadcast over envs_idx for a per-env table; (n[, width]) for a shared one, where envs_idx must be None."""
+        n = int(getattr(self.model, self._KIND_N[kind]))
+        if self.dm.batched(name):
+            k, w = len(self._idx(idx, n)), max(width, 1)
             e = self._idx(envs_idx, self._B)
-            v = torch.as_tensor(values, dtype=torch.float32, device=self.device).reshape(-1, n, max(width, 1))
-            self._write_env_table(self.dm.tensors[name].view(D * max(width, 1), self._B),
-                                  v.expand(len(e), n, max(width, 1)), dofs_idx, D, e, max(width, 1))
+            v = torch.as_tensor(values, dtype=torch.float32, device=self.device).reshape(-1, k, w)
+            self._write_env_table(self.dm.tensors[name].view(n * w, self._B), v.expand(len(e), k, w), idx, n, e, w)
             return
-        self._no_envs(envs_idx, "dofs")
-        self._model_write(name, values, dofs_idx, self.model.n_dofs, width)
+        self._no_envs(envs_idx, kind)
+        self._model_write(name, values, idx, n, width)
 
     def _write_env_table(self, table, v, idx, n, e, width):
         i = self._idx(idx, n)
         if bool(((i < 0) | (i >= n)).any()):
             raise IndexError(f"index out of range [0, {n})")
+        if bool(((e < 0) | (e >= self._B)).any()):
+            raise IndexError(f"envs_idx out of range [0, {self._B})")
         rows = (i[:, None] * width + torch.arange(width, device=self.device)[None]).reshape(-1)
         table[rows[:, None], e[None, :]] = v.reshape(len(e), -1).T
 
-    def _get_dofs(self, name, dofs_idx, envs_idx, width=1):
-        if self.dm.batch_dofs and name in BATCHED_DOF_TABLES:
-            D = self.model.n_dofs
-            i = self._idx(dofs_idx, D)
+    def _get_table(self, name, idx, envs_idx, kind, width=1):
+        """Values of entries idx: (n_envs, n[, width]) for a per-env table ((n[, width]) without batch envs), else
+        (n[, width]), where envs_idx must be None."""
+        n = int(getattr(self.model, self._KIND_N[kind]))
+        if self.dm.batched(name):
+            i = self._idx(idx, n)
+            if bool(((i < 0) | (i >= n)).any()):
+                raise IndexError(f"{name}: index out of range [0, {n})")
             rows = (i[:, None] * width + torch.arange(width, device=self.device)[None]).reshape(-1)
-            v = self.dm.tensors[name].view(D * width, self._B)[rows][:, self._idx(envs_idx, self._B)].T
+            v = self.dm.tensors[name].view(n * width, self._B)[rows][:, self._idx(envs_idx, self._B)].T
             v = v.reshape(-1, len(i), width)
             v = v[..., 0] if width == 1 else v
             return v[0] if self.n_envs == 0 else v
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read(name, dofs_idx, self.model.n_dofs)
+        self._no_envs(envs_idx, kind)
+        return self._model_read(name, idx, n)
+
+    # kept under their former names for the dof setters below
+    def _set_dofs(self, name, values, dofs_idx, envs_idx, width=0):
+        self._set_table(name, values, dofs_idx, envs_idx, "dofs", width)
+
+    def _get_dofs(self, name, dofs_idx, envs_idx, width=1):
+        return self._get_table(name, dofs_idx, envs_idx, "dofs", width)
 
     def set_dofs_kp(self, kp, dofs_idx=None, envs_idx=None):
         self._set_dofs("dof_kp", kp, dofs_idx, envs_idx)
@@ -604,26 +650,90 @@ class RigidSolver:
 
     def set_dofs_armature(self, armature, dofs_idx=None, envs_idx=None):
         """Armature enters the mass matrix, so the invweights and meaninertia are recomputed at the neutral pose
-        (rsd:1998, 1964-1969 -> _init_invweight_and_meaninertia :354-507); the envs keep their qpos and lose their
-        contact and warm-start caches, as through set_qpos."""
-        from .model import _init_invweight_and_meaninertia
+        (rsd:1964-1969 -> _init_invweight_and_meaninertia :354-507); the envs keep their qpos and lose their
+        contact and warm-start caches, as through set_qpos. Choosing envs needs batched dofs and links (rsd:360-364)."""
+        if envs_idx is not None and not (self.dm.batch_dofs and self.dm.batch_links):
+            raise GenesisException("Links and dofs must be batched to selectively update invweight and meaninertia "
+                                   "for some environment.")
         self._set_dofs("dof_armature", armature, dofs_idx, envs_idx)
-        _init_invweight_and_meaninertia(self.model)
-        for k in ("dof_invweight", "link_invweight"):
-            self.dm.tensors[k].copy_(torch.from_numpy(np.ascontiguousarray(getattr(self.model, k), np.float32).ravel()))
-        self.dm.desc.meaninertia = float(self.model.meaninertia)
-        self.reset_caches()
-        self.forward_kinematics()
+        self._update_invweight_and_meaninertia(envs_idx)
+
+    def _update_invweight_and_meaninertia(self, envs_idx=None):
+        """Link / dof inverse weights and mean inertia of envs_idx (all if None) from the float64 host restatement
+        of _init_invweight_and_meaninertia (rsd:354-515) on each env's armature and link masses at qpos0: per-env
+        tables get each env's values, shared tables the first env's (the reference's loop stops after it when
+        dofs and links are not both batched); per-env mean inertia goes to every env. Envs with equal parameters
+        share one evaluation."""
+        import copy
+
+        from .model import _init_invweight_and_meaninertia
+        dm, M = self.dm, self.model
+        e = self._idx(envs_idx, self._B).cpu().numpy()
+        if dm.env_meaninertia is None:
+            _init_invweight_and_meaninertia(M)
+            for k in ("dof_invweight", "link_invweight"):
+                dm.tensors[k].copy_(torch.from_numpy(np.ascontiguousarray(getattr(M, k), np.float32).ravel()))
+            dm.desc.meaninertia = float(M.meaninertia)
+        else:
+            D, L, B = M.n_dofs, M.n_links, self._B
+            arm = dm.tensors["dof_armature"].view(D, B).cpu().numpy() if dm.batched("dof_armature") else None
+            mass = (dm.tensors["link_inertial_mass"].view(L, B).cpu().numpy()
+                    if dm.batched("link_inertial_mass") else None)
+            memo, out = {}, []
+            for b in e:
+                key = (b"" if arm is None else arm[:, b].tobytes(), b"" if mass is None else mass[:, b].tobytes())
+                if key not in memo:
+                    Mb = copy.copy(M)
+                    if arm is not None:
+                        Mb.dof_armature = arm[:, b].copy()
+                    if mass is not None:
+                        Mb.link_inertial_mass = mass[:, b].copy()
+                    _init_invweight_and_meaninertia(Mb)
+                    memo[key] = (Mb.dof_invweight, Mb.link_invweight, np.float32(Mb.meaninertia))
+                out.append(memo[key])
+            ed = torch.as_tensor(e, device=self.device)
+            for k, j, n in (("dof_invweight", 0, D), ("link_invweight", 1, 2 * L)):
+                if dm.batched(k):
+                    vals = torch.as_tensor(np.stack([o[j].reshape(-1) for o in out], 1), device=self.device)
+                    dm.tensors[k].view(n, B)[:, ed] = vals
+                else:
+                    setattr(M, k, out[0][j].copy())
+                    dm.tensors[k].copy_(torch.from_numpy(np.ascontiguousarray(out[0][j], np.float32).ravel()))
+            dm.env_meaninertia[ed] = torch.as_tensor(np.array([o[2] for o in out], np.float32), device=self.device)
+        ei = None if envs_idx is None else torch.as_tensor(e, dtype=torch.int32, device=self.device)
+        self.reset_caches(ei)
+        self.forward_kinematics(ei)
 
     def set_links_inertial_mass(self, mass, links_idx=None, envs_idx=None):
         """Link masses enter the COM and composite inertias that the end-of-step kinematics keep for the next step,
-        so those are recomputed here for every env."""
-        self._no_envs(envs_idx, "links")
-        self._model_write("link_inertial_mass", mass, links_idx, self.model.n_links)
-        self.forward_kinematics()
+        so those are recomputed here (for envs_idx with batched links, else for every env); like the reference,
+        the inverse weights keep their build values."""
+        self._set_table("link_inertial_mass", mass, links_idx, envs_idx, "links")
+        self.forward_kinematics(None if envs_idx is None else self._idx(envs_idx, self._B).to(torch.int32))
 
     def set_geoms_friction(self, friction, geoms_idx=None):
         self._model_write("geom_friction", friction, geoms_idx, self.model.n_geoms)
+
+    def set_sol_params(self, sol_params, geoms_idx=None, envs_idx=None, *, joints_idx=None, eqs_idx=None):
+        """Constraint solver parameters (timeconst, dampratio, dmin, dmax, width, mid, power) of geoms (shared),
+        joints (per env with batch_joints_info) or equalities (always per env), sanitised with the minimum time
+        constant 2 dt as their default (rsd:1880-1925, 49-77)."""
+        from .model import TIME_CONSTANT_SAFETY_FACTOR, sanitize_sol_params
+        if sum(x is not None for x in (geoms_idx, joints_idx, eqs_idx)) > 1:
+            raise GenesisException("Cannot set more than one constraint type at once.")
+        kind, idx, name = (("eqs", eqs_idx, "eq_sol_params") if eqs_idx is not None else
+                           ("joints", joints_idx, "joint_sol_params") if joints_idx is not None else
+                           ("geoms", geoms_idx, "geom_sol_params"))
+        min_tc = TIME_CONSTANT_SAFETY_FACTOR * float(self.model.options.dt)
+        sp = np.asarray(sol_params.detach().cpu() if torch.is_tensor(sol_params) else sol_params, np.float64)
+        sp = sanitize_sol_params(sp, min_tc, min_tc).reshape(sp.shape)
+        self._set_table(name, sp, idx, envs_idx, kind, 7)
+
+    def get_sol_params(self, geoms_idx=None, envs_idx=None, *, joints_idx=None, eqs_idx=None):
+        kind, idx, name = (("eqs", eqs_idx, "eq_sol_params") if eqs_idx is not None else
+                           ("joints", joints_idx, "joint_sol_params") if joints_idx is not None else
+                           ("geoms", geoms_idx, "geom_sol_params"))
+        return self._get_table(name, idx, envs_idx, kind, 7)
 
     def get_dofs_kp(self, dofs_idx=None, envs_idx=None):
         return self._get_dofs("dof_kp", dofs_idx, envs_idx)
@@ -641,12 +751,10 @@ class RigidSolver:
         return self._get_dofs("dof_frictionloss", dofs_idx, envs_idx)
 
     def get_dofs_armature(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read("dof_armature", dofs_idx, self.model.n_dofs)
+        return self._get_dofs("dof_armature", dofs_idx, envs_idx)
 
     def get_dofs_invweight(self, dofs_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "dofs")
-        return self._model_read("dof_invweight", dofs_idx, self.model.n_dofs)
+        return self._get_dofs("dof_invweight", dofs_idx, envs_idx)
 
     def get_dofs_force_range(self, dofs_idx=None, envs_idx=None):
         t = self._get_dofs("dof_force_range", dofs_idx, envs_idx, 2)
@@ -657,12 +765,17 @@ class RigidSolver:
         return t[..., 0], t[..., 1]
 
     def get_links_inertial_mass(self, links_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "links")
-        return self._model_read("link_inertial_mass", links_idx, self.model.n_links)
+        return self._get_table("link_inertial_mass", links_idx, envs_idx, "links")
 
     def get_links_invweight(self, links_idx=None, envs_idx=None):
-        self._no_envs(envs_idx, "links")
-        return self._model_read("link_invweight", links_idx, self.model.n_links)
+        return self._get_table("link_invweight", links_idx, envs_idx, "links", 2)
+
+    def get_meaninertia(self, envs_idx=None):
+        """The solver's per-env mean inertia (rigid_global_info.meaninertia)."""
+        e = self._idx(envs_idx, self._B)
+        if self.dm.env_meaninertia is not None:
+            return self.dm.env_meaninertia[e].clone()
+        return torch.full((len(e),), float(self.dm.desc.meaninertia), dtype=torch.float32, device=self.device)
 
     def get_geoms_friction(self, geoms_idx=None):
         return self._model_read("geom_friction", geoms_idx, self.model.n_geoms)
@@ -832,6 +945,34 @@ class Collider:
 EQUALITY_WELD = 1  # gs.EQUALITY_TYPE: CONNECT 0, WELD 1, JOINT 2
 
 
+# quaternion helpers over (4, n) / vectors over (3, n) float32 tensors, in the operation order of
+# genesis/utils/geom.py (ti_inv_quat, ti_quat_mul, ti_transform_by_quat)
+def _inv_quat(q):
+    return torch.stack([q[0], -q[1], -q[2], -q[3]])
+
+
+def _quat_mul(u, v):
+    w = u[0] * v[0] - u[1] * v[1] - u[2] * v[2] - u[3] * v[3]
+    x = u[0] * v[1] + u[1] * v[0] + u[2] * v[3] - u[3] * v[2]
+    y = u[0] * v[2] - u[1] * v[3] + u[2] * v[0] + u[3] * v[1]
+    z = u[0] * v[3] + u[1] * v[2] - u[2] * v[1] + u[3] * v[0]
+    return torch.stack([w, x, y, z])
+
+
+def _transform_by_quat(v, q):
+    qw, qx, qy, qz = q
+    xx, xy, xz, wx = qx * qx, qx * qy, qx * qz, qx * qw
+    yy, yz, wy = qy * qy, qy * qz, qy * qw
+    zz, wz = qz * qz, qz * qw
+    ww = qw * qw
+    out = torch.stack([
+        v[0] * (xx + ww - yy - zz) + v[1] * (2.0 * xy - 2.0 * wz) + v[2] * (2.0 * xz + 2.0 * wy),
+        v[0] * (2.0 * wz + 2.0 * xy) + v[1] * (ww - xx + yy - zz) + v[2] * (-2.0 * wx + 2.0 * yz),
+        v[0] * (-2.0 * wy + 2.0 * xz) + v[1] * (2.0 * wx + 2.0 * yz) + v[2] * (ww - xx - yy + zz),
+    ])
+    return out / (ww + xx + yy + zz)
+
+
 class ConstraintSolver:
     """Equality-constraint readback of the reference's ConstraintSolver (constraint_solver_decomp.py:235-322)."""
 
@@ -839,27 +980,98 @@ class ConstraintSolver:
         self._solver = solver
 
     def get_equality_constraints(self, as_tensor: bool = True, to_torch: bool = True):
-        """{type, obj_a, obj_b, force}: (n_envs, n_equalities[, 6]) with as_tensor, else a tuple per env. Every env
-        holds the model's equalities, in model order."""
+        """{type, obj_a, obj_b, force}: (n_envs, max equalities of an env[, 6]) with as_tensor (type -1 past an env's
+        count), else a tuple per env. An env holds the model's equalities, in model order, then its own welds."""
         S = self._solver
-        B, E = S._B, S.model.n_equalities
-        iout = torch.full((B * E, 3), -1, dtype=torch.int32, device=S.device)
-        fout = torch.zeros((B * E, 6), dtype=torch.float32, device=S.device)
-        if B * E > 0:
+        B, E, K = S._B, S.model.n_equalities, S.model.max_dyn_welds
+        N = E + K
+        iout = torch.full((B * N, 3), -1, dtype=torch.int32, device=S.device)
+        fout = torch.zeros((B * N, 6), dtype=torch.float32, device=S.device)
+        if B * N > 0:
             rc = S._lib.gsr_get_equalities(S.dm.desc, S.st.desc, ctypes.c_void_p(iout.data_ptr()),
                                            ctypes.c_void_p(fout.data_ptr()), S._stream())
             _lib.check(rc, "gsr_get_equalities")
-        iout, fout = iout.reshape(B, E, 3), fout.reshape(B, E, 6)
+        counts = E + (S.st.t["eq_dyn_n"][0].to(torch.int64) if K > 0 else torch.zeros(B, dtype=torch.int64,
+                                                                                          device=S.device))
+        n_max = int(counts.max().item()) if B > 0 else 0
+        iout, fout = iout.reshape(B, N, 3)[:, :n_max], fout.reshape(B, N, 6)[:, :n_max]
         if as_tensor:
             values = (iout[..., 0], iout[..., 1], iout[..., 2], fout)
             if not to_torch:
                 values = _to_numpy(values)
         else:
-            per_env = [(iout[b, :, 0], iout[b, :, 1], iout[b, :, 2], fout[b]) for b in range(B)]
+            cnt = counts.tolist()
+            per_env = [(iout[b, :cnt[b], 0], iout[b, :cnt[b], 1], iout[b, :cnt[b], 2], fout[b, :cnt[b]])
+                       for b in range(B)]
             if not to_torch:
                 per_env = [_to_numpy(env) for env in per_env]
             values = tuple(zip(*per_env))
         return dict(zip(("type", "obj_a", "obj_b", "force"), values))
+
+    def add_weld_constraint(self, link1_idx, link2_idx, envs_idx=None):
+        """Registers a weld of link2 to link1 in their current relative pose (anchors at link1's origin, relative
+        quaternion inv(q1) q2, torque scale 1; kernel_add_weld_constraint, constraint_solver_decomp.py:2433-2480)
+        in the listed envs. The pair must not be welded already; an env whose list is full
+        (RigidOptions.max_dynamic_constraints) keeps its welds and is reported in a warning."""
+        S = self._solver
+        l1, l2 = int(link1_idx), int(link2_idx)
+        if not (0 <= l1 < S.model.n_links and 0 <= l2 < S.model.n_links):
+            raise IndexError("link index out of range")
+        info = self.get_weld_constraints(as_tensor=True, to_torch=True)
+        la, lb = info["link_a"], info["link_b"]
+        if bool((((la == l1) | (lb == l1)) & ((la == l2) | (lb == l2))).any()):
+            raise GenesisException(f"links {l1} and {l2} are welded already")
+        K = S.model.max_dyn_welds
+        envs = S._idx(envs_idx, S._B)
+        t = S.st.t
+        n = t["eq_dyn_n"][0, envs].to(torch.int64)
+        full = n >= K
+        if bool(full.any()):
+            import warnings
+            warnings.warn(f"{int(full.sum())} env(s) hold max_dynamic_constraints = {K} welds already; the weld is "
+                          "not added there")
+        envs, n = envs[~full], n[~full]
+        if envs.numel() == 0:
+            return
+        pos = t["links_pos"].view(S.model.n_links, 3, -1)
+        quat = t["links_quat"].view(S.model.n_links, 4, -1)
+        p1, p2 = pos[l1][:, envs], pos[l2][:, envs]
+        q1, q2 = quat[l1][:, envs], quat[l2][:, envs]
+        shared = p1
+        data = torch.empty((11, envs.numel()), dtype=torch.float32, device=S.device)
+        data[3:6] = _transform_by_quat(shared - p1, _inv_quat(q1))
+        data[0:3] = _transform_by_quat(shared - p2, _inv_quat(q2))
+        data[6:10] = _quat_mul(_inv_quat(q1), q2)
+        data[10] = 1.0
+        rows = 11 * n[None, :] + torch.arange(11, device=S.device)[:, None]
+        t["eq_dyn_data"][rows, envs[None, :]] = data
+        t["eq_dyn_obj"][2 * n, envs] = l1
+        t["eq_dyn_obj"][2 * n + 1, envs] = l2
+        t["eq_dyn_n"][0, envs] = (n + 1).to(torch.int32)
+
+    def delete_weld_constraint(self, link1_idx, link2_idx, envs_idx=None):
+        """Removes the weld of (link1, link2) registered by add_weld_constraint from the listed envs; the env's last
+        weld takes its place (kernel_delete_weld_constraint, constraint_solver_decomp.py:2484-2506, moves only the
+        type of the last entry: here the whole entry moves, so the remaining welds are the ones still registered)."""
+        S = self._solver
+        l1, l2 = int(link1_idx), int(link2_idx)
+        K = S.model.max_dyn_welds
+        if K == 0:
+            return
+        t = S.st.t
+        envs = S._idx(envs_idx, S._B)
+        obj = t["eq_dyn_obj"][:, envs].view(K, 2, -1)
+        n = t["eq_dyn_n"][0, envs].to(torch.int64)
+        slots = torch.arange(K, device=S.device)[:, None]
+        match = (obj[:, 0] == l1) & (obj[:, 1] == l2) & (slots < n[None, :])
+        for b_local in torch.nonzero(match.any(0)).flatten().tolist():
+            b = int(envs[b_local])
+            k = int(torch.nonzero(match[:, b_local])[0])
+            last = int(n[b_local]) - 1
+            if k < last:
+                t["eq_dyn_obj"][2 * k:2 * k + 2, b] = t["eq_dyn_obj"][2 * last:2 * last + 2, b]
+                t["eq_dyn_data"][11 * k:11 * k + 11, b] = t["eq_dyn_data"][11 * last:11 * last + 11, b]
+            t["eq_dyn_n"][0, b] = last
 
     def get_weld_constraints(self, as_tensor: bool = True, to_torch: bool = True):
         """The WELD subset of get_equality_constraints, with obj_a / obj_b renamed link_a / link_b

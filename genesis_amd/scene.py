@@ -312,6 +312,13 @@ class RigidEntity:
         self._solver.set_links_inertial_mass(mass, self._global(links_idx_local, self.n_links, self.link_start),
                                              envs_idx)
 
+    def get_links_invweight(self, links_idx_local=None, envs_idx=None):
+        return self._solver.get_links_invweight(self._global(links_idx_local, self.n_links, self.link_start),
+                                                envs_idx)
+
+    def get_dofs_invweight(self, dofs_idx_local=None, envs_idx=None):
+        return self._solver.get_dofs_invweight(self._dofs(dofs_idx_local), envs_idx)
+
     # per-env shifts (rigid_entity.py:2852-2935)
     def set_friction_ratio(self, friction_ratio, links_idx_local=None, envs_idx=None):
         """Friction ratio per listed link, applied to every geom of the link (shape (n_envs, n_links))."""
@@ -410,7 +417,8 @@ class Scene:
             enable_collision=ro.enable_collision and not ro.disable_constraint, enable_joint_limit=ro.enable_joint_limit,
             enable_self_collision=ro.enable_self_collision, enable_adjacent_collision=ro.enable_adjacent_collision,
             enable_multi_contact=ro.enable_multi_contact, max_dynamic_constraints=ro.max_dynamic_constraints,
-            batch_dofs_info=ro.batch_dofs_info)
+            batch_dofs_info=ro.batch_dofs_info, batch_links_info=ro.batch_links_info,
+            batch_joints_info=ro.batch_joints_info)
 
     def build(self, n_envs: int = 0, device="cuda", solver_variant=None):
         ro = self.rigid_options

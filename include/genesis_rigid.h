@@ -56,12 +56,18 @@ typedef struct gsr_model {
   int32_t solver_variant;  /* 0: one lane per env (rows env-minor), 1: one wavefront per env (rows env-major) */
   int32_t collide_variant; /* 0: one lane per env, 1: one wavefront per env (pairs across lanes) */
   int32_t dyn_variant;     /* step_1 / step_2: 0 one lane per env, 1 one wavefront per env (entities across lanes) */
-  int32_t batch_dofs;      /* 1: dof_kp, dof_kv, dof_force_range, dof_stiffness, dof_damping, dof_frictionloss and
-                              dof_limit hold one value per env, env-minor like the state (component c of dof d of env b
-                              at [(d * k + c) * n_envs + b]); 0: one value per dof shared by every env */
+  int32_t batch_dofs;      /* 1: dof_kp, dof_kv, dof_force_range, dof_stiffness, dof_damping, dof_frictionloss,
+                              dof_limit, dof_armature and dof_invweight hold one value per env, env-minor like the state
+                              (component c of dof d of env b at [(d * k + c) * n_envs + b]); 0: one value per dof
+                              shared by every env (RigidOptions.batch_dofs_info) */
+  int32_t batch_links;     /* 1: link_inertial_mass and link_invweight hold one value per env, env-minor
+                              (RigidOptions.batch_links_info) */
+  int32_t batch_joints;    /* 1: joint_sol_params holds one value per env, env-minor (RigidOptions.batch_joints_info) */
+  int32_t batch_eqs;       /* 1: eq_sol_params holds one value per env, env-minor (the reference's equality solver
+                              parameters are always per env: RigidSolver.set_sol_params(eqs_idx, envs_idx)) */
   int32_t wave_h_floats;   /* wavefront solver: LDS floats reserved for one env's Hessian (a larger pattern keeps it
                               in global memory); sets the solver's LDS per env, so how many envs share a CU */
-  int32_t pad_model;
+  int32_t max_dyn_welds;   /* capacity of the per-env weld list (gsr_state.eq_dyn_*), RigidOptions.max_dynamic_constraints */
   float dt, tolerance, ls_tolerance, eps, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
   float gravity[4];
   /* links */
@@ -94,6 +100,9 @@ typedef struct gsr_model {
   /* collision pair table (G*G, -1 = never collide) and CRBA ancestor mask (D*D) */
   const int32_t *pair_idx;
   const float *mass_parent_mask;
+  /* per-env mean inertia (n_envs; the solver's tolerance scale, rigid_global_info.meaninertia[i_b]) when per-env
+     armature or link masses make it differ between envs, else null and meaninertia above holds for every env */
+  const float *env_meaninertia;
 } gsr_model;
 
 /* Per-env state and scratch of n_envs envs of one scene type (all env-minor). */
@@ -109,6 +118,10 @@ typedef struct gsr_state {
   uint32_t *errno_env;
   /* per-env model shifts: links_state.mass_shift (L), links_state.i_pos_shift (L x 3), geoms_state.friction_ratio (G) */
   float *mass_shift, *i_pos_shift, *friction_ratio;
+  /* welds registered per env after the build (ConstraintSolver.add_weld_constraint), after the model's own equalities:
+     count (1), (link1, link2) pairs (2 x max_dyn_welds), data in the model's eq_data layout (11 x max_dyn_welds) */
+  int32_t *eq_dyn_n, *eq_dyn_obj;
+  float *eq_dyn_data;
   /* kinematics / dynamics intermediates */
   float *cdof_ang, *cdof_vel, *cdofd_ang, *cdofd_vel, *root_com, *cd_vel, *cd_ang, *cinr_inertial, *cinr_pos;
   float *cinr_mass, *links_iquat, *mass_blk, *mass_L_blk, *mass_D_inv, *qf_smooth, *force;
@@ -182,8 +195,9 @@ int gsr_get_contacts(const gsr_model* model, const gsr_state* state, const int32
 int gsr_links_contact_force(const gsr_model* model, const gsr_state* state, const int32_t* dev_envs_idx,
                             int32_t n_idx, float* dev_out, hipStream_t stream);
 /* Replaces kernel_get_equality_constraints (constraint_solver_decomp.py:2510-2558) behind
- * ConstraintSolver.get_equality_constraints (:235-287): row b * n_equalities + i holds iout[3r..] = {type, obj1, obj2}
- * and fout[6r..] = the equality's row forces (one for a JOINT equality, the rest zero). */
+ * ConstraintSolver.get_equality_constraints (:235-287): row b * (n_equalities + max_dyn_welds) + i holds
+ * iout[3r..] = {type, obj1, obj2} and fout[6r..] = the equality's row forces (1 JOINT, 3 CONNECT, 6 WELD, the rest
+ * zero): the model's equalities, then the env's own welds; rows past the env's count hold type -1 and zeros. */
 int gsr_get_equalities(const gsr_model* model, const gsr_state* state, int32_t* dev_iout, float* dev_fout,
                        hipStream_t stream);
 

@@ -48,7 +48,7 @@ struct RawModel {
 
 template <class T>
 struct Model {
-  int L, J, D, Q, G, E, NEQ, P, max_contacts, max_broad, max_rows;
+  int L, J, D, Q, G, E, NEQ, P, max_contacts, max_broad, max_rows, max_dyn_welds;
   int iterations, ls_iterations, n_contacts_per_pair;
   int enable_collision, enable_joint_limit, enable_multi_contact;
   T dt, tolerance, ls_tolerance, EPS, meaninertia, mc_perturbation, mc_tolerance, mpr_to_gjk;
@@ -88,6 +88,7 @@ struct Model {
     G = r.scalar_i("n_geoms"); E = r.scalar_i("n_entities"); NEQ = r.scalar_i("n_equalities");
     P = r.scalar_i("n_pairs"); max_contacts = r.scalar_i("max_contacts"); max_broad = r.scalar_i("max_broad");
     max_rows = r.scalar_i("max_rows");
+    max_dyn_welds = r.scalar_i("max_dyn_welds");
     iterations = r.scalar_i("iterations"); ls_iterations = r.scalar_i("ls_iterations");
     n_contacts_per_pair = r.scalar_i("n_contacts_per_pair");
     enable_collision = r.scalar_i("enable_collision"); enable_joint_limit = r.scalar_i("enable_joint_limit");
@@ -162,6 +163,10 @@ struct Env {
   std::vector<V3<T>> i_pos_shift;             // links_state.i_pos_shift (L)
   std::vector<T> sort_value;
   std::vector<int32_t> sort_ig, sort_ismax;
+  // welds registered after the build (constraint_state.ti_n_equalities beyond the model's): count, link pairs, data
+  int eq_dyn_n = 0;
+  std::vector<int32_t> eq_dyn_obj;
+  std::vector<T> eq_dyn_data;
   int first_time = 1;
   uint32_t errno_ = 0;
   // kinematics
@@ -202,6 +207,7 @@ struct Env {
     cache_normal.assign(std::max(m.P, 1), V3<T>());
     mass_shift.assign(m.L, 0); i_pos_shift.assign(m.L, V3<T>()); friction_ratio.assign(m.G, 1);
     sort_value.assign(2 * G, 0); sort_ig.assign(2 * G, 0); sort_ismax.assign(2 * G, 0);
+    eq_dyn_obj.assign(2 * (size_t)m.max_dyn_welds, 0); eq_dyn_data.assign(11 * (size_t)m.max_dyn_welds, 0);
     for (auto* v : {&l_pos, &l_ipos_bw, &l_ipos, &root_com, &root_com_bw, &j_pos, &cd_vel, &cd_ang, &cdd_vel, &cdd_ang,
                     &cinr_pos, &cfrc_vel, &cfrc_ang, &crb_pos})
       v->assign(L, V3<T>());
@@ -1658,13 +1664,130 @@ void narrow_phase(const Model<T>& m, Env<T>& e) {
 // ------------------------------------------------------------------------------------------------------------
 // Constraints (constraint_solver_decomp.py)
 // ------------------------------------------------------------------------------------------------------------
+// Row n: component i3 of the velocity difference of the point p1 fixed to link l1 (sign +1) and p2 fixed to l2
+// (sign -1), both root chains walked from the link up, each link's dofs last first (constraint_solver_decomp.py
+// 586-616); returns J qvel in that order.
+template <class T>
+T point_diff_row(const Model<T>& m, Env<T>& e, int n, int l1, V3<T> p1, int l2, V3<T> p2, int i3) {
+  const int D = m.D;
+  for (int d = 0; d < D; d++) e.jac[(size_t)n * D + d] = 0;
+  T jac_qvel = 0;
+  for (int iab = 0; iab < 2; iab++) {
+    const T sign = iab == 0 ? T(1) : T(-1);
+    int link = iab == 0 ? l1 : l2;
+    const V3<T> pos = iab == 0 ? p1 : p2;
+    while (link > -1) {
+      const int nd = m.link_dof_end[link] - m.link_dof_start[link];
+      for (int k = 0; k < nd; k++) {
+        const int d = m.link_dof_end[link] - 1 - k;
+        const V3<T> tpos = pos - e.root_com[link];
+        const V3<T> vel = transform_by_quat(e.cdof_vel[d] - cross(tpos, e.cdof_ang[d]), Q4<T>(1, 0, 0, 0));
+        const V3<T> diff = sign * vel;
+        const T jac = diff[i3];
+        jac_qvel = jac_qvel + jac * e.vel[d];
+        e.jac[(size_t)n * D + d] = e.jac[(size_t)n * D + d] + jac;
+      }
+      link = m.link_parent[link];
+    }
+  }
+  return jac_qvel;
+}
+
+template <class T>
+void eq_row_finish(const Model<T>& m, Env<T>& e, int n, const T* sp, T pen, T jac_qvel, T pos, T invw) {
+  T imp, aref;
+  imp_aref(sp, -pen, jac_qvel, pos, imp, aref);
+  const T diag = std::max(invw * (T(1) - imp) / imp, m.EPS);
+  e.diag[n] = diag; e.aref[n] = aref; e.efc_D[n] = T(1) / diag;
+}
+
+// func_equality_connect (constraint_solver_decomp.py:528-641): data[0:3] on link l1 and data[3:6] on l2 coincide
+template <class T>
+void add_connect(const Model<T>& m, Env<T>& e, int l1, int l2, const T* data, const T* sp) {
+  const V3<T> ga1 = transform_by_trans_quat(V3<T>(data[0], data[1], data[2]), e.l_pos[l1], e.l_quat[l1]);
+  const V3<T> ga2 = transform_by_trans_quat(V3<T>(data[3], data[4], data[5]), e.l_pos[l2], e.l_quat[l2]);
+  const T invw = m.link_invweight[2 * l1] + m.link_invweight[2 * l2];
+  for (int i3 = 0; i3 < 3; i3++) {
+    const int n = e.n_con++;
+    e.n_con_eq++;
+    const T jq = point_diff_row(m, e, n, l1, ga1, l2, ga2, i3);
+    const V3<T> pd = ga1 - ga2;
+    eq_row_finish(m, e, n, sp, norm(pd), jq, pd[i3], invw);
+  }
+}
+
+// ti_quat_mul_axis (genesis/utils/geom.py:195-204): q * (0, v)
+template <class T>
+Q4<T> quat_mul_axis(Q4<T> q, V3<T> v) {
+  return Q4<T>(-q.x * v[0] - q.y * v[1] - q.z * v[2], +q.w * v[0] + q.y * v[2] - q.z * v[1],
+               +q.w * v[1] + q.z * v[0] - q.x * v[2], +q.w * v[2] + q.x * v[1] - q.y * v[0]);
+}
+
+// func_equality_weld (constraint_solver_decomp.py:824-1015): data[0:3] anchor on l2, data[3:6] anchor on l1,
+// data[6:10] pose of l2 relative to l1, data[10] torque scale; 3 position rows, then 3 orientation rows
+template <class T>
+void add_weld(const Model<T>& m, Env<T>& e, int l1, int l2, const T* data, const T* sp) {
+  const int D = m.D;
+  const Q4<T> q1 = e.l_quat[l1], q2 = e.l_quat[l2];
+  const Q4<T> rel(data[6], data[7], data[8], data[9]);
+  const T ts = data[10];
+  const V3<T> ga1 = transform_by_trans_quat(V3<T>(data[3], data[4], data[5]), e.l_pos[l1], q1);
+  const V3<T> ga2 = transform_by_trans_quat(V3<T>(data[0], data[1], data[2]), e.l_pos[l2], q2);
+  const V3<T> pe = ga1 - ga2;
+  const Q4<T> q = quat_mul(q1, rel);
+  const Q4<T> iq2 = inv_quat(q2);
+  const Q4<T> eq = quat_mul(iq2, q);
+  const V3<T> re = V3<T>(eq.x, eq.y, eq.z) * ts;
+  const T pos_imp = std::sqrt(pe[0] * pe[0] + pe[1] * pe[1] + pe[2] * pe[2] + re[0] * re[0] + re[1] * re[1] + re[2] * re[2]);
+  const T invw0 = m.link_invweight[2 * l1] + m.link_invweight[2 * l2];
+  const T invw1 = m.link_invweight[2 * l1 + 1] + m.link_invweight[2 * l2 + 1];
+  for (int i = 0; i < 3; i++) {
+    const int n = e.n_con++;
+    e.n_con_eq++;
+    const T jq = point_diff_row(m, e, n, l1, ga1, l2, ga2, i);
+    eq_row_finish(m, e, n, sp, pos_imp, jq, pe[i], invw0);
+  }
+  const int n0 = e.n_con;
+  e.n_con += 3;
+  e.n_con_eq += 3;
+  for (int i = 0; i < 3; i++)
+    for (int d = 0; d < D; d++) e.jac[(size_t)(n0 + i) * D + d] = 0;
+  for (int iab = 0; iab < 2; iab++) {
+    const T sign = iab == 0 ? T(1) : T(-1);
+    int link = iab == 0 ? l1 : l2;
+    while (link > -1) {
+      const int nd = m.link_dof_end[link] - m.link_dof_start[link];
+      for (int k = 0; k < nd; k++) {
+        const int d = m.link_dof_end[link] - 1 - k;
+        const V3<T> jac = sign * e.cdof_ang[d];
+        for (int i = 0; i < 3; i++) e.jac[(size_t)(n0 + i) * D + d] = e.jac[(size_t)(n0 + i) * D + d] + jac[i];
+      }
+      link = m.link_parent[link];
+    }
+  }
+  T jq[3] = {0, 0, 0};
+  for (int d = 0; d < D; d++) {
+    const V3<T> jr(e.jac[(size_t)n0 * D + d], e.jac[(size_t)(n0 + 1) * D + d], e.jac[(size_t)(n0 + 2) * D + d]);
+    const Q4<T> q3 = quat_mul(quat_mul_axis(iq2, jr), q);
+    const T v3[3] = {q3.x, q3.y, q3.z};
+    for (int i = 0; i < 3; i++) {
+      e.jac[(size_t)(n0 + i) * D + d] = T(0.5) * v3[i] * ts;
+      jq[i] = jq[i] + e.jac[(size_t)(n0 + i) * D + d] * e.vel[d];
+    }
+  }
+  for (int i = 0; i < 3; i++) eq_row_finish(m, e, n0 + i, sp, pos_imp, jq[i], re[i], invw1);
+}
+
+// add_equality_constraints (constraint_solver_decomp.py:778-822): the model's equalities, then the env's own welds
 template <class T>
 void add_equality(const Model<T>& m, Env<T>& e) {
   const int D = m.D;
   e.n_con = 0; e.n_con_eq = 0;
   for (int ie = 0; ie < m.NEQ; ie++) {
-    if (m.eq_type[ie] != EQ_JOINT) continue;  // CONNECT/WELD are not used by the compiled workloads
     const T* sp = &m.eq_sol_params[7 * ie];
+    const T* data = &m.eq_data[11 * ie];
+    if (m.eq_type[ie] == EQ_CONNECT) { add_connect(m, e, m.eq_obj1[ie], m.eq_obj2[ie], data, sp); continue; }
+    if (m.eq_type[ie] == EQ_WELD) { add_weld(m, e, m.eq_obj1[ie], m.eq_obj2[ie], data, sp); continue; }
     int j1 = m.eq_obj1[ie], j2 = m.eq_obj2[ie];
     int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2], d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
     int n = e.n_con++;
@@ -1672,7 +1795,6 @@ void add_equality(const Model<T>& m, Env<T>& e) {
     for (int d = 0; d < D; d++) e.jac[(size_t)n * D + d] = 0;
     T pos1 = e.qpos[q1], pos2 = e.qpos[q2], ref1 = m.qpos0[q1], ref2 = m.qpos0[q2];
     T diff = pos2 - ref2, pos = pos1 - ref1, deriv = 0;
-    const T* data = &m.eq_data[11 * ie];
     for (int i5 = 0; i5 < 5; i5++) {
       T dp = tpow(diff, T(i5));
       pos = pos - dp * data[i5];
@@ -1687,6 +1809,10 @@ void add_equality(const Model<T>& m, Env<T>& e) {
     T diag = std::max(invw * (T(1) - imp) / imp, m.EPS);
     e.diag[n] = diag; e.aref[n] = aref; e.efc_D[n] = T(1) / diag;
   }
+  // welds registered per env (kernel_add_weld_constraint's solver parameters, constraint_solver_decomp.py:2475-2477)
+  const T sp_dyn[7] = {T(2) * m.dt, T(1), T(0.9f), T(0.95f), T(0.001f), T(0.5f), T(2)};
+  for (int k = 0; k < e.eq_dyn_n; k++)
+    add_weld(m, e, e.eq_dyn_obj[2 * k], e.eq_dyn_obj[2 * k + 1], &e.eq_dyn_data[11 * (size_t)k], sp_dyn);
 }
 
 template <class T>
@@ -2227,8 +2353,8 @@ extern "C" void orc_build(void* hp, int B, int precision) {
 
 // Generic state access: field names below; arrays are env-major (B, n) float64 / int32.
 #define ORC_FIELDS_F(X) X(qpos) X(vel) X(qacc_ws) X(ctrl_pos) X(ctrl_vel) X(ctrl_force) X(acc) X(acc_smooth) X(dofs_pos) X(sort_value) \
-  X(mass_shift) X(friction_ratio)
-#define ORC_FIELDS_I(X) X(ctrl_mode) X(sort_ig) X(sort_ismax)
+  X(mass_shift) X(friction_ratio) X(eq_dyn_data)
+#define ORC_FIELDS_I(X) X(ctrl_mode) X(sort_ig) X(sort_ismax) X(eq_dyn_obj)
 
 template <class T>
 static void get_set_env(Env<T>& e, const Model<T>& m, const char* name, double* f, int32_t* iv, int set) {
@@ -2241,6 +2367,7 @@ static void get_set_env(Env<T>& e, const Model<T>& m, const char* name, double* 
 #undef X
   if (n == "is_warmstart") { if (set) e.is_warmstart = iv[0]; else iv[0] = e.is_warmstart; return; }
   if (n == "first_time") { if (set) e.first_time = iv[0]; else iv[0] = e.first_time; return; }
+  if (n == "eq_dyn_n") { if (set) e.eq_dyn_n = iv[0]; else iv[0] = e.eq_dyn_n; return; }
   if (n == "errno") { if (set) e.errno_ = uint32_t(iv[0]); else iv[0] = int32_t(e.errno_); return; }
   if (n == "n_contacts") { if (set) e.n_contacts = iv[0]; else iv[0] = e.n_contacts; return; }
   if (n == "n_broad") { if (set) e.n_broad = iv[0]; else iv[0] = e.n_broad; return; }

@@ -60,10 +60,11 @@ FLOAT_TABLES = [
 
 # state field -> (kind, per-env shape builder)
 FIELDS_F = {"qpos": "Q", "vel": "D", "qacc_ws": "D", "ctrl_pos": "D", "ctrl_vel": "D", "ctrl_force": "D", "acc": "D",
-            "acc_smooth": "D", "dofs_pos": "D", "sort_value": "2G", "mass_shift": "L", "friction_ratio": "G"}
-FIELDS_I = {"ctrl_mode": "D", "sort_ig": "2G", "sort_ismax": "2G"}
+            "acc_smooth": "D", "dofs_pos": "D", "sort_value": "2G", "mass_shift": "L", "friction_ratio": "G",
+            "eq_dyn_data": "11K"}
+FIELDS_I = {"ctrl_mode": "D", "sort_ig": "2G", "sort_ismax": "2G", "eq_dyn_obj": "2K"}
 SCALAR_I = ["is_warmstart", "first_time", "errno", "n_contacts", "n_broad", "n_con", "n_newton_iters", "n_con_eq",
-            "n_con_fl"]
+            "n_con_fl", "eq_dyn_n"]
 ROW_F = ["efc_D", "efc_aref", "efc_diag", "efc_fl", "efc_force"]
 
 
@@ -89,7 +90,8 @@ class Oracle:
         ints = dict(n_links=model.n_links, n_joints=model.n_joints, n_dofs=model.n_dofs, n_qs=model.n_qs,
                     n_geoms=model.n_geoms, n_entities=model.n_entities, n_equalities=model.n_equalities,
                     n_pairs=model.n_pairs, max_contacts=model.max_contacts, max_broad=model.max_broad,
-                    max_rows=model.max_rows, iterations=o.iterations, ls_iterations=o.ls_iterations,
+                    max_rows=model.max_rows, max_dyn_welds=model.max_dyn_welds, iterations=o.iterations,
+                    ls_iterations=o.ls_iterations,
                     n_contacts_per_pair=o.n_contacts_per_pair, enable_collision=int(o.enable_collision),
                     enable_joint_limit=int(o.enable_joint_limit), enable_multi_contact=int(o.enable_multi_contact))
         f32 = lambda x: float(np.float32(x))  # noqa: E731
@@ -115,6 +117,8 @@ class Oracle:
             pass
 
     def _n(self, key):
+        if key.endswith("K"):
+            return int(key[:-1]) * self.m.max_dyn_welds
         return 2 * self.sizes["G"] if key == "2G" else self.sizes[key]
 
     def set(self, name, arr, envs=None):

@@ -21,6 +21,7 @@ def gpu_fields(solver, envs=None):
         sort_ig=(sort_ig & (MAXB - 1)).astype(np.int32), sort_ismax=((sort_ig & MAXB) != 0).astype(np.int32),
         first_time=g("first_time")[:, 0], mass_shift=g("mass_shift").astype(np.float64),
         i_pos_shift=g("i_pos_shift").astype(np.float64), friction_ratio=g("friction_ratio").astype(np.float64),
+        eq_dyn_n=g("eq_dyn_n")[:, 0], eq_dyn_obj=g("eq_dyn_obj"), eq_dyn_data=g("eq_dyn_data").astype(np.float64),
     )
 
 

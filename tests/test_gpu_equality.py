@@ -1,0 +1,126 @@
+"""CONNECT and WELD equality rows on the GPU (constraint_solver_decomp.py:528-641, 824-1015) and the welds registered
+per env after the build (ConstraintSolver.add_weld_constraint / delete_weld_constraint / get_weld_constraints,
+:289-364, 2433-2506): the HIP step is held to the oracle over resynced steps with the step parity tests' tolerance,
+and the weld readback gathers the rows' forces."""
+import numpy as np
+import pytest
+import torch
+
+from genesis_amd import workloads as W
+from genesis_amd.model import EQ_CONNECT, EQ_WELD, CompileOptions, EqualitySpec, compile_model
+from tests.sync import gpu_to_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _scene(static_eqs=True, max_dyn=8):
+    """Plane, a box welded to the plane 1 m up, a box hanging from the plane at one corner (CONNECT), and two free
+    boxes on the ground that later welds join."""
+    plane = W.plane_entity()
+    boxes = [W.box_entity((0.1, 0.1, 0.1), (0.0, 0.0, 1.0)), W.box_entity((0.1, 0.1, 0.1), (0.5, 0.0, 1.0)),
+             W.box_entity((0.1, 0.1, 0.1), (-0.4, 0.3, 0.05)), W.box_entity((0.12, 0.1, 0.08), (-0.25, 0.3, 0.04))]
+    for i, b in enumerate(boxes):
+        b.links[0].name = f"box{i}"
+    if static_eqs:
+        plane.equalities = [EqualitySpec(EQ_WELD, "plane_baselink", "box0"),
+                            EqualitySpec(EQ_CONNECT, "plane_baselink", "box1", data=np.array([0.55, 0.05, 1.05]))]
+    return compile_model([plane, *boxes], CompileOptions(dt=0.01, max_dynamic_constraints=max_dyn))
+
+
+def _perturbed(M, B, rng):
+    q = np.tile(M.qpos0.astype(np.float64), (B, 1))
+    q[:, 7:10] += rng.uniform(-0.02, 0.02, (B, 3))  # box1 starts off its anchor: the CONNECT rows pull it back
+    return torch.as_tensor(q, dtype=torch.float32)
+
+
+def _step_against_oracle(S, O, steps, label):
+    worst = 0.0
+    for step in range(steps):
+        torch.cuda.synchronize()
+        gpu_to_oracle(S, O)
+        S.substep()
+        O.run(1)
+        torch.cuda.synchronize()
+        assert (S.st.t["n_con"][1].cpu().numpy() == O.get("n_con_eq")).all(), f"{label} step {step}: equality rows"
+        for name in ("qpos", "vel"):
+            g_, o_ = S.st.t[name].T.cpu().numpy(), O.get(name)
+            err = np.abs(g_ - o_)
+            bound = TOL + TOL * np.abs(o_)
+            worst = max(worst, float((err / bound).max()))
+            assert (err <= bound).all(), f"{label} step {step} {name}: max excess {(err - bound).max():.3g}"
+    return worst
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_connect_and_weld_rows_match_oracle(variant):
+    _need_gpu()
+    from genesis_amd.rigid_solver import RigidSolver
+    from oracle.oracle import Oracle
+    B = 16
+    M = _scene()
+    S = RigidSolver(M, B, "cuda", solver_variant=variant)
+    S.set_qpos(_perturbed(M, B, np.random.default_rng(3)))
+    O = Oracle(M, B, 32)
+    worst = _step_against_oracle(S, O, 60, f"v{variant}")
+    assert int(S.st.t["n_con"][1].max()) == 9  # 6 weld rows + 3 connect rows
+    print(f"static welds v{variant}: worst |err| / bound {worst:.3g}")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_registered_welds_match_oracle_and_read_back(variant):
+    """Welds registered in some envs at step 20 (box2 to box3 where they lie, box3 to the plane), one deleted at
+    step 40: the rows exist exactly in those envs, the step follows the oracle (which receives the per-env weld list
+    with the state), and get_weld_constraints returns each env's welds with the forces of their 6 rows."""
+    _need_gpu()
+    from genesis_amd.rigid_solver import GenesisException, RigidSolver
+    from oracle.oracle import Oracle
+    B = 8
+    M = _scene(static_eqs=False, max_dyn=2)
+    S = RigidSolver(M, B, "cuda", solver_variant=variant)
+    O = Oracle(M, B, 32)
+    l2, l3, lp = 3, 4, 0
+    assert M.max_dyn_welds == 2
+    _step_against_oracle(S, O, 20, "before")
+    cs = S.constraint_solver
+    cs.add_weld_constraint(l2, l3, envs_idx=[0, 2, 5])
+    cs.add_weld_constraint(l3, lp, envs_idx=[2, 5, 6])
+    with pytest.raises(GenesisException):
+        cs.add_weld_constraint(l3, l2, envs_idx=[1])
+    n = S.st.t["eq_dyn_n"][0].cpu().numpy()
+    np.testing.assert_array_equal(n, [1, 0, 2, 0, 0, 2, 1, 0])
+    _step_against_oracle(S, O, 20, "welded")
+    torch.cuda.synchronize()
+    w = cs.get_weld_constraints()
+    assert tuple(w["link_a"].shape) == (B, 2)
+    la, lb = w["link_a"].cpu().numpy(), w["link_b"].cpu().numpy()
+    assert (la[0, 0], lb[0, 0]) == (l2, l3) and la[0, 1] == -1
+    assert (la[2, 0], lb[2, 0], la[2, 1], lb[2, 1]) == (l2, l3, l3, lp)
+    assert (la[6, 0], lb[6, 0]) == (l3, lp) and (la[1] == -1).all()
+    # the forces are the efc_force of each weld's 6 rows (the welds are the env's only equality rows)
+    ef = S.st.t["efc_force"]
+    R = S.model.max_rows
+    rows = ef.reshape(B, R) if variant == 1 else ef.reshape(R, B).T
+    f = w["force"].cpu()
+    for b in (0, 2, 5, 6):
+        k = int(n[b])
+        np.testing.assert_array_equal(f[b, :k].reshape(-1).numpy(), rows[b, :6 * k].cpu().numpy())
+    ragged = cs.get_weld_constraints(as_tensor=False)
+    assert [len(x) for x in ragged["force"]] == list(n)
+    # delete the first weld of env 2: the env's last weld takes its place
+    cs.delete_weld_constraint(l2, l3, envs_idx=[2])
+    n2 = S.st.t["eq_dyn_n"][0].cpu().numpy()
+    np.testing.assert_array_equal(n2, [1, 0, 1, 0, 0, 2, 1, 0])
+    w = cs.get_weld_constraints()
+    assert (int(w["link_a"][2, 0]), int(w["link_b"][2, 0])) == (l3, lp)
+    # a full list keeps its welds
+    with pytest.warns(UserWarning):
+        cs.add_weld_constraint(1, 2, envs_idx=[5])
+    assert int(S.st.t["eq_dyn_n"][0, 5]) == 2
+    _step_against_oracle(S, O, 20, "after delete")

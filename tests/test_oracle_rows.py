@@ -338,3 +338,228 @@ def test_mesh_cube_on_plane_known_answer(Oracle):
     np.testing.assert_allclose(c[:, 7:10], np.tile([0, 0, -1.0], (4, 1)), atol=2e-4)
     xy = sorted(map(tuple, np.round(c[:, 4:6] / s).astype(int).tolist()))
     assert xy == [(-1, -1), (-1, 1), (1, -1), (1, 1)]
+
+
+def _two_arms(eqs):
+    """Two 2-dof hinge arms on fixed bases (hinges about y then x), so every dof is an angle and a row's Jacobian is
+    the derivative of its residual with respect to qpos."""
+    def arm(name, base):
+        fix = LinkSpec(f"{name}0", -1, np.asarray(base, float), np.array([1.0, 0, 0, 0]), [JointSpec("f", JOINT_FIXED)], [])
+        l1 = LinkSpec(f"{name}1", 0, np.array([0, 0, 0.3]), np.array([1.0, 0, 0, 0]),
+                      [JointSpec(f"{name}h1", JOINT_REVOLUTE, axis=np.array([0, 1.0, 0]))], [],
+                      inertial_pos=np.array([0.2, 0, 0]), inertial_quat=np.array([1.0, 0, 0, 0]),
+                      inertial_i=np.eye(3) * 1e-3, inertial_mass=1.0)
+        l2 = LinkSpec(f"{name}2", 1, np.array([0.4, 0, 0]), np.array([1.0, 0, 0, 0]),
+                      [JointSpec(f"{name}h2", JOINT_REVOLUTE, axis=np.array([1.0, 0, 0]))], [],
+                      inertial_pos=np.array([0.1, 0.05, 0]), inertial_quat=np.array([1.0, 0, 0, 0]),
+                      inertial_i=np.eye(3) * 1e-3, inertial_mass=0.5)
+        return [fix, l1, l2]
+    a = EntitySpec(arm("a", [0, 0, 0]), equalities=eqs, name="a")
+    b = EntitySpec(arm("b", [0.9, 0.2, 0]), name="b")
+    return compile_model([a, b], CompileOptions(dt=0.01, enable_collision=False, enable_joint_limit=False))
+
+
+def _anchor_gap(M, q, l1, a1, l2, a2):
+    from genesis_amd.model import _host_fk_crba, transform_by_quat
+    lp, lq = _host_fk_crba(M, q)[:2]
+    return lp[l1] + transform_by_quat(a1, lq[l1]) - (lp[l2] + transform_by_quat(a2, lq[l2])), lq
+
+
+def test_connect_rows_known_answer(Oracle):
+    """CONNECT (constraint_solver_decomp.py:528-641): anchor data[0:3] on link a2 and data[3:6] on link b2 (taken from
+    the build pose, where the two points coincide). At another pose each row i is the derivative of the anchor gap's
+    component i with respect to the joint angles (central differences of the float64 kinematics), and aref is
+    imp_aref(-|gap|, J qvel, gap_i) evaluated by hand."""
+    from genesis_amd.model import EQ_CONNECT
+    M = _two_arms([EqualitySpec(EQ_CONNECT, "a2", "b2", data=np.array([0.15, 0.0, 0.0]))])
+    l1, l2 = M.eq_obj1[0], M.eq_obj2[0]
+    d = M.eq_data[0].astype(np.float64)
+    gap0, _ = _anchor_gap(M, M.qpos0.astype(np.float64), l1, d[0:3], l2, d[3:6])
+    np.testing.assert_allclose(gap0, 0, atol=1e-6)
+    q = np.array([0.3, -0.2, 0.1, 0.25])
+    qd = np.array([0.4, -0.7, 0.2, 0.5])
+    o = Oracle(M, 1, 64)
+    o.set("qpos", q[None])
+    o.set("vel", qd[None])
+    o.run(1, stage=4)
+    assert o.get("n_con_eq")[0] == 3
+    J = o.get("efc_jac")[0][:3]
+    h = 1e-6
+    Jfd = np.stack([(_anchor_gap(M, q + h * np.eye(4)[k], l1, d[0:3], l2, d[3:6])[0]
+                     - _anchor_gap(M, q - h * np.eye(4)[k], l1, d[0:3], l2, d[3:6])[0]) / (2 * h) for k in range(4)], 1)
+    np.testing.assert_allclose(J, Jfd, atol=1e-7)
+    gap, _ = _anchor_gap(M, q, l1, d[0:3], l2, d[3:6])
+    sp = M.eq_sol_params[0].astype(np.float64)
+    x = min(np.linalg.norm(gap) / sp[4], 1.0)
+    y = x ** 2 / sp[5] if x < sp[5] else 1 - (1 - x) ** 2 / (1 - sp[5])
+    imp = sp[2] + y * (sp[3] - sp[2]) if x < 1 else sp[3]
+    tc = max(sp[0], 2 * f32(0.01))
+    b, k = 2 / (sp[3] * tc), 1 / (sp[3] ** 2 * tc ** 2 * sp[1] ** 2)
+    np.testing.assert_allclose(o.get("efc_aref")[0][:3], -b * (J @ qd) - k * imp * gap, rtol=1e-6)
+    iw = M.link_invweight[l1, 0] + M.link_invweight[l2, 0]
+    np.testing.assert_allclose(o.get("efc_diag")[0][:3], iw * (1 - imp) / imp, rtol=1e-6)
+
+
+def test_weld_rows_known_answer(Oracle):
+    """WELD (constraint_solver_decomp.py:824-1015) of b2 to a1 from the build pose (anchor at a1's origin, relative
+    quaternion inv(q_a1) q_b2, torque scale 1): the position rows are the derivatives of the anchor gap, the
+    orientation rows the derivatives of the vector part of the error quaternion inv(q_b2) q_a1 rel (its time
+    derivative is 0.5 inv(q_b2) (0, w_a1 - w_b2) q_a1 rel, which the rows restate), both by central differences;
+    at the build pose every residual is zero."""
+    from genesis_amd.model import EQ_WELD, quat_mul
+    M = _two_arms([EqualitySpec(EQ_WELD, "a1", "b2")])
+    l1, l2 = M.eq_obj1[0], M.eq_obj2[0]
+    d = M.eq_data[0].astype(np.float64)
+    np.testing.assert_allclose(d[3:6], 0, atol=1e-7)
+    assert d[10] == 1.0
+
+    def residual(q):
+        gap, lq = _anchor_gap(M, q, l1, d[3:6], l2, d[0:3])
+        e = quat_mul(np.array([lq[l2][0], -lq[l2][1], -lq[l2][2], -lq[l2][3]]), quat_mul(lq[l1], d[6:10]))
+        return np.concatenate([gap, e[1:] * d[10]])
+
+    np.testing.assert_allclose(residual(M.qpos0.astype(np.float64)), 0, atol=1e-6)
+    q = np.array([0.3, -0.2, 0.1, 0.25])
+    o = Oracle(M, 1, 64)
+    o.set("qpos", q[None])
+    o.run(1, stage=4)
+    assert o.get("n_con_eq")[0] == 6
+    J = o.get("efc_jac")[0][:6]
+    h = 1e-6
+    Jfd = np.stack([(residual(q + h * np.eye(4)[k]) - residual(q - h * np.eye(4)[k])) / (2 * h) for k in range(4)], 1)
+    np.testing.assert_allclose(J, Jfd, atol=1e-7)
+    # orientation rows weigh by the rotational inverse weights, position rows by the translational ones
+    sp = M.eq_sol_params[0].astype(np.float64)
+    r = residual(q)
+    x = min(np.linalg.norm(r) / sp[4], 1.0)
+    imp = sp[3] if x >= 1 else None
+    assert imp is not None  # the residual is far beyond the impedance width here
+    iw0 = M.link_invweight[l1, 0] + M.link_invweight[l2, 0]
+    iw1 = M.link_invweight[l1, 1] + M.link_invweight[l2, 1]
+    np.testing.assert_allclose(o.get("efc_diag")[0][:6], np.repeat([iw0, iw1], 3) * (1 - imp) / imp, rtol=1e-6)
+
+
+def test_welded_box_falls_with_its_carrier(Oracle):
+    """A free box welded (from the build pose) to a second free box: under gravity the pair falls together like one
+    free body (the semi-implicit Euler drop g dt^2 n (n + 1) / 2 after n steps) and keeps its relative pose."""
+    from genesis_amd.model import EQ_WELD
+    a = W.box_entity((0.1, 0.1, 0.1), (0, 0, 2.0))
+    b = W.box_entity((0.1, 0.1, 0.1), (0.15, 0.0, 2.05))
+    a.links[0].name, b.links[0].name = "boxa", "boxb"
+    a.equalities = [EqualitySpec(EQ_WELD, "boxa", "boxb")]
+    M = compile_model([a, b], CompileOptions(dt=0.01, enable_collision=False))
+    o = Oracle(M, 1, 64)
+    for _ in range(40):
+        o.run(1)
+    q = o.get("qpos")[0]
+    drop = 9.81 * 0.01 ** 2 * 40 * 41 / 2
+    np.testing.assert_allclose([2.0 - q[2], 2.05 - q[9]], drop, rtol=1e-5)
+    np.testing.assert_allclose(q[7:10] - q[0:3], [0.15, 0.0, 0.05], atol=1e-6)
+    np.testing.assert_allclose(q[10:14], q[3:7], atol=1e-6)
+
+
+def test_box_welded_to_the_world_holds_against_gravity(Oracle):
+    """A free box welded to the plane's fixed link 1 m up, and one connected to it at a corner: after 2 s both are at
+    rest within a few millimetres of the build pose (the soft constraint settles where its spring balances gravity),
+    and the connected box hangs from its corner (its COM below the anchor)."""
+    from genesis_amd.model import EQ_CONNECT, EQ_WELD
+    plane = W.plane_entity()
+    a = W.box_entity((0.1, 0.1, 0.1), (0.0, 0.0, 1.0))
+    b = W.box_entity((0.1, 0.1, 0.1), (0.5, 0.0, 1.0))
+    a.links[0].name, b.links[0].name = "boxa", "boxb"
+    plane.equalities = [EqualitySpec(EQ_WELD, "plane_baselink", "boxa"),
+                        EqualitySpec(EQ_CONNECT, "plane_baselink", "boxb", data=np.array([0.55, 0.05, 1.05]))]
+    M = compile_model([plane, a, b], CompileOptions(dt=0.01, enable_collision=False))
+    o = Oracle(M, 1, 64)
+    for _ in range(200):
+        o.run(1)
+    q, v = o.get("qpos")[0], o.get("vel")[0]
+    np.testing.assert_allclose(q[0:3], [0, 0, 1.0], atol=5e-3)
+    np.testing.assert_allclose(v[0:6], 0, atol=1e-3)
+    # the connected box's corner stays at the anchor; the box swings below it
+    from genesis_amd.model import transform_by_quat
+    corner = q[7:10] + transform_by_quat(np.array([0.05, 0.05, 0.05]), q[10:14])
+    np.testing.assert_allclose(corner, [0.55, 0.05, 1.05], atol=5e-3)
+    assert q[9] < 1.05 - 0.05
+
+
+MJCF_EQ = """<mujoco>
+  <worldbody>
+    <body name="post" pos="0 0 1">
+      <inertial pos="0 0 0" mass="1" diaginertia="0.01 0.01 0.01"/>
+      <body name="arm" pos="0 0 0">
+        <joint name="hinge" type="hinge" axis="0 1 0"/>
+        <inertial pos="0.2 0 0" mass="1" diaginertia="0.01 0.01 0.01"/>
+      </body>
+    </body>
+    <body name="bob" pos="0.4 0 1">
+      <freejoint/>
+      <inertial pos="0 0 0" mass="0.5" diaginertia="0.002 0.002 0.002"/>
+    </body>
+    <body name="lid" pos="0.4 0.3 1.2" quat="0.9238795 0 0 0.3826834">
+      <freejoint/>
+      <inertial pos="0 0 0" mass="0.2" diaginertia="0.001 0.001 0.001"/>
+    </body>
+    <body name="cap" pos="-0.5 0 0.5">
+      <freejoint/>
+      <inertial pos="0 0 0" mass="0.3" diaginertia="0.001 0.001 0.001"/>
+    </body>
+  </worldbody>
+  <equality>
+    <connect body1="arm" body2="bob" anchor="0.4 0 0"/>
+    <weld body1="bob" body2="lid" anchor="0.05 0 0" torquescale="0.5"/>
+    <weld body1="lid" body2="cap" anchor="0 0.1 0" relpose="0 0 -0.2 1 0 0 1"/>
+    <connect body1="cap" anchor="0 0 0.1" solref="0.05 1"/>
+  </equality>
+</mujoco>
+"""
+
+
+def test_mjcf_connect_and_weld_compile_to_build_pose_data(tmp_path, Oracle):
+    """MJCF <connect> / <weld> (mjcf.py:700-725 takes them from MuJoCo's compiled eq_data): body1's CONNECT anchor in
+    data[0:3] and its body2-frame image at the build pose in data[3:6]; a WELD without relpose welds body2's anchor
+    where it is at the build pose (zero residual there, torquescale kept); a WELD with relpose puts body1's anchor at
+    relpose's position plus the rotated anchor, with relpose's normalised quaternion; an omitted body2 is the world
+    frame (a fixed link at the origin). The build data survive the JSON round trip, and the oracle's CONNECT to the
+    world holds the free body it carries up against gravity."""
+    from genesis_amd.loaders import entity_from_json, entity_to_json, parse_mjcf
+    from genesis_amd.model import EQ_CONNECT, EQ_WELD, quat_mul
+    p = tmp_path / "eq.xml"
+    p.write_text(MJCF_EQ)
+    ent = parse_mjcf(str(p))
+    assert [e.type for e in ent.equalities] == [EQ_CONNECT, EQ_WELD, EQ_WELD, EQ_CONNECT]
+    M = compile_model([ent], CompileOptions(dt=0.01, enable_collision=False))
+    d = M.eq_data.astype(np.float64)
+    q0 = M.qpos0.astype(np.float64)
+    for i in (0, 3):  # CONNECT: body1's anchor and body2's image coincide at the build pose
+        np.testing.assert_allclose(_anchor_gap(M, q0, M.eq_obj1[i], d[i, 0:3], M.eq_obj2[i], d[i, 3:6])[0], 0,
+                                   atol=1e-6)
+    np.testing.assert_allclose(d[0, 0:3], [0.4, 0, 0])
+    np.testing.assert_allclose(d[3, 0:3], [0, 0, 0.1])
+    np.testing.assert_allclose(d[3, 3:6], [-0.5, 0, 0.6], atol=1e-6)  # the world frame is the identity
+    # WELD without relpose: anchor on body2, zero position and orientation residual at the build pose
+    np.testing.assert_allclose(d[1, 0:3], [0.05, 0, 0])
+    gap, lq = _anchor_gap(M, q0, M.eq_obj1[1], d[1, 3:6], M.eq_obj2[1], d[1, 0:3])
+    np.testing.assert_allclose(gap, 0, atol=1e-6)
+    l1, l2 = M.eq_obj1[1], M.eq_obj2[1]
+    err = quat_mul(np.array([lq[l2][0], -lq[l2][1], -lq[l2][2], -lq[l2][3]]), quat_mul(lq[l1], d[1, 6:10]))
+    np.testing.assert_allclose(np.abs(err), [1, 0, 0, 0], atol=1e-6)
+    assert d[1, 10] == 0.5
+    # WELD with relpose (a 90 degree turn about z, quaternion given unnormalised): body1's anchor = relpose position
+    # + the turned anchor (0, 0.1, 0) -> (-0.1, 0, 0)
+    s = np.sqrt(0.5)
+    np.testing.assert_allclose(d[2, 6:10], [s, 0, 0, s], atol=1e-7)
+    np.testing.assert_allclose(d[2, 3:6], [-0.1, 0, -0.2], atol=1e-7)
+    np.testing.assert_allclose(d[2, 0:3], [0, 0.1, 0])
+    assert M.eq_sol_params[3][0] == np.float32(0.05)
+    M2 = compile_model([entity_from_json(entity_to_json(ent))], CompileOptions(dt=0.01, enable_collision=False))
+    np.testing.assert_array_equal(M2.eq_data, M.eq_data)
+    # under gravity the chain (hinged arm - bob - lid - cap - world) holds together: after 2 s every anchor pair is
+    # within a few millimetres
+    o = Oracle(M, 1, 64)
+    for _ in range(200):
+        o.run(1)
+    q = o.get("qpos")[0].astype(np.float64)
+    for i, (a1, a2) in enumerate(((0, 3), (3, 0), (3, 0), (0, 3))):
+        gap = _anchor_gap(M, q, M.eq_obj1[i], d[i, a1:a1 + 3], M.eq_obj2[i], d[i, a2:a2 + 3])[0]
+        assert np.linalg.norm(gap) < 5e-3, (i, gap)
