@@ -124,3 +124,52 @@ def test_registered_welds_match_oracle_and_read_back(variant):
         cs.add_weld_constraint(1, 2, envs_idx=[5])
     assert int(S.st.t["eq_dyn_n"][0, 5]) == 2
     _step_against_oracle(S, O, 20, "after delete")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_per_env_equality_solver_parameters_match_per_env_oracles(variant):
+    """Equality solver parameters are per env (RigidSolver.set_sol_params(eqs_idx, envs_idx),
+    rigid_solver_decomp.py:1880-1925; always batched): envs 1-3 get their own time constants and damping ratios for
+    the weld and the connect, env 0 keeps the build values. From one common start each env follows an oracle of the
+    model carrying its parameters, and a softer weld lets its box sag further."""
+    _need_gpu()
+    import copy
+    from genesis_amd.rigid_solver import GenesisException, RigidSolver
+    from oracle.oracle import Oracle
+    B = 4
+    M = _scene()
+    S = RigidSolver(M, B, "cuda", solver_variant=variant)
+    q = np.tile(M.qpos0.astype(np.float64), (B, 1))
+    q[:, 7:10] += [0.01, -0.015, 0.02]
+    S.set_qpos(torch.as_tensor(q, dtype=torch.float32))
+    sp = np.tile(M.eq_sol_params.astype(np.float64), (3, 1, 1))
+    sp[:, :, 0] = [[0.05, 0.04], [0.2, 0.1], [0.01, 0.3]]  # time constants (0.01 is raised to 2 dt = 0.02)
+    sp[:, :, 1] = [[1.0, 0.7], [0.5, 1.2], [2.0, 1.0]]
+    S.set_sol_params(sp, eqs_idx=[0, 1], envs_idx=[1, 2, 3])
+    got = S.get_sol_params(eqs_idx=[0, 1]).cpu().numpy()
+    assert got.shape == (B, 2, 7)
+    np.testing.assert_array_equal(got[0], M.eq_sol_params)
+    assert got[3, 0, 0] == np.float32(0.02) and got[2, 1, 0] == np.float32(0.1)
+    with pytest.raises(GenesisException):
+        S.set_sol_params(sp[0], eqs_idx=[0, 1], joints_idx=[0])
+    oracles = []
+    for b in range(B):
+        Mb = copy.copy(M)
+        Mb.eq_sol_params = got[b].astype(np.float32)
+        oracles.append(Oracle(Mb, 1, 32))
+    for step in range(60):
+        torch.cuda.synchronize()
+        for b, O in enumerate(oracles):
+            gpu_to_oracle(S, O, envs=slice(b, b + 1))
+        S.substep()
+        for O in oracles:
+            O.run(1)
+        torch.cuda.synchronize()
+        qg, vg = S.st.t["qpos"].T.cpu().numpy(), S.st.t["vel"].T.cpu().numpy()
+        for b, O in enumerate(oracles):
+            for name, g_ in (("qpos", qg[b]), ("vel", vg[b])):
+                o_ = O.get(name)[0]
+                err = np.abs(g_ - o_)
+                assert (err <= TOL + TOL * np.abs(o_)).all(), f"step {step} env {b} {name}"
+    z = S.st.t["qpos"][2].cpu().numpy()  # the welded box's height per env
+    assert z[2] < z[0] - 1e-3  # time constant 0.2 against the build's 0.02: a softer weld sags further
