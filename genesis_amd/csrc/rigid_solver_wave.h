@@ -1117,7 +1117,8 @@ __device__ __forceinline__ void wave_hessian_direct(const WaveCtx& w, const HS& 
   if (wave_dof_lanes(w)) {
     wave_hessian_by_dof(w, h, ptr, lst);
     __syncthreads();
-    wave_chol_factor(w, h);
+    if (!HS::kFence && w.dense_h) wave_chol_factor_dense(w);
+    else wave_chol_factor(w, h);
     return;
   }
   for (int ei = w.lane; ei < w.E; ei += kWave) {
