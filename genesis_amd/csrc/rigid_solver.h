@@ -263,6 +263,55 @@ __device__ inline int dev_frictionloss_rows(const Env& e, int n_con, int& n_fl) 
 }
 
 // the 4 pyramid rows of contact ic, stored at rows c0 .. c0 + 3 (add_collision_constraints)
+// Dofs of the chain from link l to its root: count, smallest and largest index, and whether the walk (the link's
+// dofs last first, then its parent's, ...) visits them in strictly descending order.
+__device__ __forceinline__ void chain_span(const gsr_model& m, int l, int& n, int& lo, int& hi, bool& desc) {
+  n = 0;
+  lo = 0x7fffffff;
+  hi = -1;
+  desc = true;
+  int prev = 0x7fffffff;
+  for (int link = l; link > -1; link = m.link_parent[link]) {
+    const int ds = m.link_dof_start[link], de = m.link_dof_end[link];
+    if (de <= ds) continue;
+    desc = desc && de <= prev;
+    prev = ds;
+    n += de - ds;
+    lo = min(lo, ds);
+    hi = max(hi, de - 1);
+  }
+}
+
+// The 4 pyramid rows of a contact whose two link chains have no dof in common and whose walks descend (every chain of
+// a tree whose dofs grow from the root outwards): each row is then the two chains' dofs in ascending order, the lower
+// chain first, so each walked dof has a known slot and the rows are written in place, all four from one walk of the
+// chains, with the values the general build gives (0 + jac per entry, J qvel summed in walk order).
+__device__ inline void contact_rows_disjoint(const Env& e, int c0, int la, int lb, f3 cp, const f3 (&n)[4], int na,
+                                             int nb, int off_a, int off_b, float (&jq)[4]) {
+  const gsr_model& m = e.m;
+  const gsr_state& s = e.s;
+  for (int i = 0; i < 4; i++) jq[i] = 0.f;
+  for (int iab = 0; iab < 2; iab++) {
+    const float sign = iab == 0 ? -1.f : 1.f;
+    int slot = iab == 0 ? off_a + na - 1 : off_b + nb - 1;  // the walk fills the chain's slots from the top
+    for (int link = iab == 0 ? la : lb; link > -1; link = m.link_parent[link]) {
+      const int de = m.link_dof_end[link], nd = de - m.link_dof_start[link];
+      const f3 tpos = cp - e.L3(s.root_com, link);
+      for (int k = 0; k < nd; k++, slot--) {
+        const int d = de - 1 - k;
+        const f3 vel = qrot(e.L3(s.cdof_vel, d) - cross(tpos, e.L3(s.cdof_ang, d)), qid());
+        const float qv = e.F(s.vel, d);
+        for (int i = 0; i < 4; i++) {
+          const float jac = dot(sign * vel, n[i]);
+          jq[i] = jq[i] + jac * qv;
+          e.JI(c0 + i, slot) = d;
+          e.JV(c0 + i, slot) = 0.f + jac;
+        }
+      }
+    }
+  }
+}
+
 __device__ inline void dev_contact_rows(const Env& e, int ic, int c0) {
   const gsr_model& m = e.m;
   const gsr_state& s = e.s;
@@ -276,6 +325,32 @@ __device__ inline void dev_contact_rows(const Env& e, int ic, int c0) {
   orthogonals(cn, d1, d2);
   float invw = e.LP(m.link_invweight, 2 * la);
   if (lb > -1) invw = invw + e.LP(m.link_invweight, 2 * lb);
+  {
+    int na, nb, lo_a, hi_a, lo_b, hi_b;
+    bool desc_a, desc_b;
+    chain_span(m, la, na, lo_a, hi_a, desc_a);
+    chain_span(m, lb, nb, lo_b, hi_b, desc_b);
+    if (desc_a && desc_b && (hi_a < lo_b || hi_b < lo_a)) {
+      f3 n[4];
+      for (int i = 0; i < 4; i++) n[i] = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2) * fr - cn;
+      float jq[4];
+      const bool a_first = na == 0 || nb == 0 || hi_a < lo_b;
+      contact_rows_disjoint(e, c0, la, lb, cp, n, na, nb, a_first ? 0 : nb, a_first ? na : 0, jq);
+      for (int i = 0; i < 4; i++) {
+        const int c = c0 + i;
+        e.EI(s.jac_nnz, c) = na + nb;
+        float imp, aref;
+        imp_aref(sp, -pen, jq[i], -pen, imp, aref);
+        float diag = invw + fr * fr * invw;
+        diag *= 2.f * fr * fr * (1.f - imp) / imp;
+        diag = fmaxf(diag, m.eps);
+        e.EF(s.efc_diag, c) = diag;
+        e.EF(s.efc_aref, c) = aref;
+        e.EF(s.efc_D, c) = 1.f / diag;
+      }
+      return;
+    }
+  }
   for (int i = 0; i < 4; i++) {
     f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
     f3 n = dd * fr - cn;
