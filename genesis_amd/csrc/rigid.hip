@@ -322,7 +322,7 @@ size_t carve_scratch(const gsr_model& m, int n_envs, char* base, gsr_state* s) {
   take(s->efc_jv, R); take(s->efc_quad, 3 * R); take(s->efc_active, R);
   take(s->qacc, D); take(s->Ma, D); take(s->grad, D); take(s->Mgrad, D); take(s->search, D); take(s->mv, D);
   take(s->qfrc_constraint, D); take(s->nt_vec, D); take(s->nt_H, D * (D + 1) / 2); take(s->solver_iters, 1);
-  take(s->ent_row_ptr, E + 1); take(s->ent_row_list, 2 * R); take(s->efc_ent, R);
+  take(s->ent_row_ptr, E + 1); take(s->ent_row_list, 2 * R); take(s->efc_ent, R); take(s->ent_row_run, 2 * R);
   take(s->gjk_ws, (size_t)kGjkWsWords); take(s->gjk_defer_n, 1); take(s->gjk_defer, 2 * MB);
   take(s->qpos_next, Q); take(s->vel_next, D);
   return off;

@@ -238,6 +238,41 @@ __device__ inline void wave_min_degree(const WaveCtx& w, unsigned long long (&ad
 
 // Build the factor-space structure: entity graph, minimum-degree order with its fill, pattern, dof maps, levels and
 // LDS Hessian layout, plus the per-entity ordered row lists (model numbering).
+// Row walks of multi-entity models (J^T f, J^T D J) give each lane the dofs it writes: one entity per lane, or one
+// dof per lane when rows have at most kStage entries and the dofs fit two passes of the wavefront. A dof lane reads
+// each of its entity's rows like an entity lane but does one entry's work per row instead of the row's whole pair
+// set: measured on 20-box piles (120 dofs) the solve takes 15 % less time, on 64-box piles (384 dofs, six passes)
+// 15 % more, hence the bound.
+__device__ __forceinline__ bool wave_dof_lanes(const WaveCtx& w) {
+  return w.dense_rows == 0 && w.e.m.row_width <= kStage && w.D <= 2 * kWave;
+}
+
+// The entries of entity ea (eb) in row c as one run: first dof | position << 16 | count << 24 when they are
+// consecutive dofs at consecutive positions, else -1.
+__device__ inline void row_runs(const WaveCtx& w, int c, int ea, int eb, int& run_a, int& run_b) {
+  const Env& e = w.e;
+  const int nnz = e.EI(e.s.jac_nnz, c);
+  RowStage r;
+  stage_row(e, c, e.m.row_width, r);
+  int fa = -1, pa = 0, na = 0, fb = -1, pb = 0, nb = 0;
+  bool oka = true, okb = true;
+#pragma unroll
+  for (int k = 0; k < kStage; k++) {
+    if (k < nnz) {
+      const int d = r.d[k], x = w.L.nent[d];
+      if (x == ea) {
+        if (na == 0) { fa = d; pa = k; } else oka = oka && d == fa + na && k == pa + na;
+        na++;
+      } else if (x == eb) {
+        if (nb == 0) { fb = d; pb = k; } else okb = okb && d == fb + nb && k == pb + nb;
+        nb++;
+      }
+    }
+  }
+  run_a = oka ? (fa | pa << 16 | na << 24) : -1;
+  run_b = okb ? (fb | pb << 16 | nb << 24) : -1;
+}
+
 __device__ inline void wave_build_structure(const WaveCtx& w) {
   const Env& e = w.e;
   const gsr_model& m = e.m;
@@ -335,11 +370,16 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
     ptr[w.E] = acc;
   }
   __syncthreads();
+  int* run = s.ent_row_run + (size_t)e.b * 2 * m.max_rows;
+  const bool runs = wave_dof_lanes(w);
   for (int c0 = 0; c0 < w.n_con; c0 += kWave) {
     int c = c0 + w.lane;
     bool valid = c < w.n_con;
     int r = valid ? rent[c] : -1;
     int ea = valid ? (r & 0xffff) : -1, eb = valid ? (r >> 16) : -2;
+    // where each entity's entries sit in the row, for the dof-lane J^T f
+    int run_a = -1, run_b = -1;
+    if (runs && valid) row_runs(w, c, ea, eb, run_a, run_b);
     int rank_a = 0, rank_b = 0, cnt_a = 0, cnt_b = 0;
     for (int j = 0; j < kWave; j++) {
       int ra = __shfl(ea, j, kWave), rb = __shfl(eb, j, kWave);
@@ -352,6 +392,10 @@ __device__ inline void wave_build_structure(const WaveCtx& w) {
     if (valid) {
       lst[L.cursor[ea] + rank_a] = c;
       if (eb != ea) lst[L.cursor[eb] + rank_b] = c;
+      if (runs) {
+        run[L.cursor[ea] + rank_a] = run_a;
+        if (eb != ea) run[L.cursor[eb] + rank_b] = run_b;
+      }
     }
     __syncthreads();
     // the last lane touching an entity in this chunk advances its cursor
@@ -596,14 +640,6 @@ __device__ inline void wave_mass_mul(const WaveCtx& w, const lds_f* x, lds_f* ou
 }
 
 // func_update_constraint; returns (cost, gauss)
-// Row walks of multi-entity models (J^T f, J^T D J) give each lane the dofs it writes: one entity per lane, or one
-// dof per lane when rows have at most kStage entries and the dofs fit two passes of the wavefront. A dof lane reads
-// each of its entity's rows like an entity lane but does one entry's work per row instead of the row's whole pair
-// set: measured on 20-box piles (120 dofs) the solve takes 15 % less time, on 64-box piles (384 dofs, six passes)
-// 15 % more, hence the bound.
-__device__ __forceinline__ bool wave_dof_lanes(const WaveCtx& w) {
-  return w.dense_rows == 0 && w.e.m.row_width <= kStage && w.D <= 2 * kWave;
-}
 
 __device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, float& gauss) {
   const Env& e = w.e;
@@ -643,32 +679,44 @@ __device__ inline void wave_update_constraint(const WaveCtx& w, double& cost, fl
     // one lane per dof walks the rows of its dof's entity, two rows per iteration (both rows' loads issued before
     // either is used, the next pair's indices loaded meanwhile), and sums entry(row, d) * force in row order from
     // +0 in a register: the per-entity walk's terms in its order, so the same bits
+    const int* run = s.ent_row_run + (size_t)e.b * 2 * e.m.max_rows;
     for (int d = w.lane; d < w.D; d += kWave) {
       const int ie = L.nent[d];
       const int t_end = ptr[ie + 1];
       const int W = e.m.row_width;
       float acc = 0.f;
-      int ca_n = ptr[ie] < t_end ? lst[ptr[ie]] : 0, cb_n = ptr[ie] + 1 < t_end ? lst[ptr[ie] + 1] : 0;
-      for (int t = ptr[ie]; t < t_end; t += 2) {
-        const bool two = t + 1 < t_end;
-        const int ca = ca_n, cb = two ? cb_n : ca;
-        if (t + 2 < t_end) ca_n = lst[t + 2];
-        if (t + 3 < t_end) cb_n = lst[t + 3];
-        const float fa = e.EF(s.efc_force, ca), fb = two ? e.EF(s.efc_force, cb) : 0.f;
-        const int na = e.EI(s.jac_nnz, ca), nb = e.EI(s.jac_nnz, cb);
-        RowStage ra, rb;
-        stage_row(e, ca, W, ra);
-        stage_row(e, cb, W, rb);
-        float va = 0.f, vb = 0.f;
-        bool ha = false, hb = false;
+      // four rows per iteration: the rows' list entries and runs, then their forces and this dof's entries (one
+      // load each where the entity's entries form a run), all issued before any is summed (in row order)
+      for (int t = ptr[ie]; t < t_end; t += 4) {
+        int c[4], rn[4];
+        float f[4], v[4];
+        bool h[4];
 #pragma unroll
-        for (int k = 0; k < kStage; k++) {
-          if (k < na && ra.d[k] == d) { va = ra.v[k]; ha = true; }
-          if (k < nb && rb.d[k] == d) { vb = rb.v[k]; hb = true; }
+        for (int q = 0; q < 4; q++) {
+          c[q] = t + q < t_end ? lst[t + q] : 0;
+          rn[q] = t + q < t_end ? run[t + q] : 0;
         }
-        // a zero force adds +-0 to a sum that starts at +0 and so never holds -0: skipping a row changes no bit
-        if (ha && fa != 0.f) acc = acc + va * fa;
-        if (hb && fb != 0.f) acc = acc + vb * fb;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int f0 = rn[q] & 0xffff, p0 = (rn[q] >> 16) & 0xff, n0 = (rn[q] >> 24) & 0xff;
+          h[q] = t + q < t_end && rn[q] >= 0 && d >= f0 && d < f0 + n0;
+          f[q] = t + q < t_end ? e.EF(s.efc_force, c[q]) : 0.f;
+          v[q] = h[q] ? e.JV(c[q], p0 + (d - f0)) : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          if (t + q < t_end && rn[q] < 0) {
+            // the entity's entries are not one run: find d among the row's entries
+            const int nnz = e.EI(s.jac_nnz, c[q]);
+            RowStage r;
+            stage_row(e, c[q], W, r);
+#pragma unroll
+            for (int k = 0; k < kStage; k++)
+              if (k < nnz && r.d[k] == d) { v[q] = r.v[k]; h[q] = true; }
+          }
+          // a zero force adds +-0 to a sum that starts at +0 and so never holds -0: skipping a row changes no bit
+          if (h[q] && f[q] != 0.f) acc = acc + v[q] * f[q];
+        }
       }
       L.qfrc[d] = acc;
     }

@@ -52,8 +52,8 @@ MODEL_TABLES_F = [
 
 INT_STATE = {"is_warmstart", "ctrl_mode", "sort_ig", "first_time", "errno_env", "n_broad", "broad_pairs",
              "n_contacts", "contact_geoms", "active_buffer", "n_con", "jac_idx", "jac_nnz", "efc_active",
-             "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_defer_n", "gjk_defer", "eq_dyn_n",
-             "eq_dyn_obj"}
+             "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "ent_row_run", "gjk_defer_n", "gjk_defer",
+             "eq_dyn_n", "eq_dyn_obj"}
 
 # Entity counts above this cannot use the wavefront solver's pattern bitsets.
 WAVE_MAX_ENTITIES = 128
@@ -106,7 +106,7 @@ def state_sizes(M: RigidModel) -> dict:
         efc_force=R, efc_Jaref=R, efc_jv=R, efc_quad=3 * R, efc_active=R,
         qacc=D, Ma=D, grad=D, Mgrad=D, search=D, mv=D, qfrc_constraint=D, nt_vec=D, nt_H=D * (D + 1) // 2,
         solver_iters=1, qpos_next=Q, vel_next=D, ent_row_ptr=M.n_entities + 1, ent_row_list=2 * R, efc_ent=R,
-        gjk_ws=_lib.load().gsr_gjk_workspace_words(), gjk_defer_n=1, gjk_defer=2 * max(M.max_broad, 1),
+        ent_row_run=2 * R, gjk_ws=_lib.load().gsr_gjk_workspace_words(), gjk_defer_n=1, gjk_defer=2 * max(M.max_broad, 1),
     )
     missing = set(_lib.STATE_POINTERS) - set(sz)
     assert not missing, f"state arrays without a size rule: {missing}"
@@ -256,7 +256,8 @@ class DeviceState:
 CHECKPOINT_SKIP = frozenset({
     "n_broad", "broad_pairs", "active_buffer", "jac_val", "jac_idx", "jac_nnz", "efc_diag", "efc_aref", "efc_D",
     "efc_fl", "efc_force", "efc_Jaref", "efc_jv", "efc_quad", "efc_active", "n_con", "Ma", "grad", "Mgrad", "search",
-    "mv", "qfrc_constraint", "nt_vec", "nt_H", "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_ws",
+    "mv", "qfrc_constraint", "nt_vec", "nt_H", "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "ent_row_run",
+    "gjk_ws",
     "gjk_defer_n", "gjk_defer", "qpos_next", "vel_next",
 })
 

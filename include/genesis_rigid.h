@@ -142,6 +142,9 @@ typedef struct gsr_state {
   int32_t *solver_iters;
   int32_t *ent_row_ptr, *ent_row_list; /* per-entity ordered constraint-row lists (wavefront solver) */
   int32_t *efc_ent;                    /* entity pair of each row (wavefront solver) */
+  int32_t *ent_row_run;                /* beside each ent_row_list entry: where the entity's entries sit in that row
+                                          (first dof | position << 16 | count << 24), -1 when not one run of
+                                          consecutive dofs (wavefront solver, dof-lane row walks) */
   float *gjk_ws;                       /* GJK/EPA polytope workspace, gsr_gjk_workspace_words() per env, env-major */
   /* pairs the narrowphase handed to the GJK/EPA pass of this substep: count, then (ga << 16 | gb, contact index the
      pair's contacts are inserted at) per pair, max_broad pairs per env */
@@ -165,7 +168,7 @@ int gsr_step(const gsr_model* model, const gsr_state* state, int32_t n_substeps,
  * tight serving loops, harnesses that forbid hipMalloc): gsr_step with the per-substep scratch -- constraint rows
  * (jac_val, jac_idx, jac_nnz, efc_diag, efc_aref, efc_D, efc_fl, efc_Jaref, efc_jv, efc_quad, efc_active), the
  * Newton solver's vectors and Hessian (qacc, Ma, grad, Mgrad, search, mv, qfrc_constraint, nt_vec, nt_H,
- * solver_iters), the per-entity row lists (ent_row_ptr, ent_row_list, efc_ent), the GJK/EPA workspace and deferral
+ * solver_iters), the per-entity row lists (ent_row_ptr, ent_row_list, efc_ent, ent_row_run), the GJK/EPA workspace and deferral
  * list (gjk_ws, gjk_defer_n, gjk_defer) and the integration scratch (qpos_next, vel_next) -- taken from a static
  * device array of GSR_NOALLOC_SCRATCH_BYTES per GPU instead of the state (those state pointers are ignored and may
  * be NULL). The step allocates and frees nothing; results are those of gsr_step. Every call on one GPU shares that

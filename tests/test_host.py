@@ -151,7 +151,7 @@ def test_meaninertia_divides_inside_entity_loop():
 
 NOALLOC_SCRATCH = ["jac_val", "jac_idx", "jac_nnz", "efc_diag", "efc_aref", "efc_D", "efc_fl", "efc_Jaref", "efc_jv",
                    "efc_quad", "efc_active", "qacc", "Ma", "grad", "Mgrad", "search", "mv", "qfrc_constraint",
-                   "nt_vec", "nt_H", "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "gjk_ws",
+                   "nt_vec", "nt_H", "solver_iters", "ent_row_ptr", "ent_row_list", "efc_ent", "ent_row_run", "gjk_ws",
                    "gjk_defer_n", "gjk_defer", "qpos_next", "vel_next"]
 
 
