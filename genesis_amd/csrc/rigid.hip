@@ -651,13 +651,13 @@ int gsr_graph_capture(const void* dev_table, const gsr_batch_info* info, int32_t
   hipGraph_t graph = nullptr;
   hipError_t err = hipStreamEndCapture(stream, &graph);
   if (rc) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return rc;
   }
   if (err != hipSuccess || !graph) return GSR_E_LAUNCH;
   hipGraphExec_t exec = nullptr;
   err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
+  (void)hipGraphDestroy(graph);
   if (err != hipSuccess) return GSR_E_LAUNCH;
   *graph_exec = exec;
   return GSR_OK;
@@ -771,13 +771,13 @@ int gsr_serve_graph_capture(void* dev_state, void* dev_table, const gsr_batch_in
   hipGraph_t graph = nullptr;
   hipError_t err = hipStreamEndCapture(stream, &graph);
   if (rc) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return rc;
   }
   if (err != hipSuccess || !graph) return GSR_E_LAUNCH;
   hipGraphExec_t exec = nullptr;
   err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
+  (void)hipGraphDestroy(graph);
   if (err != hipSuccess) return GSR_E_LAUNCH;
   *graph_exec = exec;
   return GSR_OK;
